@@ -1,0 +1,183 @@
+/*
+ * gf2t.h -- C ABI of the MI355X (gfx950) GF(2) trinomial smallest-factor
+ * engine: Zech-log coarse sieve, squaring+fold Frobenius chains and the
+ * Cantor additive-FFT multiply, on one GPU.
+ *
+ * Conventions
+ *  - Polynomials over GF(2) are little-endian uint64_t word arrays: bit i
+ *    of word w is the coefficient of x^(64w+i).  A residue modulo
+ *    T = x^r + x^s + 1 has nw = ceil(r/64) words and all bits >= r zero,
+ *    on input and on output.  Valid trinomials: r >= 3, 1 <= s, 2s < r.
+ *  - Pointers named d* are device pointers owned by the caller; h* are
+ *    host pointers.  Contexts own their scratch and plans.
+ *  - Every call returns 0 on success and a negative GF2T_E* code on
+ *    failure, never exit(); gf2t_last_error() describes the last failure
+ *    of the calling thread.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Calls are asynchronous unless stated; results are ordered on the
+ *    stream.  One host thread per context.
+ *
+ * Reference citations are file:line in nclvrps/mersenne_trinomial.
+ */
+#ifndef GF2T_H
+#define GF2T_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF2T_OK 0
+#define GF2T_EINVAL (-1)   /* bad argument (reference: exit(1) paths) */
+#define GF2T_EHIP (-2)     /* HIP runtime error */
+#define GF2T_ENOMEM (-3)   /* device allocation failed */
+#define GF2T_ERANGE (-4)   /* size beyond a supported limit */
+#define GF2T_EINTERNAL (-5)/* internal consistency failure */
+
+/* ------------------------------------------------------------------ *
+ * Runtime, memory, streams, events.  The reference uses the CUDA
+ * runtime directly (CUCHK, src/gf2_cantor_cuda.h:65-69); these are the
+ * plain-pointer equivalents so a host without HIP headers can drive the
+ * library.
+ * ------------------------------------------------------------------ */
+const char *gf2t_last_error(void);
+int gf2t_version(void);
+int gf2t_init(int device);
+int gf2t_device_count(int *count);
+int gf2t_device_name(char *buf, size_t cap);
+int gf2t_malloc(void **dptr, size_t bytes);
+int gf2t_free(void *dptr);
+int gf2t_mem_info(size_t *free_bytes, size_t *total_bytes);
+int gf2t_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int gf2t_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int gf2t_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int gf2t_memset(void *dptr, int value, size_t bytes, void *stream);
+int gf2t_stream_create(void **stream);
+int gf2t_stream_destroy(void *stream);
+int gf2t_stream_sync(void *stream);
+int gf2t_device_sync(void);
+int gf2t_event_create(void **event);
+int gf2t_event_destroy(void *event);
+int gf2t_event_record(void *event, void *stream);
+int gf2t_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ------------------------------------------------------------------ *
+ * (b) Residue ring GF(2)[x]/(x^r + x^s + 1)
+ * ------------------------------------------------------------------ */
+
+/* dout = din^2 mod T in one pass (src != dst).  Replaces k_square_fused
+ * (src/tsfactor.cu:173-186) as used by TsGPU::square_slot
+ * (src/tsfactor.cu:278-300) and TrinGPU::square_A
+ * (src/gf2_trinomial_cuda.h:149-154). */
+int gf2t_square_fold(const uint64_t *din, uint64_t *dout, uint64_t r,
+                     uint64_t s, void *stream);
+
+/* dres = dres^(2^steps) mod T; dtmp is nw words of scratch.  The
+ * per-s Frobenius chain of trinomial_stage2 run (k_set_x then k x
+ * square_A, src/trinomial_stage2.cu:228-235). */
+int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r,
+                           uint64_t s, uint64_t steps, void *stream);
+
+/* Batched Frobenius chains: residue i (r[i], s[i], nw_i words in
+ * dres[i], scratch dtmp[i]) is squared steps[i] times; steps may differ
+ * (uneven frontiers) and r may differ between residues.  Results end in
+ * dres[i].  All arrays are host arrays of n entries.  No reference
+ * counterpart: it batches the per-s loop of src/trinomial_scan.cu:364-
+ * 366 / src/tsfactor.cu:294-300 across s values. */
+int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
+                           const uint64_t *r, const uint64_t *s,
+                           const uint64_t *steps, int n, void *stream);
+
+/* Fold a product of src_bits <= 2r-1 bits (dsrc, words(src_bits) words)
+ * into dout (nw words) modulo T.  Replaces TrinGPU::reduce / TsGPU::
+ * reduce_into (two k_fold_pass launches, src/gf2_trinomial_cuda.h:63-74,
+ * :136-146; src/tsfactor.cu:323-331). */
+int gf2t_fold(const uint64_t *dsrc, uint64_t src_bits, uint64_t *dout,
+              uint64_t r, uint64_t s, void *stream);
+
+/* dst ^= x*src mod T (k_mulx_xor, src/tsfactor.cu:109-126). */
+int gf2t_mulx_xor(uint64_t *ddst, const uint64_t *dsrc, uint64_t r,
+                  uint64_t s, void *stream);
+
+/* p[0..nw) = x (k_set_x), = 1 (k_set_one), p ^= x (k_xor_x)
+ * (src/gf2_trinomial_cuda.h:76-84, src/tsfactor.cu:198-202). */
+int gf2t_set_x(uint64_t *dp, size_t nw, void *stream);
+int gf2t_set_one(uint64_t *dp, size_t nw, void *stream);
+int gf2t_xor_x(uint64_t *dp, void *stream);
+
+/* ------------------------------------------------------------------ *
+ * (c) Cantor additive FFT over GF(2^64) and GF(2)[x] products
+ * ------------------------------------------------------------------ */
+
+/* Cantor basis b[0]=1, b[i]^2+b[i]=b[i-1]; returns the chain length
+ * (64) in *len (CantorBasis::build/verify, src/gf2_cantor_engine.h:74-
+ * 121). */
+int gf2t_cantor_basis(uint64_t *hbasis, int *len);
+
+typedef struct gf2t_fft_plan gf2t_fft_plan;
+/* Plan for 2^m points, 1 <= m <= 27 (CudaFFT::init, src/gf2_cantor_cuda.
+ * h:365-375; the plan computes twiddles on the fly, no table). */
+int gf2t_fft_plan_create(int m, gf2t_fft_plan **plan);
+int gf2t_fft_plan_destroy(gf2t_fft_plan *plan);
+/* number of HBM passes of one transform, for reporting */
+int gf2t_fft_plan_passes(const gf2t_fft_plan *plan, int *fwd_passes,
+                         int *inv_passes);
+/* In-place forward / inverse transform of 2^m elements at df; output is
+ * bit-identical to CantorFFT::fwd/inv (src/gf2_cantor_engine.h:146-187)
+ * and CudaFFT::fwd/inv (src/gf2_cantor_cuda.h:497-520). */
+int gf2t_fft_fwd(const gf2t_fft_plan *plan, uint64_t *df, void *stream);
+int gf2t_fft_inv(const gf2t_fft_plan *plan, uint64_t *df, void *stream);
+
+/* Number of output words of a product: words(abits+bbits)+1
+ * (mul_out_words, src/gf2_cantor_engine.h:201-203). */
+size_t gf2t_mul_out_words(uint64_t abits, uint64_t bbits);
+
+/* Workspace (bytes) that gf2t_mul / gf2t_mulmod need for these sizes. */
+size_t gf2t_mul_workspace_bytes(uint64_t abits, uint64_t bbits);
+
+/* dout = a*b over GF(2)[x] (gf2t_mul_out_words words, fully written);
+ * dwork is caller-owned scratch of gf2t_mul_workspace_bytes bytes.
+ * Replaces gf2x_fft_mul_buf (src/gf2_cantor_engine.h:207-229) and
+ * TrinGPU::mul_dev / TsGPU::mul_full (src/gf2_trinomial_cuda.h:157-169,
+ * src/tsfactor.cu:309-321). */
+int gf2t_mul(const uint64_t *da, uint64_t abits, const uint64_t *db,
+             uint64_t bbits, uint64_t *dout, void *dwork, void *stream);
+
+/* dout (nw words) = a*b mod x^r + x^s + 1 with abits + bbits - 1 <=
+ * 2r - 1.  Replaces TrinGPU::accumulate's mul_dev+reduce and
+ * TsGPU::mul_A_by_C (src/gf2_trinomial_cuda.h:172-177,
+ * src/tsfactor.cu:333-336).  dout may alias da or db. */
+int gf2t_mulmod(const uint64_t *da, uint64_t abits, const uint64_t *db,
+                uint64_t bbits, uint64_t r, uint64_t s, uint64_t *dout,
+                void *dwork, void *stream);
+
+/* ------------------------------------------------------------------ *
+ * (a) Zech-log coarse sieve
+ * ------------------------------------------------------------------ */
+typedef struct gf2t_sieve_stats {
+    int d;                    /* degree */
+    uint64_t q;               /* primitive modulus used */
+    uint64_t classes;         /* cyclotomic classes (= irreducibles of degree d) */
+    uint64_t marking;         /* classes whose progression reaches s <= smax */
+    uint64_t heavy;           /* progressions deferred to the heavy pass */
+    float ms_table, ms_classes, ms_heavy;
+} gf2t_sieve_stats;
+
+/* Mark every s in [1, smax] divisible by an irreducible of degree d in
+ * [dmin, dmax] (2 <= dmin <= dmax <= 32) into dbest (smax+1 uint64 on
+ * the device, atomic min of (d<<48)|mask; initialise to ~0 or a prior
+ * image).  Replaces run_gpu/sieve_degree_gpu (src/coarse_sieve.cu:852-
+ * 918, :1119-1225).  stats (host, dmax+1 entries, may be NULL) gets one
+ * record per degree.  Synchronous. */
+int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax,
+               uint64_t *dbest, gf2t_sieve_stats *stats);
+
+/* Smallest primitive polynomial of degree d (find_primitive_poly,
+ * src/gf2_field_setup.h:49-57); 0 if d is out of range. */
+uint64_t gf2t_primitive_poly(int d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GF2T_H */

@@ -1,0 +1,60 @@
+"""Build the gfx950 engine library (libgf2t.so) in-tree with hipcc.
+
+Every HIP source is compiled for gfx950 only, with packed-FP32 generation
+disabled, and linked into ``mersenne_trinomial_amd/libgf2t.so``.  The C++
+command-line drivers under ``cli/`` are linked against it.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libgf2t.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+SOURCES = ["capi.hip", "square.hip", "fft.hip", "sieve.hip"]
+HOST_SOURCES = ["host.cpp"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+          "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+            "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + p.stdout + p.stderr)
+        raise RuntimeError(f"compile failed: {cmd[-1]}")
+    return p
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = []
+    for src in SOURCES:
+        obj = os.path.join(OBJ, src + ".o")
+        jobs.append([HIPCC, *COMMON, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HOST_SOURCES:
+        path = os.path.join(CSRC, src)
+        if os.path.exists(path):
+            obj = os.path.join(OBJ, src + ".o")
+            jobs.append([HIPCC, *COMMON, "-c", path, "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        list(ex.map(_run, jobs))
+    objs = [j[-1] for j in jobs]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,187 @@
+// capi.hip -- runtime plumbing of the C ABI: errors, device selection,
+// memory, streams and events.  Kernels live in square.hip, fft.hip and
+// sieve.hip.
+#include "gf2t_internal.hpp"
+#include "../../include/gf2t.h"
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+namespace gf2t {
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    set_error("HIP error '%s' in %s (%s:%d)", hipGetErrorString(e), what, file, line);
+    return e == hipErrorOutOfMemory ? GF2T_ENOMEM : GF2T_EHIP;
+}
+}  // namespace gf2t
+
+extern "C" const char *gf2t_last_error(void) { return gf2t::g_err; }
+
+extern "C" int gf2t_version(void) { return 1; }
+
+extern "C" int gf2t_device_count(int *count) {
+    if (!count) return GF2T_EINVAL;
+    GF2T_HIP(hipGetDeviceCount(count));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_init(int device) {
+    int n = 0;
+    GF2T_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        gf2t::set_error("gf2t_init: device %d of %d", device, n);
+        return GF2T_EINVAL;
+    }
+    GF2T_HIP(hipSetDevice(device));
+    GF2T_HIP(hipFree(nullptr));   // create the context now
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_device_name(char *buf, size_t cap) {
+    if (!buf || !cap) return GF2T_EINVAL;
+    int dev = 0;
+    GF2T_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    GF2T_HIP(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, cap, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_malloc(void **dptr, size_t bytes) {
+    if (!dptr) return GF2T_EINVAL;
+    *dptr = nullptr;
+    if (!bytes) bytes = 8;
+    GF2T_HIP(hipMalloc(dptr, bytes));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_free(void *dptr) {
+    if (dptr) GF2T_HIP(hipFree(dptr));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_mem_info(size_t *free_bytes, size_t *total_bytes) {
+    size_t f = 0, t = 0;
+    GF2T_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return GF2T_OK;
+    if (stream) {
+        GF2T_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+        GF2T_HIP(hipStreamSynchronize((hipStream_t)stream));
+    } else {
+        GF2T_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    }
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return GF2T_OK;
+    if (stream) {
+        GF2T_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+        GF2T_HIP(hipStreamSynchronize((hipStream_t)stream));
+    } else {
+        GF2T_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    }
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return GF2T_OK;
+    GF2T_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return GF2T_OK;
+}
+
+namespace {
+__global__ void k_fill_bytes(unsigned char *p, size_t n, unsigned char v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+__global__ void k_fill_u64(u64 *p, size_t n, u64 v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+}  // namespace
+
+// Fill by kernel rather than hipMemsetAsync so the call is safe inside a
+// captured HIP graph.
+extern "C" int gf2t_memset(void *dptr, int value, size_t bytes, void *stream) {
+    if (!bytes) return GF2T_OK;
+    if (!dptr) return GF2T_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char v = (unsigned char)value;
+    if (((uintptr_t)dptr & 7) == 0 && (bytes & 7) == 0) {
+        u64 w = 0x0101010101010101ULL * v;
+        size_t n = bytes / 8;
+        unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, st, (u64 *)dptr, n, w);
+    } else {
+        unsigned blocks = (unsigned)std::min<size_t>((bytes + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_fill_bytes, dim3(blocks), dim3(256), 0, st, (unsigned char *)dptr,
+                           bytes, v);
+    }
+    GF2T_CHECK_LAUNCH("gf2t_memset");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_stream_create(void **stream) {
+    if (!stream) return GF2T_EINVAL;
+    hipStream_t s;
+    GF2T_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void *)s;
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_stream_destroy(void *stream) {
+    if (stream) GF2T_HIP(hipStreamDestroy((hipStream_t)stream));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_stream_sync(void *stream) {
+    GF2T_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_device_sync(void) {
+    GF2T_HIP(hipDeviceSynchronize());
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_event_create(void **event) {
+    if (!event) return GF2T_EINVAL;
+    hipEvent_t e;
+    GF2T_HIP(hipEventCreate(&e));
+    *event = (void *)e;
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_event_destroy(void *event) {
+    if (event) GF2T_HIP(hipEventDestroy((hipEvent_t)event));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_event_record(void *event, void *stream) {
+    GF2T_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_event_elapsed_ms(void *start, void *stop, float *ms) {
+    if (!ms) return GF2T_EINVAL;
+    GF2T_HIP(hipEventSynchronize((hipEvent_t)stop));
+    GF2T_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return GF2T_OK;
+}

@@ -1,0 +1,455 @@
+// sieve.hip -- (a) Zech-log coarse sieve on gfx950.
+//
+// For each degree d: GF(2^d) = GF(2)[x]/q with q primitive, so g = x has
+// order M = 2^d - 1.  An irreducible P of degree d with root beta = g^j
+// divides x^r + x^s + 1 iff beta^r + beta^s = 1, i.e. j*s == Z(u) (mod M)
+// with u = j*r mod M and Z the Zech logarithm Z(u) = log(g^u + 1): an
+// arithmetic progression in s of modulus n = M / gcd(j, M).  One root per
+// conjugacy class suffices: the class representative is the j that is
+// strictly smaller than all its other rotations j*2^i mod M (a binary
+// Lyndon word of length d), which also excludes every element of a
+// proper subfield.  Each progression is marked into best[s] with an
+// atomic min of (d << 48) | mask, i.e. smallest degree, then numerically
+// least mask (reference: src/sieve_ref.cpp:1-30, src/coarse_sieve.cu:
+// 237-299).
+//
+// Per degree, on the device:
+//   k_log_build   L[g^i] = i for all i < M (LFSR chain per lane, random
+//                 4-byte scatter: the HBM-bound part, 4*2^d bytes)
+//   k_lyndon      one lane per candidate j, 64-lane ballot compaction of
+//                 the class representatives into a dense list
+//   k_resolve     one lane per class: Zech log (one random gather), the
+//                 progression, and -- only if it reaches s <= smax -- the
+//                 minimal polynomial and the marks (inline up to 1024,
+//                 else queued for k_mark_heavy)
+//   k_mark_heavy  many blocks per heavy progression
+// The dense tables fit HBM for every d <= 32 (16 GiB at d = 32), so no
+// bucketed low-memory mode is needed on a 288 GB part.
+#include "gf2t_internal.hpp"
+#include "../../include/gf2t.h"
+#include <vector>
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+constexpr u64 INLINE_MARKS = 1024;
+constexpr u32 HEAVY_CAP = 1u << 22;
+constexpr int LOG_CHUNK = 64;
+
+struct Heavy {
+    u64 s_first, n, key;
+};
+
+// ------------------------------------------------------------ host field
+u64 hclmul(u64 a, u64 b, u64 &hi) {
+    u64 lo = 0;
+    hi = 0;
+    for (int i = 0; i < 64; i++)
+        if ((b >> i) & 1) {
+            lo ^= a << i;
+            if (i) hi ^= a >> (64 - i);
+        }
+    return lo;
+}
+u64 hmulmod(u64 a, u64 b, u64 q, int d) {
+    u64 hi, lo = hclmul(a, b, hi);
+    for (int bit = 2 * d - 2; bit >= d; bit--) {
+        u64 have = bit >= 64 ? (hi >> (bit - 64)) & 1 : (lo >> bit) & 1;
+        if (!have) continue;
+        int sh = bit - d;
+        lo ^= q << sh;
+        if (sh) hi ^= q >> (64 - sh);
+    }
+    return lo;
+}
+u64 hpgcd(u64 a, u64 b) {
+    while (b) {
+        int db = 63 - __builtin_clzll(b);
+        while (a && 63 - __builtin_clzll(a) >= db) a ^= b << ((63 - __builtin_clzll(a)) - db);
+        std::swap(a, b);
+    }
+    return a;
+}
+int hprimes(u64 m, u64 *ps) {
+    int n = 0;
+    for (u64 p = 2; p * p <= m; p += (p == 2 ? 1 : 2))
+        if (m % p == 0) { ps[n++] = p; while (m % p == 0) m /= p; }
+    if (m > 1) ps[n++] = m;
+    return n;
+}
+// Rabin: x^(2^d) == x and gcd(x^(2^(d/p)) - x, q) = 1 for primes p | d
+bool hirreducible(u64 q, int d) {
+    u64 chain[65];
+    u64 t = 2;
+    chain[0] = t;
+    for (int i = 1; i <= d; i++) { t = hmulmod(t, t, q, d); chain[i] = t; }
+    if (chain[d] != 2) return false;
+    u64 ps[16];
+    int np = hprimes((u64)d, ps);
+    for (int i = 0; i < np; i++)
+        if (hpgcd(q, chain[d / ps[i]] ^ 2) != 1) return false;
+    return true;
+}
+u64 hpow(u64 b, u64 e, u64 q, int d) {
+    u64 r = 1;
+    while (e) {
+        if (e & 1) r = hmulmod(r, b, q, d);
+        b = hmulmod(b, b, q, d);
+        e >>= 1;
+    }
+    return r;
+}
+u64 primitive_poly(int d) {
+    u64 M = ((u64)1 << d) - 1, ps[16];
+    int np = hprimes(M, ps);
+    for (u64 q = ((u64)1 << d) | 1; q < ((u64)2 << d); q += 2) {
+        if (!hirreducible(q, d)) continue;
+        bool prim = true;
+        for (int i = 0; i < np && prim; i++)
+            if (hpow(2, M / ps[i], q, d) == 1) prim = false;
+        if (prim) return q;
+    }
+    return 0;
+}
+u64 gcd_host(u64 a, u64 b) {
+    while (b) { u64 t = a % b; a = b; b = t; }
+    return a;
+}
+int64_t mobius(u64 n) {
+    int64_t mu = 1;
+    for (u64 p = 2; p * p <= n; p++) {
+        if (n % p) continue;
+        n /= p;
+        if (n % p == 0) return 0;
+        mu = -mu;
+    }
+    if (n > 1) mu = -mu;
+    return mu;
+}
+u64 irreducible_count(int d) {
+    int64_t acc = 0;
+    for (int k = 1; k <= d; k++)
+        if (d % k == 0) acc += mobius((u64)k) * ((int64_t)1 << (d / k));
+    return (u64)(acc / d);
+}
+
+// ---------------------------------------------------------- device side
+struct DegreeParams {
+    int d;
+    u64 M, q, rmod, smax;
+    const u32 *Alo, *Ahi;    // g^k and g^(k<<16), k < 2^16
+};
+
+GF2T_DEV u32 antilog(const DegreeParams &P, u64 e) {
+    if (P.d <= 16) return P.Alo[e];
+    return fmul(P.Ahi[e >> 16], P.Alo[e & 0xFFFF], P.q, P.d);
+}
+
+GF2T_DEV u64 mod_mersenne(u64 x, int d, u64 M) {
+    while (x > M) x = (x & M) + (x >> d);
+    return x == M ? 0 : x;
+}
+
+__global__ void k_log_build(DegreeParams P, u32 *__restrict__ L) {
+    u64 chunk = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 i0 = chunk * LOG_CHUNK;
+    if (i0 >= P.M) return;
+    u32 a = antilog(P, i0);
+    const u32 top = 1u << (P.d - 1);
+    // (u32)q keeps the x^d term for d < 32 (it cancels the shifted-out
+    // bit) and drops it for d = 32 (the shift already discarded it)
+    const u32 qlow = (u32)P.q;
+    u64 iend = i0 + LOG_CHUNK < P.M ? i0 + LOG_CHUNK : P.M;
+    for (u64 i = i0; i < iend; i++) {
+        L[a] = (u32)i;
+        a = (a << 1) ^ ((a & top) ? qlow : 0u);
+    }
+}
+
+// strict minimum of its rotations within d bits
+GF2T_DEV bool is_lyndon(u64 j, int d, u64 M) {
+    u64 jj = j;
+    for (int i = 1; i < d; i++) {
+        jj = ((jj << 1) | (jj >> (d - 1))) & M;
+        if (jj <= j) return false;
+    }
+    return true;
+}
+
+__global__ void k_lyndon(int d, u64 M, u32 *__restrict__ surv, unsigned long long *__restrict__ nsurv,
+                         u64 cap) {
+    const int lane = threadIdx.x & 63;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    // wave-uniform loop: every lane of a wave takes part in each ballot
+    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < M;
+         wbase += stride) {
+        u64 j = wbase + lane;
+        bool keep = j >= 1 && j < M && is_lyndon(j, d, M);
+        // 64-lane ballot compaction: one atomic per wave
+        unsigned long long m = __ballot(keep);
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(nsurv, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (keep) {
+            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
+            if (pos < cap) surv[pos] = (u32)j;
+        }
+    }
+}
+
+GF2T_DEV u64 gcd_dev(u64 a, u64 b) {
+    while (b) { u64 t = a % b; a = b; b = t; }
+    return a;
+}
+GF2T_DEV u64 inv_dev(u64 a, u64 m) {
+    int64_t t = 0, nt = 1, rr = (int64_t)m, nr = (int64_t)(a % m);
+    while (nr) {
+        int64_t qq = rr / nr, tmp;
+        tmp = t - qq * nt; t = nt; nt = tmp;
+        tmp = rr - qq * nr; rr = nr; nr = tmp;
+    }
+    if (t < 0) t += (int64_t)m;
+    return (u64)t;
+}
+
+// minimal polynomial prod_{i<d} (X + beta^(2^i)), coefficients in GF(2)
+template <int D>
+GF2T_DEV u64 minpoly_mask(u32 beta, u64 q) {
+    // P <- P * (X + root) over all D conjugates; the update runs over
+    // every k (entries above the current degree are 0 and stay 0) so
+    // that P[] has static indices and lives in registers
+    u32 P[D + 1];
+#pragma unroll
+    for (int k = 0; k <= D; k++) P[k] = 0;
+    P[0] = 1;
+    u32 root = beta;
+#pragma unroll 1
+    for (int i = 0; i < D; i++) {
+#pragma unroll
+        for (int k = D; k >= 1; k--) P[k] = P[k - 1] ^ fmul(root, P[k], q, D);
+        P[0] = fmul(root, P[0], q, D);
+        root = fsqr(root, q, D);
+    }
+    u64 mask = 0;
+#pragma unroll
+    for (int k = 0; k <= D; k++) {
+        if (P[k] > 1) return 0;
+        mask |= (u64)P[k] << k;
+    }
+    return mask;
+}
+
+template <int D>
+__global__ void k_resolve(DegreeParams P, const u32 *__restrict__ surv, u64 nsurv,
+                          const u32 *__restrict__ L, unsigned long long *__restrict__ best,
+                          Heavy *__restrict__ heavy, unsigned *__restrict__ nheavy,
+                          unsigned long long *__restrict__ nmark, unsigned *__restrict__ bad) {
+    u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsurv) return;
+    const u64 M = P.M;
+    u64 j = surv[t];
+    u64 u = mod_mersenne(j * P.rmod, D, M);
+    if (u == 0) return;                           // beta^r = 1: never a root
+    u64 v = L[antilog(P, u) ^ 1u];                // Zech logarithm of u
+    u64 gg = gcd_dev(j, M);
+    if (v % gg) return;
+    u64 n = M / gg;
+    u64 s0 = (inv_dev(j / gg, n) * (v / gg)) % n;
+    u64 sf = s0 ? s0 : n;
+    if (sf > P.smax) return;
+    atomicAdd(nmark, 1ULL);
+    u64 mask = minpoly_mask<D>(antilog(P, j), P.q);
+    if (mask == 0 || !((mask >> D) & 1) || !(mask & 1)) { atomicAdd(bad, 1u); return; }
+    unsigned long long key = ((unsigned long long)D << 48) | mask;
+    u64 count = (P.smax - sf) / n + 1;
+    if (count <= INLINE_MARKS) {
+        for (u64 s = sf; s <= P.smax; s += n)
+            if (best[s] > key) atomicMin(&best[s], key);
+    } else {
+        unsigned idx = atomicAdd(nheavy, 1u);
+        if (idx < HEAVY_CAP) heavy[idx] = Heavy{sf, n, key};
+        else atomicAdd(bad, 2u);
+    }
+}
+
+__global__ void k_mark_heavy(const Heavy *__restrict__ heavy, unsigned nheavy, u64 smax,
+                             unsigned long long *__restrict__ best) {
+    for (unsigned rec = blockIdx.y; rec < nheavy; rec += gridDim.y) {
+        const Heavy h = heavy[rec];
+        u64 count = (smax - h.s_first) / h.n + 1;
+        for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+             k += (u64)gridDim.x * blockDim.x) {
+            u64 s = h.s_first + k * h.n;
+            if (best[s] > h.key) atomicMin(&best[s], (unsigned long long)h.key);
+        }
+    }
+}
+
+__global__ void k_zero_counters(unsigned long long *a, unsigned *b) {
+    if (threadIdx.x < 4) a[threadIdx.x] = 0;
+    if (threadIdx.x < 4) b[threadIdx.x] = 0;
+}
+
+
+template <int D>
+void launch_resolve(DegreeParams P, const u32 *surv, u64 nsurv, const u32 *L,
+                    unsigned long long *best, Heavy *heavy, unsigned *nheavy,
+                    unsigned long long *nmark, unsigned *bad) {
+    unsigned blocks = (unsigned)((nsurv + 255) / 256);
+    hipLaunchKernelGGL(k_resolve<D>, dim3(blocks), dim3(256), 0, 0, P, surv, nsurv, L, best, heavy,
+                       nheavy, nmark, bad);
+}
+
+typedef void (*ResolveFn)(DegreeParams, const u32 *, u64, const u32 *, unsigned long long *,
+                          Heavy *, unsigned *, unsigned long long *, unsigned *);
+
+template <int D>
+struct ResolveTable {
+    static void fill(ResolveFn *tab) {
+        tab[D] = &launch_resolve<D>;
+        ResolveTable<D - 1>::fill(tab);
+    }
+};
+template <>
+struct ResolveTable<1> {
+    static void fill(ResolveFn *) {}
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int ensure(size_t b) {
+        if (bytes >= b) return GF2T_OK;
+        if (p) GF2T_HIP(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        GF2T_HIP(hipMalloc(&p, b));
+        bytes = b;
+        return GF2T_OK;
+    }
+};
+
+}  // namespace
+
+extern "C" uint64_t gf2t_primitive_poly(int d) {
+    if (d < 2 || d > 32) return 0;
+    return primitive_poly(d);
+}
+
+extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
+                          gf2t_sieve_stats *stats) {
+    if (dmin < 2) dmin = 2;
+    if (dmax > 32 || dmin > dmax || !dbest || smax < 1) {
+        gf2t::set_error("gf2t_sieve: need 2 <= dmin <= dmax <= 32, smax >= 1, dbest");
+        return GF2T_EINVAL;
+    }
+    ResolveFn resolve_tab[33] = {};
+    ResolveTable<32>::fill(resolve_tab);
+
+    DevBuf bL, bSurv, bHeavy, bTab, bCnt;
+    int rc;
+    if ((rc = bL.ensure(((size_t)1 << dmax) * sizeof(u32)))) return rc;
+    if ((rc = bTab.ensure(2 * 65536 * sizeof(u32)))) return rc;
+    if ((rc = bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
+    if ((rc = bCnt.ensure(64))) return rc;
+    unsigned long long *cnt64 = (unsigned long long *)bCnt.p;   // [0] nsurv [1] nmark
+    unsigned *cnt32 = (unsigned *)((char *)bCnt.p + 32);         // [0] nheavy [1] bad
+    u32 *Alo = (u32 *)bTab.p, *Ahi = Alo + 65536;
+    std::vector<u32> hlo(65536), hhi(65536);
+    hipEvent_t ev[4];
+    for (auto &e : ev) GF2T_HIP(hipEventCreate(&e));
+
+    int status = GF2T_OK;
+    for (int d = dmin; d <= dmax && status == GF2T_OK; d++) {
+        const u64 M = ((u64)1 << d) - 1;
+        const u64 q = primitive_poly(d);
+        const u64 rmod = r % M;
+        if (!q || gcd_host(rmod ? rmod : M, M) != 1) {
+            gf2t::set_error("gf2t_sieve: d=%d: gcd(r, 2^d-1) != 1, unsupported r", d);
+            status = GF2T_EINVAL;
+            break;
+        }
+        // g^k and g^(k<<16) tables on the host (LFSR steps)
+        u64 a = 1;
+        for (u64 k = 0; k < 65536; k++) {
+            hlo[k] = (u32)a;
+            a <<= 1;
+            if ((a >> d) & 1) a ^= q;
+        }
+        u64 g16 = a, b = 1;   // a = g^65536
+        for (u64 k = 0; k < 65536; k++) {
+            hhi[k] = (u32)b;
+            b = hmulmod(b, g16, q, d);
+        }
+        GF2T_HIP(hipMemcpy(Alo, hlo.data(), 65536 * 4, hipMemcpyHostToDevice));
+        GF2T_HIP(hipMemcpy(Ahi, hhi.data(), 65536 * 4, hipMemcpyHostToDevice));
+        DegreeParams P{d, M, q, rmod, smax, Alo, Ahi};
+
+        const u64 ncls = irreducible_count(d);
+        if ((rc = bSurv.ensure((ncls + 64) * sizeof(u32)))) { status = rc; break; }
+        u32 *surv = (u32 *)bSurv.p;
+        u32 *L = (u32 *)bL.p;
+        Heavy *heavy = (Heavy *)bHeavy.p;
+
+        GF2T_HIP(hipEventRecord(ev[0], 0));
+        u64 nchunks = (M + LOG_CHUNK - 1) / LOG_CHUNK;
+        hipLaunchKernelGGL(k_log_build, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, 0,
+                           P, L);
+        GF2T_CHECK_LAUNCH("k_log_build");
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
+        GF2T_HIP(hipEventRecord(ev[1], 0));
+        u64 lyn_blocks = std::min<u64>((M + 255) / 256, 1u << 20);
+        hipLaunchKernelGGL(k_lyndon, dim3((unsigned)lyn_blocks), dim3(256), 0, 0, d, M, surv,
+                           cnt64, (u64)(ncls + 64));
+        GF2T_CHECK_LAUNCH("k_lyndon");
+        unsigned long long hs[2];
+        GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
+        u64 nsurv = hs[0];
+        if (nsurv != ncls) {
+            gf2t::set_error("gf2t_sieve: d=%d: %llu classes, expected %llu", d,
+                            (unsigned long long)nsurv, (unsigned long long)ncls);
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        if (nsurv) {
+            resolve_tab[d](P, surv, nsurv, L, (unsigned long long *)dbest, heavy, cnt32, cnt64 + 1,
+                           cnt32 + 1);
+            GF2T_CHECK_LAUNCH("k_resolve");
+        }
+        GF2T_HIP(hipEventRecord(ev[2], 0));
+        unsigned h32[2];
+        GF2T_HIP(hipMemcpy(h32, cnt32, 8, hipMemcpyDeviceToHost));
+        GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
+        if (h32[1]) {
+            gf2t::set_error("gf2t_sieve: d=%d: %s", d,
+                            (h32[1] & 1) ? "minimal polynomial not over GF(2)"
+                                         : "heavy queue overflow");
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        unsigned nheavy = h32[0];
+        if (nheavy) {
+            unsigned gy = nheavy < 65535 ? nheavy : 65535;
+            hipLaunchKernelGGL(k_mark_heavy, dim3(64, gy), dim3(256), 0, 0, heavy, nheavy, smax,
+                               (unsigned long long *)dbest);
+            GF2T_CHECK_LAUNCH("k_mark_heavy");
+        }
+        GF2T_HIP(hipEventRecord(ev[3], 0));
+        GF2T_HIP(hipEventSynchronize(ev[3]));
+        if (stats) {
+            gf2t_sieve_stats &S = stats[d];
+            S.d = d;
+            S.q = q;
+            S.classes = nsurv;
+            S.marking = hs[1];
+            S.heavy = nheavy;
+            GF2T_HIP(hipEventElapsedTime(&S.ms_table, ev[0], ev[1]));
+            GF2T_HIP(hipEventElapsedTime(&S.ms_classes, ev[1], ev[2]));
+            GF2T_HIP(hipEventElapsedTime(&S.ms_heavy, ev[2], ev[3]));
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return status;
+}

@@ -1,0 +1,294 @@
+// square.hip -- (b) squaring + trinomial fold on gfx950.
+//
+// F <- F^2 mod T, T = x^r + x^s + 1, 1 <= s, 2s < r.  Squaring over GF(2)
+// is the bit spread SP (SP[2k] = F bit k, SP[odd] = 0); the fold
+// x^e -> x^(e-r) + x^(e-r+s) applied twice reduces SP (degree <= 2r-2)
+// to r bits.  Writing SP = L + x^r H, H = H_lo + x^(r-s) H_hi:
+//   F^2 mod T = SP[0,r) + SP[r,2r-1)>>r + SP[r,2r-s)>>(r-s)
+//             + SP[2r-s,2r-1)>>(2r-s) + SP[2r-s,2r-1)>>(2r-2s)
+// (the same five windows as the reference's fused kernel,
+// src/tsfactor.cu:128-186).  Each output word is written by exactly one
+// lane (gather form), so there are no write races.
+//
+// Mapping: one lane produces one 128-bit output pair (two words) from
+// 64-bit input windows; waves cover 64 consecutive pairs = 1 KiB of
+// output, so stores and the dominant loads are fully coalesced.  Per
+// residue-step the kernel moves 16*nw algorithmic bytes (read F, write
+// F^2); the second read of the upper input half by window 3 hits in
+// L2 / Infinity Cache.
+#include "gf2t_internal.hpp"
+#include "../../include/gf2t.h"
+#include <vector>
+#include <algorithm>
+#include <numeric>
+
+namespace {
+
+constexpr int SQ_THREADS = 256;
+
+// 128-bit window of the virtual spread of `in` at SP position P, keeping
+// only SP indices in [lo, hi).
+GF2T_DEV void vspread128(const u64 *__restrict__ in, size_t nw, u64 P, u64 lo,
+                         u64 hi, u64 &w0, u64 &w1) {
+    if (P >= hi || P + 128 <= lo) { w0 = 0; w1 = 0; return; }
+    u64 v = win64(in, nw, (P + 1) >> 1);
+    u64 a = spread32((u32)v), b = spread32((u32)(v >> 32));
+    if (P & 1) { a <<= 1; b <<= 1; }
+    int lc = lo > P ? (int)(lo - P) : 0;
+    int hc = (hi - P) >= 128 ? 128 : (int)(hi - P);
+    w0 = a & range_mask64(lc, hc);
+    w1 = b & range_mask64(lc - 64, hc - 64);
+}
+
+GF2T_DEV void square_pair(const u64 *__restrict__ in, u64 *__restrict__ out,
+                          size_t nw, u64 r, u64 s, size_t i) {
+    const u64 b = (u64)i << 7;
+    u64 o0, o1, t0, t1;
+    vspread128(in, nw, b, 0, r, o0, o1);
+    vspread128(in, nw, b + r, r, 2 * r - 1, t0, t1);
+    o0 ^= t0; o1 ^= t1;
+    vspread128(in, nw, b + r - s, r, 2 * r - s, t0, t1);
+    o0 ^= t0; o1 ^= t1;
+    vspread128(in, nw, b + 2 * r - s, 2 * r - s, 2 * r - 1, t0, t1);
+    o0 ^= t0; o1 ^= t1;
+    vspread128(in, nw, b + 2 * r - 2 * s, 2 * r - s, 2 * r - 1, t0, t1);
+    o0 ^= t0; o1 ^= t1;
+    out[2 * i] = o0;
+    if (2 * i + 1 < nw) out[2 * i + 1] = o1;
+}
+
+__global__ __launch_bounds__(SQ_THREADS) void k_square_fold(
+        const u64 *__restrict__ in, u64 *__restrict__ out, size_t nw, u64 r,
+        u64 s) {
+    size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
+    if (i >= (nw + 1) / 2) return;
+    square_pair(in, out, nw, r, s, i);
+}
+
+struct SqDesc {
+    const u64 *in;
+    u64 *out;
+    u64 r, s;
+    u64 nw;
+};
+
+// batched: blockIdx.y selects the residue, blockIdx.x the 512-word tile
+__global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
+        const SqDesc *__restrict__ desc) {
+    const SqDesc d = desc[blockIdx.y];
+    size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
+    if (i >= (d.nw + 1) / 2) return;
+    square_pair(d.in, d.out, d.nw, d.r, d.s, i);
+}
+
+// generic fold of a product P (L <= 2r-1 bits) into r bits
+GF2T_DEV u64 pwin(const u64 *__restrict__ p, size_t pw, u64 pos, u64 lo, u64 hi) {
+    if (pos >= hi || pos + 64 <= lo) return 0;
+    u64 v = win64(p, pw, pos);
+    int lc = lo > pos ? (int)(lo - pos) : 0;
+    int hc = (hi - pos) >= 64 ? 64 : (int)(hi - pos);
+    return v & range_mask64(lc, hc);
+}
+
+__global__ __launch_bounds__(SQ_THREADS) void k_fold(
+        const u64 *__restrict__ p, size_t pw, u64 L, u64 *__restrict__ out,
+        size_t nw, u64 r, u64 s) {
+    size_t w = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
+    if (w >= nw) return;
+    u64 b = (u64)w << 6;
+    u64 lim1 = L < r ? L : r;
+    u64 v = pwin(p, pw, b, 0, lim1);
+    if (L > r) {
+        u64 t3 = L < 2 * r - s ? L : 2 * r - s;
+        v ^= pwin(p, pw, b + r, r, L);
+        v ^= pwin(p, pw, b + r - s, r, t3);
+        v ^= pwin(p, pw, b + 2 * r - s, 2 * r - s, L);
+        v ^= pwin(p, pw, b + 2 * r - 2 * s, 2 * r - s, L);
+    }
+    out[w] = v;
+}
+
+// dst ^= x*src mod T
+__global__ __launch_bounds__(SQ_THREADS) void k_mulx_xor(
+        u64 *__restrict__ dst, const u64 *__restrict__ src, size_t nw, u64 r,
+        u64 s) {
+    size_t w = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
+    if (w >= nw) return;
+    u64 a = src[w];
+    u64 prev = w ? src[w - 1] : 0;
+    u64 v = (a << 1) | (prev >> 63);
+    if (w == (size_t)(r >> 6)) v &= ~(1ULL << (r & 63));
+    u64 wrap = (src[(r - 1) >> 6] >> ((r - 1) & 63)) & 1;   // coeff of x^(r-1)
+    if (w == 0) v ^= wrap;
+    if (w == (size_t)(s >> 6)) v ^= wrap << (s & 63);
+    dst[w] ^= v;
+}
+
+__global__ void k_set_word0(u64 *p, size_t nw, u64 v0) {
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) p[w] = w ? 0 : v0;
+}
+
+__global__ void k_xor_word0(u64 *p, u64 v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) p[0] ^= v;
+}
+
+bool valid_rs(u64 r, u64 s) { return r >= 3 && s >= 1 && 2 * s < r; }
+
+unsigned blocks_for(size_t work, int threads) {
+    return (unsigned)((work + threads - 1) / threads);
+}
+
+}  // namespace
+
+extern "C" int gf2t_square_fold(const uint64_t *din, uint64_t *dout, uint64_t r,
+                                uint64_t s, void *stream) {
+    if (!valid_rs(r, s) || !din || !dout || din == dout) {
+        gf2t::set_error("gf2t_square_fold: need r >= 3, 1 <= s, 2s < r, din != dout");
+        return GF2T_EINVAL;
+    }
+    size_t nw = words_of(r), np = (nw + 1) / 2;
+    hipLaunchKernelGGL(k_square_fold, dim3(blocks_for(np, SQ_THREADS)), dim3(SQ_THREADS), 0,
+                       (hipStream_t)stream, din, dout, nw, r, s);
+    GF2T_CHECK_LAUNCH("k_square_fold");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r,
+                                      uint64_t s, uint64_t steps, void *stream) {
+    if (!valid_rs(r, s) || !dres || !dtmp || dres == dtmp) {
+        gf2t::set_error("gf2t_square_fold_steps: bad arguments");
+        return GF2T_EINVAL;
+    }
+    size_t nw = words_of(r), np = (nw + 1) / 2;
+    hipStream_t st = (hipStream_t)stream;
+    u64 *a = dres, *b = dtmp;
+    for (u64 k = 0; k < steps; k++) {
+        hipLaunchKernelGGL(k_square_fold, dim3(blocks_for(np, SQ_THREADS)), dim3(SQ_THREADS),
+                           0, st, a, b, nw, r, s);
+        std::swap(a, b);
+    }
+    GF2T_CHECK_LAUNCH("k_square_fold");
+    if (steps & 1) GF2T_HIP(hipMemcpyAsync(dres, dtmp, nw * 8, hipMemcpyDeviceToDevice, st));
+    return GF2T_OK;
+}
+
+namespace {
+// descriptor tables for the batched chain, cached and grown on demand
+struct BatchScratch {
+    SqDesc *d = nullptr;
+    size_t cap = 0;
+    int dev = -1;
+} g_batch;
+}  // namespace
+
+extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
+                                      const uint64_t *r, const uint64_t *s,
+                                      const uint64_t *steps, int n, void *stream) {
+    if (n < 0 || (n > 0 && (!dres || !dtmp || !r || !s || !steps))) {
+        gf2t::set_error("gf2t_square_fold_batch: bad arguments");
+        return GF2T_EINVAL;
+    }
+    if (n == 0) return GF2T_OK;
+    if (n > 65535) {
+        gf2t::set_error("gf2t_square_fold_batch: at most 65535 residues per call");
+        return GF2T_ERANGE;
+    }
+    for (int i = 0; i < n; i++)
+        if (!valid_rs(r[i], s[i]) || !dres[i] || !dtmp[i] || dres[i] == dtmp[i]) {
+            gf2t::set_error("gf2t_square_fold_batch: residue %d has bad r/s/buffers", i);
+            return GF2T_EINVAL;
+        }
+    hipStream_t st = (hipStream_t)stream;
+    // order residues by remaining steps, longest first: the residues
+    // still active at step k are a prefix, so each launch covers exactly
+    // the live chains (uneven frontiers, per-s early exit)
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+    int dev = 0;
+    GF2T_HIP(hipGetDevice(&dev));
+    size_t need = 2 * (size_t)n;
+    if (g_batch.cap < need || g_batch.dev != dev) {
+        if (g_batch.d && g_batch.dev == dev) (void)hipFree(g_batch.d);
+        g_batch.d = nullptr;
+        GF2T_HIP(hipMalloc(&g_batch.d, need * sizeof(SqDesc)));
+        g_batch.cap = need;
+        g_batch.dev = dev;
+    }
+    std::vector<SqDesc> h(need);
+    size_t maxpairs = 0;
+    for (int k = 0; k < n; k++) {
+        int i = ord[k];
+        u64 nw = words_of(r[i]);
+        h[k] = SqDesc{dres[i], dtmp[i], r[i], s[i], nw};          // even steps
+        h[n + k] = SqDesc{dtmp[i], dres[i], r[i], s[i], nw};      // odd steps
+        maxpairs = std::max(maxpairs, (size_t)(nw + 1) / 2);
+    }
+    GF2T_HIP(hipMemcpyAsync(g_batch.d, h.data(), need * sizeof(SqDesc), hipMemcpyHostToDevice, st));
+    u64 kmax = steps[ord[0]];
+    int active = n;
+    for (u64 k = 0; k < kmax; k++) {
+        while (active > 0 && steps[ord[active - 1]] <= k) active--;
+        const SqDesc *tab = g_batch.d + ((k & 1) ? n : 0);
+        hipLaunchKernelGGL(k_square_fold_batch, dim3(blocks_for(maxpairs, SQ_THREADS), active),
+                           dim3(SQ_THREADS), 0, st, tab);
+    }
+    GF2T_CHECK_LAUNCH("k_square_fold_batch");
+    for (int i = 0; i < n; i++)
+        if (steps[i] & 1)
+            GF2T_HIP(hipMemcpyAsync(dres[i], dtmp[i], words_of(r[i]) * 8,
+                                    hipMemcpyDeviceToDevice, st));
+    // the host table must outlive the async copy
+    GF2T_HIP(hipStreamSynchronize(st));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_fold(const uint64_t *dsrc, uint64_t src_bits, uint64_t *dout,
+                         uint64_t r, uint64_t s, void *stream) {
+    if (!valid_rs(r, s) || !dsrc || !dout || dsrc == dout || src_bits > 2 * r - 1) {
+        gf2t::set_error("gf2t_fold: need valid r/s, src_bits <= 2r-1, dsrc != dout");
+        return GF2T_EINVAL;
+    }
+    size_t nw = words_of(r), pw = words_of(src_bits);
+    hipLaunchKernelGGL(k_fold, dim3(blocks_for(nw, SQ_THREADS)), dim3(SQ_THREADS), 0,
+                       (hipStream_t)stream, dsrc, pw, src_bits, dout, nw, r, s);
+    GF2T_CHECK_LAUNCH("k_fold");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_mulx_xor(uint64_t *ddst, const uint64_t *dsrc, uint64_t r,
+                             uint64_t s, void *stream) {
+    if (!valid_rs(r, s) || !ddst || !dsrc || ddst == dsrc) {
+        gf2t::set_error("gf2t_mulx_xor: bad arguments");
+        return GF2T_EINVAL;
+    }
+    size_t nw = words_of(r);
+    hipLaunchKernelGGL(k_mulx_xor, dim3(blocks_for(nw, SQ_THREADS)), dim3(SQ_THREADS), 0,
+                       (hipStream_t)stream, ddst, dsrc, nw, r, s);
+    GF2T_CHECK_LAUNCH("k_mulx_xor");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_set_x(uint64_t *dp, size_t nw, void *stream) {
+    if (!dp || nw == 0) { gf2t::set_error("gf2t_set_x: bad arguments"); return GF2T_EINVAL; }
+    hipLaunchKernelGGL(k_set_word0, dim3(blocks_for(nw, 256)), dim3(256), 0,
+                       (hipStream_t)stream, dp, nw, 2ULL);
+    GF2T_CHECK_LAUNCH("k_set_word0");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_set_one(uint64_t *dp, size_t nw, void *stream) {
+    if (!dp || nw == 0) { gf2t::set_error("gf2t_set_one: bad arguments"); return GF2T_EINVAL; }
+    hipLaunchKernelGGL(k_set_word0, dim3(blocks_for(nw, 256)), dim3(256), 0,
+                       (hipStream_t)stream, dp, nw, 1ULL);
+    GF2T_CHECK_LAUNCH("k_set_word0");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_xor_x(uint64_t *dp, void *stream) {
+    if (!dp) { gf2t::set_error("gf2t_xor_x: null pointer"); return GF2T_EINVAL; }
+    hipLaunchKernelGGL(k_xor_word0, dim3(1), dim3(64), 0, (hipStream_t)stream, dp, 2ULL);
+    GF2T_CHECK_LAUNCH("k_xor_word0");
+    return GF2T_OK;
+}

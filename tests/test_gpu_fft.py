@@ -1,0 +1,107 @@
+"""(c) Cantor transform and products on the GPU against the oracle (and,
+at full scale, residues modulo independent degree-61 irreducibles as in
+the reference's bigtest).  Bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+R136 = 136279841
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import device
+    device.init(0)
+    return M
+
+
+def rand_bits(rng, bits):
+    a = rng.integers(0, 2**64, max(1, O.words(bits)), dtype=np.uint64)
+    if bits & 63:
+        a[-1] &= np.uint64((1 << (bits & 63)) - 1)
+    return a
+
+
+@pytest.mark.parametrize("m", list(range(1, 21)))
+def test_transform_bit_identical(eng, m):
+    rng = np.random.default_rng(m)
+    f = rng.integers(0, 2**64, 1 << m, dtype=np.uint64)
+    F = eng.fft_fwd(f, m)
+    assert np.array_equal(F, O.fft_fwd(f, m))
+    assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f)
+
+
+@pytest.mark.parametrize("abits,bbits", [(1, 1), (31, 33), (64, 64), (65, 1), (1000, 2500),
+                                         (4097, 3333), (1 << 16, (1 << 16) - 5),
+                                         (100000, 70001), (1 << 20, 1 << 20),
+                                         (3021377, 3021377)])
+def test_product_vs_oracle(eng, abits, bbits):
+    rng = np.random.default_rng(abits * 3 + bbits)
+    a, b = rand_bits(rng, abits), rand_bits(rng, bbits)
+    p = eng.mul(a, abits, b, bbits)
+    want = O.fft_mul(a, abits, b, bbits) if abits > 5000 else O.mul_comb(a, abits, b, bbits)
+    assert np.array_equal(p, want)
+
+
+@pytest.mark.parametrize("r,s", [(4423, 1200), (132049, 1), (132049, 66024), (756839, 377)])
+def test_mulmod_vs_oracle(eng, r, s):
+    rng = np.random.default_rng(r ^ s)
+    a, b = rand_bits(rng, r), rand_bits(rng, r)
+    assert np.array_equal(eng.mulmod(a, r, b, r, r, s), O.mulmod(a, r, b, r, r, s))
+
+
+def test_full_scale_product_residues(eng):
+    # one 136279841 x 136279841-bit product (2^24 points), checked modulo
+    # three degree-61 irreducibles (the reference's bigtest seeds,
+    # src/cantor_ref.cpp:159-224), then folded mod x^r + x^474 + 1 and
+    # compared with the oracle's linear fold
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+    rng = np.random.default_rng(136)
+    a, b = rand_bits(rng, R136), rand_bits(rng, R136)
+    p = eng.mul(a, R136, b, R136)
+    for seed in (0x1234567, 0xDEADBEEF, 0xC0FFEE):
+        q = O.find_irreducible(61, seed)
+        ra, rb = O.residue(a, R136, q, 61), O.residue(b, R136, q, 61)
+        prod = O.mul_comb(np.array([ra], np.uint64), 64, np.array([rb], np.uint64), 64)
+        assert O.residue(p, 2 * R136, q, 61) == O.residue(prod, 128, q, 61), hex(seed)
+    src = DeviceBuffer.from_array(p)
+    out = DeviceBuffer.words(O.words(R136))
+    check(lib.gf2t_fold(src.ptr, 2 * R136 - 1, out.ptr, R136, 474, None))
+    assert np.array_equal(out.download(), O.reduce_fold(p, 2 * R136 - 1, R136, 474))
+
+
+def test_square_via_fft_equals_spread_square(eng):
+    # the reference's bench cross-check (src/trinomial_stage2.cu:165-174)
+    r, s = 756839, 12345
+    rng = np.random.default_rng(1)
+    a = rand_bits(rng, r)
+    assert np.array_equal(eng.mulmod(a, r, a, r, r, s), eng.square_fold(a, r, s, 1))
+
+
+def test_trin_gpu_interval_product(eng):
+    # ACC = prod_{k=5..12} (F_k + x) at r=4423 s=1200
+    # (src/trinomial_stage2.cu:109-137), against the oracle
+    r, s = 4423, 1200
+    G = eng.TrinGPU()
+    G.init(r)
+    G.set_s(s)
+    G.set_x()
+    G.reset_acc()
+    F = O.x_residue(r)
+    acc = np.zeros(O.words(r), np.uint64)
+    acc[0] = 1
+    for k in range(1, 13):
+        G.square_A()
+        F = O.square_fold(F, r, s)
+        if k >= 5:
+            G.accumulate()
+            t = F.copy()
+            t[0] ^= np.uint64(2)
+            acc = O.mulmod(acc, r, t, r, r, s)
+    assert np.array_equal(G.A(), F)
+    assert np.array_equal(G.ACC(), acc)
+    G.fini()

@@ -1,0 +1,79 @@
+"""(a) coarse sieve on the GPU against the oracle, the reference's golden
+output and closed forms.  Bit-exact best[] images."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+R136 = 136279841
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import device
+    device.init(0)
+    return M
+
+
+def md5_lines(lines):
+    return hashlib.md5(("\n".join(lines) + "\n").encode()).hexdigest()
+
+
+def test_golden_r132049_d20(eng):
+    best, st = eng.run_gpu(132049, 20, 66024)
+    lines = O.cert_lines(best)
+    assert len(lines) == 60233
+    assert md5_lines(lines) == "737739bafe9d565781900c8f45828440"
+    for s in st:
+        assert s.classes == O.count_irreducible(s.d)
+
+
+@pytest.mark.parametrize("r,depth,smax", [(136279841, 11, 30000), (74207281, 10, 20000),
+                                          (101, 9, 100), (3, 2, 1)])
+def test_vs_brute_force(eng, r, depth, smax):
+    best, _ = eng.run_gpu(r, depth, smax)
+    assert np.array_equal(best, O.brute_force(r, depth, smax))
+
+
+def test_published_count_d20(eng):
+    best, _ = eng.run_gpu(R136, 20, 999999)
+    assert int(np.count_nonzero(best[1:] != O.NONE)) == 913104
+    ob, _ = O.sieve(R136, 20, 999999, nthreads=8)
+    assert np.array_equal(best, ob)
+
+
+def test_c2_shape_vs_oracle_d24(eng):
+    best, _ = eng.run_gpu(3021377, 24, 1510688)
+    ob, _ = O.sieve(3021377, 24, 1510688, nthreads=8)
+    assert np.array_equal(best, ob)
+
+
+def test_incremental_degrees_equal_one_shot(eng):
+    a, _ = eng.run_gpu(756839, 22, 378419)
+    b, _ = eng.run_gpu(756839, 16, 378419)
+    c, _ = eng.run_gpu(756839, 22, 378419, best=b, min_depth=17)
+    assert np.array_equal(a, c)
+
+
+def test_high_degrees_marks_verified(eng):
+    # d <= 28 at r=3021377: every class count equals the irreducible count,
+    # the d <= 24 part matches the oracle, and sampled higher-degree marks
+    # are genuine irreducible factors
+    best, st = eng.run_gpu(3021377, 28, 1510688)
+    for s in st:
+        assert s.classes == O.count_irreducible(s.d), s.d
+    ob, _ = O.sieve(3021377, 24, 1510688, nthreads=8)
+    low = (best >> np.uint64(48)) <= 24
+    assert np.array_equal(best[low], ob[low])
+    assert np.all(ob[~low] == O.NONE)
+    hi_s = np.nonzero(~low & (best != O.NONE))[0]
+    rng = np.random.default_rng(0)
+    for s in rng.choice(hi_s, size=min(200, hi_s.size), replace=False):
+        k = int(best[s])
+        d, mask = k >> 48, k & 0xFFFFFFFFFFFF
+        assert O.is_irreducible_small(mask, d)
+        assert O.divides_trinomial_small(3021377, int(s), mask, d)
