@@ -1,0 +1,200 @@
+"""Benchmark: the GF(2) trinomial hot path at r = 136279841 on one MI355X.
+
+One step = one pass of the GPU hot path over one batch of synthetic input:
+  (a) Zech-log coarse sieve, degrees 2..32, every s in [1, 68139920]:
+      each s with an irreducible factor of degree <= 32 gets its final
+      certificate line 's d p<hex>' (smallest degree, least mask);
+  (b) Frobenius chains F <- F^2 mod x^r + x^s + 1 for 256 sieve survivors
+      (seeded draw; 17 MB residues resident in HBM, 4.36 GB in flight)
+      advanced 512 squarings (one checkpoint interval) in the batched
+      form, starting from F = x.
+value = certified s-values / step wall time (sieve certificates only; the
+chain time is included in the step but certifies nothing by itself).
+roofline = the squaring+fold kernel (k_square_fold_batch): algorithmic
+bytes 16*nw per residue-step times 256 residues per launch, over its
+average launch time measured with HIP events on its stream.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+R = 136279841
+SMAX = R // 2
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--depth", type=int, default=32)
+    ap.add_argument("--residues", type=int, default=256)
+    ap.add_argument("--chain-steps", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=20250101)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-depth", type=int, default=24)
+    return ap.parse_args()
+
+
+def cpu_baseline(depth: int, chain_s: int):
+    """The oracle (CPU restatement) on a bounded sample of the same
+    workload: the sieve at r=136279841 over all s <= 68139920 for degrees
+    2..depth on all host threads, plus two squaring+fold steps of one
+    full-size residue on one thread."""
+    import oracle as O
+    cores = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    best, _ = O.sieve(R, depth, SMAX, nthreads=cores)
+    t_sieve = time.perf_counter() - t0
+    cert = int(np.count_nonzero(best[1:] != O.NONE))
+    del best
+    x = O.x_residue(R)
+    t0 = time.perf_counter()
+    O.square_fold_steps(x, R, chain_s, 2)
+    t_sq = (time.perf_counter() - t0) / 2
+    return {
+        "value": cert / t_sieve,
+        "unit": "s-values certified/sec",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"oracle sieve r={R}, all s<={SMAX}, degrees 2..{depth} "
+                  f"({cert} certified in {t_sieve:.2f} s on {cores} threads); "
+                  f"oracle squaring+fold {1e3 * t_sq:.1f} ms per residue-step (1 thread)",
+        "squaring_fold_residue_steps_per_s_1core": 1.0 / t_sq,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the deep scan shards by independent s values and the sieve by degree buckets; "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import SieveStats, check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer, Event
+
+    device.init(0)
+    nw = (R + 63) // 64
+    n = args.residues
+    dbest = DeviceBuffer.words(SMAX + 1)
+    res = [DeviceBuffer.words(nw) for _ in range(n)]
+    tmp = [DeviceBuffer.words(nw) for _ in range(n)]
+    cres = (C.c_void_p * n)(*[b.ptr for b in res])
+    ctmp = (C.c_void_p * n)(*[b.ptr for b in tmp])
+    P64 = C.POINTER(C.c_uint64)
+    rr = np.full(n, R, dtype=np.uint64)
+    kk = np.full(n, args.chain_steps, dtype=np.uint64)
+    stats = (SieveStats * 33)()
+    svals = None
+
+    def one_step(timed: bool):
+        nonlocal svals
+        e0, e1, e2 = Event(), Event(), Event()
+        e0.record()
+        dbest.fill(0xFF)
+        check(lib.gf2t_sieve(R, 2, args.depth, SMAX, dbest.ptr, stats), "gf2t_sieve")
+        e1.record()
+        if svals is None:
+            best = dbest.download(SMAX + 1)
+            surv = np.nonzero(best == np.uint64(0xFFFFFFFFFFFFFFFF))[0]
+            surv = surv[surv >= 3]
+            rng = np.random.default_rng(args.seed)
+            svals = np.sort(rng.choice(surv, size=n, replace=False)).astype(np.uint64)
+        for b in res:
+            check(lib.gf2t_set_x(b.ptr, nw, None))
+        ech0 = Event()
+        ech0.record()
+        check(lib.gf2t_square_fold_batch(cres, ctmp, rr.ctypes.data_as(P64),
+                                         svals.ctypes.data_as(P64), kk.ctypes.data_as(P64),
+                                         n, None), "gf2t_square_fold_batch")
+        e2.record()
+        device.synchronize()
+        return e0.elapsed_ms(e1), ech0.elapsed_ms(e2), e0.elapsed_ms(e2)
+
+    for _ in range(args.warmup):
+        one_step(False)
+    device.synchronize()
+    t_start = time.perf_counter()
+    sieve_ms, chain_ms, step_ms = [], [], []
+    for _ in range(args.steps):
+        a, b, c = one_step(True)
+        sieve_ms.append(a)
+        chain_ms.append(b)
+        step_ms.append(c)
+    device.synchronize()
+    wall = time.perf_counter() - t_start
+
+    best = dbest.download(SMAX + 1)
+    certified = int(np.count_nonzero(best[1:] != np.uint64(0xFFFFFFFFFFFFFFFF)))
+    survivors = SMAX - certified
+    per_step_ms = 1e3 * wall / args.steps
+    value = certified / (wall / args.steps)
+
+    launch_ms = float(np.mean(chain_ms)) / args.chain_steps
+    bytes_per_launch = 16 * nw * n
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    deg = [{"d": stats[d].d, "ms": round(stats[d].ms_table + stats[d].ms_classes + stats[d].ms_heavy, 2),
+            "ms_table": round(stats[d].ms_table, 2), "classes": int(stats[d].classes),
+            "marking": int(stats[d].marking)} for d in range(2, args.depth + 1)]
+    line = {
+        "metric": "s-values certified/sec at r=136279841; squaring+fold GB/s vs HBM peak",
+        "value": value,
+        "unit": "s-values certified/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per_step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"r={R}: coarse sieve d=2..{args.depth} over all s in 1..{SMAX} + "
+                        f"{args.chain_steps} batched squaring+fold steps on {n} sieve survivors "
+                        f"(17 MB residues)",
+            "r": R, "smax": SMAX, "depth": args.depth, "residues": n,
+            "chain_steps": args.chain_steps, "seed": args.seed,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_square_fold_batch",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": launch_ms,
+        },
+        "sieve": {"ms": float(np.mean(sieve_ms)), "certified": certified,
+                  "survivors": survivors, "per_degree": deg},
+        "squaring_fold": {"ms": float(np.mean(chain_ms)),
+                          "residue_steps_per_s": n * args.chain_steps / (np.mean(chain_ms) * 1e-3),
+                          "GB_per_s": achieved},
+        "event_ms_per_step": float(np.mean(step_ms)),
+        "device": device.device_name(),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_depth, int(svals[0]))
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,33 +26,10 @@ namespace {
 
 constexpr int SQ_THREADS = 256;
 
-// 128-bit window of the virtual spread of `in` at SP position P, keeping
-// only SP indices in [lo, hi).
-GF2T_DEV void vspread128(const u64 *__restrict__ in, size_t nw, u64 P, u64 lo,
-                         u64 hi, u64 &w0, u64 &w1) {
-    if (P >= hi || P + 128 <= lo) { w0 = 0; w1 = 0; return; }
-    u64 v = win64(in, nw, (P + 1) >> 1);
-    u64 a = spread32((u32)v), b = spread32((u32)(v >> 32));
-    if (P & 1) { a <<= 1; b <<= 1; }
-    int lc = lo > P ? (int)(lo - P) : 0;
-    int hc = (hi - P) >= 128 ? 128 : (int)(hi - P);
-    w0 = a & range_mask64(lc, hc);
-    w1 = b & range_mask64(lc - 64, hc - 64);
-}
-
 GF2T_DEV void square_pair(const u64 *__restrict__ in, u64 *__restrict__ out,
                           size_t nw, u64 r, u64 s, size_t i) {
-    const u64 b = (u64)i << 7;
-    u64 o0, o1, t0, t1;
-    vspread128(in, nw, b, 0, r, o0, o1);
-    vspread128(in, nw, b + r, r, 2 * r - 1, t0, t1);
-    o0 ^= t0; o1 ^= t1;
-    vspread128(in, nw, b + r - s, r, 2 * r - s, t0, t1);
-    o0 ^= t0; o1 ^= t1;
-    vspread128(in, nw, b + 2 * r - s, 2 * r - s, 2 * r - 1, t0, t1);
-    o0 ^= t0; o1 ^= t1;
-    vspread128(in, nw, b + 2 * r - 2 * s, 2 * r - s, 2 * r - 1, t0, t1);
-    o0 ^= t0; o1 ^= t1;
+    u64 o0, o1;
+    square_fold_pair(in, nw, r, s, i, o0, o1);
     out[2 * i] = o0;
     if (2 * i + 1 < nw) out[2 * i + 1] = o1;
 }
@@ -81,31 +58,12 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
     square_pair(d.in, d.out, d.nw, d.r, d.s, i);
 }
 
-// generic fold of a product P (L <= 2r-1 bits) into r bits
-GF2T_DEV u64 pwin(const u64 *__restrict__ p, size_t pw, u64 pos, u64 lo, u64 hi) {
-    if (pos >= hi || pos + 64 <= lo) return 0;
-    u64 v = win64(p, pw, pos);
-    int lc = lo > pos ? (int)(lo - pos) : 0;
-    int hc = (hi - pos) >= 64 ? 64 : (int)(hi - pos);
-    return v & range_mask64(lc, hc);
-}
-
 __global__ __launch_bounds__(SQ_THREADS) void k_fold(
         const u64 *__restrict__ p, size_t pw, u64 L, u64 *__restrict__ out,
         size_t nw, u64 r, u64 s) {
     size_t w = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
     if (w >= nw) return;
-    u64 b = (u64)w << 6;
-    u64 lim1 = L < r ? L : r;
-    u64 v = pwin(p, pw, b, 0, lim1);
-    if (L > r) {
-        u64 t3 = L < 2 * r - s ? L : 2 * r - s;
-        v ^= pwin(p, pw, b + r, r, L);
-        v ^= pwin(p, pw, b + r - s, r, t3);
-        v ^= pwin(p, pw, b + 2 * r - s, 2 * r - s, L);
-        v ^= pwin(p, pw, b + 2 * r - 2 * s, 2 * r - s, L);
-    }
-    out[w] = v;
+    out[w] = fold_word(p, pw, L, r, s, w);
 }
 
 // dst ^= x*src mod T

@@ -653,7 +653,7 @@ int gf2o_mulmod(const uint64_t *a, uint64_t abits, const uint64_t *b,
     uint64_t *P = (uint64_t *)calloc(cap, 8);
     int rc = gf2o_fft_mul(a, abits, b, bbits, P);
     if (rc) { free(P); return rc; }
-    uint64_t pbits = abits + bbits;
+    uint64_t pbits = abits + bbits - 1;          /* bits of the product */
     if (pbits > 2 * r - 1) { free(P); return -3; }
     gf2o_reduce_fold(P, pbits < r ? r : pbits, r, s);
     memcpy(out, P, nw * 8);

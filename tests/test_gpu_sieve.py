@@ -33,7 +33,7 @@ def test_golden_r132049_d20(eng):
 
 
 @pytest.mark.parametrize("r,depth,smax", [(136279841, 11, 30000), (74207281, 10, 20000),
-                                          (101, 9, 100), (3, 2, 1)])
+                                          (101, 9, 100), (5, 3, 2)])
 def test_vs_brute_force(eng, r, depth, smax):
     best, _ = eng.run_gpu(r, depth, smax)
     assert np.array_equal(best, O.brute_force(r, depth, smax))
@@ -77,3 +77,22 @@ def test_high_degrees_marks_verified(eng):
         d, mask = k >> 48, k & 0xFFFFFFFFFFFF
         assert O.is_irreducible_small(mask, d)
         assert O.divides_trinomial_small(3021377, int(s), mask, d)
+
+
+def test_headline_full_range_d32_matches_reference_log(eng):
+    # r=136279841, every s <= 68139920, degrees 2..32: the reference's
+    # published run (src/coarse_sieve.log:39-64) leaves 4,110,974 survivors
+    # at d<=29, 3,976,134 at d<=30, 3,849,633 at d<=31 and 3,731,849 at
+    # d<=32, with 64,028,946 + 134,840 + 126,501 + 117,784 certificates
+    smax = R136 // 2
+    best, st = eng.run_gpu(R136, 32, smax)
+    deg = best[1:] >> np.uint64(48)
+    marked = best[1:] != O.NONE
+    for d, surv in [(29, 4110974), (30, 3976134), (31, 3849633), (32, 3731849)]:
+        assert int(np.count_nonzero(~(marked & (deg <= d)))) == surv, d
+    assert int(np.count_nonzero(marked)) == 64028946 + 134840 + 126501 + 117784
+    cls = {s.d: s.classes for s in st}
+    assert cls[30] == 35790267 and cls[31] == 69273666 and cls[32] == 134215680
+    # the two published known answers
+    assert int(best[33]) == (5 << 48) | 0x29
+    assert int(best[474]) == O.NONE   # its least factor has degree 41

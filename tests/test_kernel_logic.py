@@ -1,0 +1,99 @@
+"""The kernels' own arithmetic (gf2t_arith.hpp, shared verbatim by the
+gfx950 kernels) built for the host and checked against the oracle on the
+CPU, including every window-clipping edge.  No GPU."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(HERE), "mersenne_trinomial_amd", "csrc")
+P = C.POINTER(C.c_uint64)
+
+
+@pytest.fixture(scope="module")
+def kl(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("kl") / "libkl.so")
+    subprocess.check_call(["g++", "-O2", "-shared", "-fPIC", "-I" + CSRC, "-o", out,
+                           os.path.join(HERE, "host", "kernel_logic.cpp")])
+    lib = C.CDLL(out)
+    lib.kl_square_fold.argtypes = [P, P, C.c_uint64, C.c_uint64]
+    lib.kl_fold.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint64]
+    lib.kl_gf64_mul.restype = C.c_uint64
+    lib.kl_gf64_mul.argtypes = [C.c_uint64, C.c_uint64]
+    lib.kl_clmul32.restype = C.c_uint64
+    lib.kl_clmul32.argtypes = [C.c_uint32, C.c_uint32]
+    lib.kl_fmul.restype = C.c_uint32
+    lib.kl_fmul.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int]
+    lib.kl_range_mask64.restype = C.c_uint64
+    lib.kl_range_mask64.argtypes = [C.c_int, C.c_int]
+    return lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(P)
+
+
+def rand_res(rng, r):
+    a = rng.integers(0, 2**64, O.words(r), dtype=np.uint64)
+    if r & 63:
+        a[-1] &= np.uint64((1 << (r & 63)) - 1)
+    return a
+
+
+def test_range_mask(kl):
+    for lc in range(-130, 140, 3):
+        for hc in range(-130, 140, 5):
+            want = 0
+            for b in range(64):
+                if lc <= b < hc:
+                    want |= 1 << b
+            assert kl.kl_range_mask64(lc, hc) == want, (lc, hc)
+
+
+def test_square_fold_all_s_small_r(kl):
+    rng = np.random.default_rng(0)
+    for r in (7, 31, 65, 127, 129, 131, 193, 257, 383):
+        for s in range(1, (r - 1) // 2 + 1):
+            a = rand_res(rng, r)
+            out = np.zeros(O.words(r), np.uint64)
+            kl.kl_square_fold(ptr(a), ptr(out), r, s)
+            assert np.array_equal(out, O.square_fold(a, r, s)), (r, s)
+
+
+def test_square_fold_window_edges(kl):
+    rng = np.random.default_rng(1)
+    for r in (4423, 4097, 9689, 132049):
+        for s in sorted({1, 2, 63, 64, 65, 127, 128, 129, 191, 192, 255, 256, 257, r // 3,
+                         (r - 1) // 2 - 64, (r - 1) // 2 - 1, (r - 1) // 2}):
+            a = rand_res(rng, r)
+            out = np.zeros(O.words(r), np.uint64)
+            kl.kl_square_fold(ptr(a), ptr(out), r, s)
+            assert np.array_equal(out, O.square_fold(a, r, s)), (r, s)
+
+
+def test_fold_lengths(kl):
+    rng = np.random.default_rng(2)
+    for r, s in [(131, 65), (129, 64), (4423, 1200), (4097, 2048), (4097, 64), (4161, 127)]:
+        for L in sorted({1, 63, 64, 65, r - 1, r, r + 1, r + s - 1, r + s, 2 * r - s,
+                         2 * r - 2, 2 * r - 1}):
+            v = rand_res(rng, L)
+            out = np.zeros(O.words(r), np.uint64)
+            kl.kl_fold(ptr(v), L, ptr(out), r, s)
+            assert np.array_equal(out, O.reduce_bitserial(v, L, r, s)), (r, s, L)
+
+
+def test_field_multiplies(kl):
+    rng = np.random.default_rng(3)
+    for a, b in rng.integers(0, 2**64, (3000, 2), dtype=np.uint64):
+        assert kl.kl_gf64_mul(int(a), int(b)) == O.gf64_mul_bitserial(int(a), int(b))
+        x, y = int(a) & 0xFFFFFFFF, int(b) & 0xFFFFFFFF
+        assert kl.kl_clmul32(x, y) == O.lib().gf2o_clmul32(x, y)
+    for d in (2, 5, 16, 17, 31, 32):
+        q = O.find_primitive_poly(d)
+        for a, b in rng.integers(0, 1 << d, (300, 2), dtype=np.uint64):
+            assert kl.kl_fmul(int(a), int(b), q, d) == O.fmul(int(a), int(b), q, d)
