@@ -126,6 +126,100 @@ GF2T_HD void square_fold_pair(const u64 *in, size_t nw, u64 r, u64 s, size_t i, 
     o0 ^= t0; o1 ^= t1;
 }
 
+// ---- fast path: unmasked windows, 32-bit spreads
+// spread of the low 16 bits of x into 32 bits (bit k -> bit 2k)
+GF2T_HD u32 spread16(u32 x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+GF2T_HD void spread64(u64 v, u64 &lo, u64 &hi) {
+    u32 a = (u32)v, b = (u32)(v >> 32);
+    lo = (u64)spread16(a) | ((u64)spread16(a >> 16) << 32);
+    hi = (u64)spread16(b) | ((u64)spread16(b >> 16) << 32);
+}
+
+// Window k of the squaring reads the 64 input bits starting at input bit
+// 64i + Q_k (i = output pair index) and, when the spread position is odd,
+// shifts the spread left by one.  Q_k, and so the word offset, the bit
+// shift and the parity, are the same for every pair of one residue.
+struct SqWin {
+    u64 off;
+    u32 sh, odd;
+};
+struct SqConst {
+    u64 r, s;
+    size_t nw;
+    SqWin w2, w3, w4, w5;
+};
+GF2T_HD SqWin sq_win(u64 P0) {   // P0 = window offset in spread positions (b = 0)
+    u64 Q = (P0 + 1) >> 1;
+    SqWin w;
+    w.off = Q >> 6;
+    w.sh = (u32)(Q & 63);
+    w.odd = (u32)(P0 & 1);
+    return w;
+}
+GF2T_HD SqConst sq_const(u64 r, u64 s) {
+    SqConst c;
+    c.r = r; c.s = s; c.nw = (size_t)((r + 63) >> 6);
+    c.w2 = sq_win(r);
+    c.w3 = sq_win(r - s);
+    c.w4 = sq_win(2 * r - s);
+    c.w5 = sq_win(2 * r - 2 * s);
+    return c;
+}
+GF2T_HD void win_spread(const u64 *in, size_t nw, size_t i, const SqWin &w, u64 &lo, u64 &hi) {
+    size_t k = i + (size_t)w.off;
+    u64 v = k < nw ? in[k] : 0;
+    if (w.sh) {
+        u64 h = k + 1 < nw ? in[k + 1] : 0;
+        v = (v >> w.sh) | (h << (64 - w.sh));
+    }
+    spread64(v, lo, hi);
+    if (w.odd) { lo <<= 1; hi <<= 1; }   // spread bits are even: no carry between words
+}
+
+// Output pair i of in^2 mod T.  Away from the points s, 2s and r every
+// window is either fully inside or fully outside its clip range:
+//   [0, s)  : SP[0,r) + SP[r,..)>>r + SP[2r-s,..)>>(2r-s)          (W1 W2 W4)
+//   [s, 2s) : W1 + W2 + SP[r,2r-s)>>(r-s) + SP[2r-s,..)>>(2r-2s)   (W1 W2 W3 W5)
+//   [2s, r) : W1 + W2 + W3
+// so those pairs skip all masking; pairs straddling s, 2s or r take the
+// clipped general path.
+GF2T_HD void square_fold_pair_fast(const u64 *in, const SqConst &C, size_t i, u64 &o0, u64 &o1) {
+    const u64 b = (u64)i << 7;
+    const u64 s = C.s, r = C.r;
+    int region;
+    if (b + 128 <= s) region = 0;
+    else if (b >= s && b + 128 <= 2 * s) region = 1;
+    else if (b >= 2 * s && b + 128 <= r) region = 2;
+    else region = 3;
+    if (region == 3) {
+        square_fold_pair(in, C.nw, r, s, i, o0, o1);
+        return;
+    }
+    u64 t0, t1;
+    u64 x = i < C.nw ? in[i] : 0;
+    spread64(x, o0, o1);
+    win_spread(in, C.nw, i, C.w2, t0, t1);
+    o0 ^= t0; o1 ^= t1;
+    if (region == 0) {
+        win_spread(in, C.nw, i, C.w4, t0, t1);
+        o0 ^= t0; o1 ^= t1;
+    } else {
+        win_spread(in, C.nw, i, C.w3, t0, t1);
+        o0 ^= t0; o1 ^= t1;
+        if (region == 1) {
+            win_spread(in, C.nw, i, C.w5, t0, t1);
+            o0 ^= t0; o1 ^= t1;
+        }
+    }
+}
+
 // 64-bit window of p (pw words) at bit pos, restricted to bits [lo, hi)
 GF2T_HD u64 pwin64(const u64 *p, size_t pw, u64 pos, u64 lo, u64 hi) {
     if (pos >= hi || pos + 64 <= lo) return 0;
@@ -148,4 +242,59 @@ GF2T_HD u64 fold_word(const u64 *p, size_t pw, u64 L, u64 r, u64 s, size_t w) {
         v ^= pwin64(p, pw, b + 2 * r - 2 * s, 2 * r - s, L);
     }
     return v;
+}
+
+// ------------------------------------------------ sieve number theory
+// x mod M for M = 2^d - 1 (Mersenne folding)
+GF2T_HD u64 mod_mersenne(u64 x, int d, u64 M) {
+    while (x > M) x = (x & M) + (x >> d);
+    return x == M ? 0 : x;
+}
+GF2T_HD u32 gcd32(u32 a, u32 b) {
+    while (b) { u32 t = a % b; a = b; b = t; }
+    return a;
+}
+// inverse of a modulo m (gcd(a, m) = 1, m >= 2), extended Euclid in u32
+GF2T_HD u32 inv32(u32 a, u32 m) {
+    u32 r0 = m, r1 = a % m;
+    long long t0 = 0, t1 = 1;
+    while (r1) {
+        u32 q = r0 / r1;
+        u32 r2 = r0 - q * r1;
+        long long t2 = t0 - (long long)q * t1;
+        r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+    }
+    if (t0 < 0) t0 += m;
+    return (u32)t0;
+}
+
+// Minimal polynomial over GF(2) of beta in GF(2^d) (beta in no proper
+// subfield) from the GF(2)-valued sequence a_k = Tr(beta^k): it obeys the
+// recurrence whose characteristic polynomial is minpoly(beta) and has
+// linear complexity d, so Berlekamp-Massey on a_0..a_{2d-1} returns the
+// reversed minimal polynomial.  Tr(y) = parity(y & tmask) with tmask bit
+// i = Tr(x^i).  2d field multiplies instead of the d^2 of the product
+// over the conjugates; the polynomial is the same (it is unique).
+// Returns the (d+1)-bit mask, or 0 if the complexity is not d.
+GF2T_HD u64 minpoly_bm(u32 beta, u64 q, int d, u32 tmask) {
+    u64 seq = 0;
+    u32 pw = 1;
+    for (int k = 0; k < 2 * d; k++) {
+        seq |= (u64)(__builtin_popcount(pw & tmask) & 1) << k;
+        pw = fmul(pw, beta, q, d);
+    }
+    u64 C = 1, B = 1, W = 0;
+    int L = 0, m = 1;
+    for (int n = 0; n < 2 * d; n++) {
+        W = (W << 1) | ((seq >> n) & 1);           // bit j = a_{n-j}
+        if (!(__builtin_popcountll(C & W) & 1)) { m++; continue; }
+        u64 T = C;
+        C ^= B << m;
+        if (2 * L <= n) { L = n + 1 - L; B = T; m = 1; }
+        else m++;
+    }
+    if (L != d) return 0;
+    u64 mask = 0;
+    for (int k = 0; k <= d; k++) mask |= ((C >> (d - k)) & 1) << k;
+    return mask;
 }

@@ -18,10 +18,13 @@
 //                 4-byte scatter: the HBM-bound part, 4*2^d bytes)
 //   k_lyndon      one lane per candidate j, 64-lane ballot compaction of
 //                 the class representatives into a dense list
-//   k_resolve     one lane per class: Zech log (one random gather), the
-//                 progression, and -- only if it reaches s <= smax -- the
-//                 minimal polynomial and the marks (inline up to 1024,
-//                 else queued for k_mark_heavy)
+//   k_zech        one lane per class: Zech log (one random gather) and the
+//                 progression; classes whose progression reaches s <= smax
+//                 are ballot-compacted into a queue (about 1.6% of them at
+//                 d = 32 for smax = 68,139,920)
+//   k_mark        one lane per queued class: minimal polynomial by
+//                 Berlekamp-Massey on Tr(beta^k), then the marks (inline up
+//                 to 1024, longer progressions to k_mark_heavy)
 //   k_mark_heavy  many blocks per heavy progression
 // The dense tables fit HBM for every d <= 32 (16 GiB at d = 32), so no
 // bucketed low-memory mode is needed on a 288 GB part.
@@ -138,17 +141,13 @@ u64 irreducible_count(int d) {
 struct DegreeParams {
     int d;
     u64 M, q, rmod, smax;
+    u32 tmask;               // Tr(y) = parity(y & tmask)
     const u32 *Alo, *Ahi;    // g^k and g^(k<<16), k < 2^16
 };
 
 GF2T_DEV u32 antilog(const DegreeParams &P, u64 e) {
     if (P.d <= 16) return P.Alo[e];
     return fmul(P.Ahi[e >> 16], P.Alo[e & 0xFFFF], P.q, P.d);
-}
-
-GF2T_DEV u64 mod_mersenne(u64 x, int d, u64 M) {
-    while (x > M) x = (x & M) + (x >> d);
-    return x == M ? 0 : x;
 }
 
 __global__ void k_log_build(DegreeParams P, u32 *__restrict__ L) {
@@ -198,70 +197,66 @@ __global__ void k_lyndon(int d, u64 M, u32 *__restrict__ surv, unsigned long lon
     }
 }
 
-GF2T_DEV u64 gcd_dev(u64 a, u64 b) {
-    while (b) { u64 t = a % b; a = b; b = t; }
-    return a;
-}
-GF2T_DEV u64 inv_dev(u64 a, u64 m) {
-    int64_t t = 0, nt = 1, rr = (int64_t)m, nr = (int64_t)(a % m);
-    while (nr) {
-        int64_t qq = rr / nr, tmp;
-        tmp = t - qq * nt; t = nt; nt = tmp;
-        tmp = rr - qq * nr; rr = nr; nr = tmp;
+struct MarkRec {
+    u32 j, n, sf, pad;
+};
+
+// Phase 1, one lane per class: Zech logarithm (one random gather), the
+// progression s == s0 (mod n), and -- only for the few classes whose
+// progression reaches s <= smax -- a ballot-compacted queue entry, so the
+// expensive minimal polynomial below runs on dense waves.
+__global__ void k_zech(DegreeParams P, const u32 *__restrict__ surv, u64 nsurv,
+                       const u32 *__restrict__ L, MarkRec *__restrict__ queue,
+                       unsigned long long *__restrict__ nqueue, u64 qcap) {
+    const int lane = threadIdx.x & 63;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u32 M = (u32)P.M;
+    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < nsurv;
+         wbase += stride) {
+        u64 t = wbase + lane;
+        bool keep = false;
+        MarkRec rec{0, 0, 0, 0};
+        if (t < nsurv) {
+            u32 j = surv[t];
+            u64 u = mod_mersenne((u64)j * P.rmod, P.d, P.M);
+            if (u) {                                   // u = 0: beta^r = 1, never a root
+                u32 v = L[antilog(P, u) ^ 1u];         // Zech logarithm of u
+                u32 gg = gcd32(j, M);
+                if (v % gg == 0) {
+                    u32 n = M / gg;
+                    u32 iv = inv32(j / gg, n);
+                    u64 prod = (u64)iv * (v / gg);
+                    u32 s0 = (u32)(gg == 1 ? mod_mersenne(prod, P.d, P.M) : prod % n);
+                    u32 sf = s0 ? s0 : n;
+                    if (sf <= P.smax) { keep = true; rec = MarkRec{j, n, sf, 0}; }
+                }
+            }
+        }
+        unsigned long long m = __ballot(keep);
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(nqueue, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (keep) {
+            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
+            if (pos < qcap) queue[pos] = rec;
+        }
     }
-    if (t < 0) t += (int64_t)m;
-    return (u64)t;
 }
 
-// minimal polynomial prod_{i<d} (X + beta^(2^i)), coefficients in GF(2)
-template <int D>
-GF2T_DEV u64 minpoly_mask(u32 beta, u64 q) {
-    // P <- P * (X + root) over all D conjugates; the update runs over
-    // every k (entries above the current degree are 0 and stay 0) so
-    // that P[] has static indices and lives in registers
-    u32 P[D + 1];
-#pragma unroll
-    for (int k = 0; k <= D; k++) P[k] = 0;
-    P[0] = 1;
-    u32 root = beta;
-#pragma unroll 1
-    for (int i = 0; i < D; i++) {
-#pragma unroll
-        for (int k = D; k >= 1; k--) P[k] = P[k - 1] ^ fmul(root, P[k], q, D);
-        P[0] = fmul(root, P[0], q, D);
-        root = fsqr(root, q, D);
-    }
-    u64 mask = 0;
-#pragma unroll
-    for (int k = 0; k <= D; k++) {
-        if (P[k] > 1) return 0;
-        mask |= (u64)P[k] << k;
-    }
-    return mask;
-}
-
-template <int D>
-__global__ void k_resolve(DegreeParams P, const u32 *__restrict__ surv, u64 nsurv,
-                          const u32 *__restrict__ L, unsigned long long *__restrict__ best,
-                          Heavy *__restrict__ heavy, unsigned *__restrict__ nheavy,
-                          unsigned long long *__restrict__ nmark, unsigned *__restrict__ bad) {
+// Phase 2, one lane per queued class: minimal polynomial by
+// Berlekamp-Massey on the trace sequence, then the marks (inline up to
+// INLINE_MARKS, longer progressions to the heavy queue).
+__global__ void k_mark(DegreeParams P, const MarkRec *__restrict__ queue, u64 nq,
+                       unsigned long long *__restrict__ best, Heavy *__restrict__ heavy,
+                       unsigned *__restrict__ nheavy, unsigned *__restrict__ bad) {
     u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nsurv) return;
-    const u64 M = P.M;
-    u64 j = surv[t];
-    u64 u = mod_mersenne(j * P.rmod, D, M);
-    if (u == 0) return;                           // beta^r = 1: never a root
-    u64 v = L[antilog(P, u) ^ 1u];                // Zech logarithm of u
-    u64 gg = gcd_dev(j, M);
-    if (v % gg) return;
-    u64 n = M / gg;
-    u64 s0 = (inv_dev(j / gg, n) * (v / gg)) % n;
-    u64 sf = s0 ? s0 : n;
-    if (sf > P.smax) return;
-    atomicAdd(nmark, 1ULL);
-    u64 mask = minpoly_mask<D>(antilog(P, j), P.q);
-    if (mask == 0 || !((mask >> D) & 1) || !(mask & 1)) { atomicAdd(bad, 1u); return; }
-    unsigned long long key = ((unsigned long long)D << 48) | mask;
+    if (t >= nq) return;
+    const MarkRec rec = queue[t];
+    const int d = P.d;
+    u64 mask = minpoly_bm(antilog(P, rec.j), P.q, d, P.tmask);
+    if (mask == 0 || !((mask >> d) & 1) || !(mask & 1)) { atomicAdd(bad, 1u); return; }
+    unsigned long long key = ((unsigned long long)d << 48) | mask;
+    u64 n = rec.n, sf = rec.sf;
     u64 count = (P.smax - sf) / n + 1;
     if (count <= INLINE_MARKS) {
         for (u64 s = sf; s <= P.smax; s += n)
@@ -292,30 +287,6 @@ __global__ void k_zero_counters(unsigned long long *a, unsigned *b) {
 }
 
 
-template <int D>
-void launch_resolve(DegreeParams P, const u32 *surv, u64 nsurv, const u32 *L,
-                    unsigned long long *best, Heavy *heavy, unsigned *nheavy,
-                    unsigned long long *nmark, unsigned *bad) {
-    unsigned blocks = (unsigned)((nsurv + 255) / 256);
-    hipLaunchKernelGGL(k_resolve<D>, dim3(blocks), dim3(256), 0, 0, P, surv, nsurv, L, best, heavy,
-                       nheavy, nmark, bad);
-}
-
-typedef void (*ResolveFn)(DegreeParams, const u32 *, u64, const u32 *, unsigned long long *,
-                          Heavy *, unsigned *, unsigned long long *, unsigned *);
-
-template <int D>
-struct ResolveTable {
-    static void fill(ResolveFn *tab) {
-        tab[D] = &launch_resolve<D>;
-        ResolveTable<D - 1>::fill(tab);
-    }
-};
-template <>
-struct ResolveTable<1> {
-    static void fill(ResolveFn *) {}
-};
-
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -345,16 +316,13 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
         gf2t::set_error("gf2t_sieve: need 2 <= dmin <= dmax <= 32, smax >= 1, dbest");
         return GF2T_EINVAL;
     }
-    ResolveFn resolve_tab[33] = {};
-    ResolveTable<32>::fill(resolve_tab);
-
-    DevBuf bL, bSurv, bHeavy, bTab, bCnt;
+    DevBuf bL, bSurv, bQueue, bHeavy, bTab, bCnt;
     int rc;
     if ((rc = bL.ensure(((size_t)1 << dmax) * sizeof(u32)))) return rc;
     if ((rc = bTab.ensure(2 * 65536 * sizeof(u32)))) return rc;
     if ((rc = bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
     if ((rc = bCnt.ensure(64))) return rc;
-    unsigned long long *cnt64 = (unsigned long long *)bCnt.p;   // [0] nsurv [1] nmark
+    unsigned long long *cnt64 = (unsigned long long *)bCnt.p;   // [0] nsurv [1] nqueue
     unsigned *cnt32 = (unsigned *)((char *)bCnt.p + 32);         // [0] nheavy [1] bad
     u32 *Alo = (u32 *)bTab.p, *Ahi = Alo + 65536;
     std::vector<u32> hlo(65536), hhi(65536);
@@ -385,7 +353,20 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
         }
         GF2T_HIP(hipMemcpy(Alo, hlo.data(), 65536 * 4, hipMemcpyHostToDevice));
         GF2T_HIP(hipMemcpy(Ahi, hhi.data(), 65536 * 4, hipMemcpyHostToDevice));
-        DegreeParams P{d, M, q, rmod, smax, Alo, Ahi};
+        // trace mask: bit i = Tr(x^i) = sum_k (x^i)^(2^k), which lies in GF(2)
+        u32 tmask = 0;
+        for (int i = 0; i < d; i++) {
+            u64 y = (u64)1 << i, acc = y;
+            for (int k = 1; k < d; k++) { y = hmulmod(y, y, q, d); acc ^= y; }
+            if (acc > 1) {
+                gf2t::set_error("gf2t_sieve: d=%d: trace not in GF(2)", d);
+                status = GF2T_EINTERNAL;
+                break;
+            }
+            tmask |= (u32)acc << i;
+        }
+        if (status) break;
+        DegreeParams P{d, M, q, rmod, smax, tmask, Alo, Ahi};
 
         const u64 ncls = irreducible_count(d);
         if ((rc = bSurv.ensure((ncls + 64) * sizeof(u32)))) { status = rc; break; }
@@ -413,15 +394,25 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
             status = GF2T_EINTERNAL;
             break;
         }
+        if ((rc = bQueue.ensure((ncls + 64) * sizeof(MarkRec)))) { status = rc; break; }
+        MarkRec *queue = (MarkRec *)bQueue.p;
         if (nsurv) {
-            resolve_tab[d](P, surv, nsurv, L, (unsigned long long *)dbest, heavy, cnt32, cnt64 + 1,
-                           cnt32 + 1);
-            GF2T_CHECK_LAUNCH("k_resolve");
+            u64 zb = std::min<u64>((nsurv + 255) / 256, 1u << 18);
+            hipLaunchKernelGGL(k_zech, dim3((unsigned)zb), dim3(256), 0, 0, P, surv, nsurv, L,
+                               queue, cnt64 + 1, (u64)(ncls + 64));
+            GF2T_CHECK_LAUNCH("k_zech");
+            GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
+            u64 nq = std::min<u64>(hs[1], ncls + 64);
+            if (nq) {
+                hipLaunchKernelGGL(k_mark, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, 0, P,
+                                   queue, nq, (unsigned long long *)dbest, heavy, cnt32,
+                                   cnt32 + 1);
+                GF2T_CHECK_LAUNCH("k_mark");
+            }
         }
         GF2T_HIP(hipEventRecord(ev[2], 0));
         unsigned h32[2];
         GF2T_HIP(hipMemcpy(h32, cnt32, 8, hipMemcpyDeviceToHost));
-        GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
         if (h32[1]) {
             gf2t::set_error("gf2t_sieve: d=%d: %s", d,
                             (h32[1] & 1) ? "minimal polynomial not over GF(2)"

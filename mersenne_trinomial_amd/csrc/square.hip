@@ -26,27 +26,20 @@ namespace {
 
 constexpr int SQ_THREADS = 256;
 
-GF2T_DEV void square_pair(const u64 *__restrict__ in, u64 *__restrict__ out,
-                          size_t nw, u64 r, u64 s, size_t i) {
-    u64 o0, o1;
-    square_fold_pair(in, nw, r, s, i, o0, o1);
-    out[2 * i] = o0;
-    if (2 * i + 1 < nw) out[2 * i + 1] = o1;
-}
-
 __global__ __launch_bounds__(SQ_THREADS) void k_square_fold(
-        const u64 *__restrict__ in, u64 *__restrict__ out, size_t nw, u64 r,
-        u64 s) {
+        const u64 *__restrict__ in, u64 *__restrict__ out, const SqConst c) {
     size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
-    if (i >= (nw + 1) / 2) return;
-    square_pair(in, out, nw, r, s, i);
+    if (i >= (c.nw + 1) / 2) return;
+    u64 o0, o1;
+    square_fold_pair_fast(in, c, i, o0, o1);
+    out[2 * i] = o0;
+    if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
 }
 
 struct SqDesc {
     const u64 *in;
     u64 *out;
-    u64 r, s;
-    u64 nw;
+    SqConst c;
 };
 
 // batched: blockIdx.y selects the residue, blockIdx.x the 512-word tile
@@ -54,8 +47,11 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
         const SqDesc *__restrict__ desc) {
     const SqDesc d = desc[blockIdx.y];
     size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
-    if (i >= (d.nw + 1) / 2) return;
-    square_pair(d.in, d.out, d.nw, d.r, d.s, i);
+    if (i >= (d.c.nw + 1) / 2) return;
+    u64 o0, o1;
+    square_fold_pair_fast(d.in, d.c, i, o0, o1);
+    d.out[2 * i] = o0;
+    if (2 * i + 1 < d.c.nw) d.out[2 * i + 1] = o1;
 }
 
 __global__ __launch_bounds__(SQ_THREADS) void k_fold(
@@ -105,9 +101,10 @@ extern "C" int gf2t_square_fold(const uint64_t *din, uint64_t *dout, uint64_t r,
         gf2t::set_error("gf2t_square_fold: need r >= 3, 1 <= s, 2s < r, din != dout");
         return GF2T_EINVAL;
     }
-    size_t nw = words_of(r), np = (nw + 1) / 2;
+    SqConst c = sq_const(r, s);
+    size_t np = (c.nw + 1) / 2;
     hipLaunchKernelGGL(k_square_fold, dim3(blocks_for(np, SQ_THREADS)), dim3(SQ_THREADS), 0,
-                       (hipStream_t)stream, din, dout, nw, r, s);
+                       (hipStream_t)stream, din, dout, c);
     GF2T_CHECK_LAUNCH("k_square_fold");
     return GF2T_OK;
 }
@@ -118,12 +115,13 @@ extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r
         gf2t::set_error("gf2t_square_fold_steps: bad arguments");
         return GF2T_EINVAL;
     }
-    size_t nw = words_of(r), np = (nw + 1) / 2;
+    SqConst c = sq_const(r, s);
+    size_t nw = c.nw, np = (nw + 1) / 2;
     hipStream_t st = (hipStream_t)stream;
     u64 *a = dres, *b = dtmp;
     for (u64 k = 0; k < steps; k++) {
         hipLaunchKernelGGL(k_square_fold, dim3(blocks_for(np, SQ_THREADS)), dim3(SQ_THREADS),
-                           0, st, a, b, nw, r, s);
+                           0, st, a, b, c);
         std::swap(a, b);
     }
     GF2T_CHECK_LAUNCH("k_square_fold");
@@ -178,9 +176,10 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
     size_t maxpairs = 0;
     for (int k = 0; k < n; k++) {
         int i = ord[k];
-        u64 nw = words_of(r[i]);
-        h[k] = SqDesc{dres[i], dtmp[i], r[i], s[i], nw};          // even steps
-        h[n + k] = SqDesc{dtmp[i], dres[i], r[i], s[i], nw};      // odd steps
+        SqConst c = sq_const(r[i], s[i]);
+        u64 nw = c.nw;
+        h[k] = SqDesc{dres[i], dtmp[i], c};          // even steps
+        h[n + k] = SqDesc{dtmp[i], dres[i], c};      // odd steps
         maxpairs = std::max(maxpairs, (size_t)(nw + 1) / 2);
     }
     GF2T_HIP(hipMemcpyAsync(g_batch.d, h.data(), need * sizeof(SqDesc), hipMemcpyHostToDevice, st));

@@ -12,6 +12,15 @@ void kl_square_fold(const u64 *in, u64 *out, u64 r, u64 s) {
         if (2 * i + 1 < nw) out[2 * i + 1] = o1;
     }
 }
+void kl_square_fold_fast(const u64 *in, u64 *out, u64 r, u64 s) {
+    SqConst c = sq_const(r, s);
+    for (size_t i = 0; i < (c.nw + 1) / 2; i++) {
+        u64 o0, o1;
+        square_fold_pair_fast(in, c, i, o0, o1);
+        out[2 * i] = o0;
+        if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
+    }
+}
 void kl_fold(const u64 *p, u64 L, u64 *out, u64 r, u64 s) {
     size_t nw = (size_t)((r + 63) >> 6), pw = (size_t)((L + 63) >> 6);
     for (size_t w = 0; w < nw; w++) out[w] = fold_word(p, pw, L, r, s, w);
@@ -20,4 +29,8 @@ u64 kl_gf64_mul(u64 a, u64 b) { return gf64_mul(a, b); }
 u64 kl_clmul32(u32 a, u32 b) { return clmul32(a, b); }
 u32 kl_fmul(u32 a, u32 b, u64 q, int d) { return fmul(a, b, q, d); }
 u64 kl_range_mask64(int lc, int hc) { return range_mask64(lc, hc); }
+u32 kl_gcd32(u32 a, u32 b) { return gcd32(a, b); }
+u32 kl_inv32(u32 a, u32 m) { return inv32(a, m); }
+u64 kl_mod_mersenne(u64 x, int d) { return mod_mersenne(x, d, ((u64)1 << d) - 1); }
+u64 kl_minpoly_bm(u32 beta, u64 q, int d, u32 tmask) { return minpoly_bm(beta, q, d, tmask); }
 }

@@ -22,6 +22,7 @@ def kl(tmp_path_factory):
                            os.path.join(HERE, "host", "kernel_logic.cpp")])
     lib = C.CDLL(out)
     lib.kl_square_fold.argtypes = [P, P, C.c_uint64, C.c_uint64]
+    lib.kl_square_fold_fast.argtypes = [P, P, C.c_uint64, C.c_uint64]
     lib.kl_fold.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint64]
     lib.kl_gf64_mul.restype = C.c_uint64
     lib.kl_gf64_mul.argtypes = [C.c_uint64, C.c_uint64]
@@ -31,6 +32,14 @@ def kl(tmp_path_factory):
     lib.kl_fmul.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int]
     lib.kl_range_mask64.restype = C.c_uint64
     lib.kl_range_mask64.argtypes = [C.c_int, C.c_int]
+    lib.kl_gcd32.restype = C.c_uint32
+    lib.kl_gcd32.argtypes = [C.c_uint32, C.c_uint32]
+    lib.kl_inv32.restype = C.c_uint32
+    lib.kl_inv32.argtypes = [C.c_uint32, C.c_uint32]
+    lib.kl_mod_mersenne.restype = C.c_uint64
+    lib.kl_mod_mersenne.argtypes = [C.c_uint64, C.c_int]
+    lib.kl_minpoly_bm.restype = C.c_uint64
+    lib.kl_minpoly_bm.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_uint32]
     return lib
 
 
@@ -62,7 +71,10 @@ def test_square_fold_all_s_small_r(kl):
             a = rand_res(rng, r)
             out = np.zeros(O.words(r), np.uint64)
             kl.kl_square_fold(ptr(a), ptr(out), r, s)
-            assert np.array_equal(out, O.square_fold(a, r, s)), (r, s)
+            want = O.square_fold(a, r, s)
+            assert np.array_equal(out, want), (r, s)
+            kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
+            assert np.array_equal(out, want), ("fast", r, s)
 
 
 def test_square_fold_window_edges(kl):
@@ -73,7 +85,20 @@ def test_square_fold_window_edges(kl):
             a = rand_res(rng, r)
             out = np.zeros(O.words(r), np.uint64)
             kl.kl_square_fold(ptr(a), ptr(out), r, s)
-            assert np.array_equal(out, O.square_fold(a, r, s)), (r, s)
+            want = O.square_fold(a, r, s)
+            assert np.array_equal(out, want), (r, s)
+            kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
+            assert np.array_equal(out, want), ("fast", r, s)
+
+
+def test_square_fold_fast_full_size_sample(kl):
+    rng = np.random.default_rng(9)
+    r = 136279841
+    for s in (474, 68139920, 1 << 20, 12345679):
+        a = rand_res(rng, r)
+        out = np.zeros(O.words(r), np.uint64)
+        kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
+        assert np.array_equal(out, O.square_fold(a, r, s)), s
 
 
 def test_fold_lengths(kl):
@@ -97,3 +122,73 @@ def test_field_multiplies(kl):
         q = O.find_primitive_poly(d)
         for a, b in rng.integers(0, 1 << d, (300, 2), dtype=np.uint64):
             assert kl.kl_fmul(int(a), int(b), q, d) == O.fmul(int(a), int(b), q, d)
+
+
+def test_sieve_number_theory(kl):
+    import math
+    rng = np.random.default_rng(5)
+    for d in (2, 3, 16, 31, 32):
+        M = (1 << d) - 1
+        for j in list(rng.integers(1, M, 300)) + [1, M - 1]:
+            j = int(j)
+            assert kl.kl_gcd32(j, M) == math.gcd(j, M)
+            g = math.gcd(j, M)
+            n = M // g
+            if n > 1:
+                assert kl.kl_inv32(j // g, n) == pow(j // g, -1, n)
+            x = int(rng.integers(0, 2**63)) * 2 + 1
+            assert kl.kl_mod_mersenne(x, d) == x % M
+
+
+def _trace_mask(q, d):
+    t = 0
+    for i in range(d):
+        y = 1 << i
+        acc, cur = y, y
+        for _ in range(d - 1):
+            cur = O.fmul(cur, cur, q, d)
+            acc ^= cur
+        assert acc in (0, 1)
+        t |= acc << i
+    return t
+
+
+def _minpoly_product(beta, q, d):
+    # prod over the d conjugates (the reference's formulation)
+    P = [1] + [0] * d
+    root = beta
+    for i in range(d):
+        for k in range(i + 1, 0, -1):
+            P[k] = P[k - 1] ^ O.fmul(root, P[k], q, d)
+        P[0] = O.fmul(root, P[0], q, d)
+        root = O.fmul(root, root, q, d)
+    assert all(c in (0, 1) for c in P)
+    return sum(c << k for k, c in enumerate(P))
+
+
+def test_minpoly_bm_equals_product(kl):
+    rng = np.random.default_rng(6)
+    for d in (2, 3, 4, 5, 8, 13, 20, 31, 32):
+        q = O.find_primitive_poly(d)
+        tm = _trace_mask(q, d)
+        M = (1 << d) - 1
+        g = 2
+        n = 0
+        for j in rng.integers(1, M, 60):
+            j = int(j)
+            # skip subfield elements (orbit shorter than d)
+            rot = [((j << i) | (j >> (d - i))) & M for i in range(1, d)]
+            if j in rot:
+                continue
+            beta = 1
+            e, b = j, 2
+            while e:
+                if e & 1:
+                    beta = O.fmul(beta, b, q, d)
+                b = O.fmul(b, b, q, d)
+                e >>= 1
+            want = _minpoly_product(beta, q, d)
+            assert kl.kl_minpoly_bm(beta, q, d, tm) == want, (d, j)
+            assert O.is_irreducible_small(want, d)
+            n += 1
+        assert n > 0
