@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-depth", type=int, default=24)
+    ap.add_argument("--mode", choices=["full", "squaring", "fft"], default="full",
+                    help="full: the default bench line; squaring: a plain copy of known size "
+                         "then only the batched squaring chain (for rocprofv3 counter runs); "
+                         "fft: the Cantor-product batch (64 seeded operand pairs of 2^16..2^27 "
+                         "bits, each product folded mod x^r + x^s + 1)")
     return ap.parse_args()
 
 
@@ -75,8 +80,108 @@ def cpu_baseline(depth: int, chain_s: int):
     }
 
 
+def mode_squaring(args):
+    """Copy of 16*nw*residues bytes (calibrates rocprofv3 FETCH/WRITE_SIZE
+    against a kernel of known traffic), then the batched chain alone."""
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer, Event
+    device.init(0)
+    nw = (R + 63) // 64
+    n = args.residues
+    res = [DeviceBuffer.words(nw) for _ in range(n)]
+    tmp = [DeviceBuffer.words(nw) for _ in range(n)]
+    big_a = DeviceBuffer.words(nw * n)
+    big_b = DeviceBuffer.words(nw * n)
+    e0, e1 = Event(), Event()
+    check(lib.gf2t_memcpy_d2d(big_b.ptr, big_a.ptr, 8 * nw * n, None))
+    device.synchronize()
+    e0.record()
+    check(lib.gf2t_memcpy_d2d(big_b.ptr, big_a.ptr, 8 * nw * n, None))
+    e1.record()
+    copy_ms = e0.elapsed_ms(e1)
+    del big_a, big_b
+    rng = np.random.default_rng(args.seed)
+    svals = np.sort(rng.choice(np.arange(3, SMAX + 1), size=n, replace=False)).astype(np.uint64)
+    for b in res:
+        check(lib.gf2t_set_x(b.ptr, nw, None))
+    P64 = C.POINTER(C.c_uint64)
+    rr = np.full(n, R, dtype=np.uint64)
+    kk = np.full(n, args.chain_steps, dtype=np.uint64)
+    cres = (C.c_void_p * n)(*[b.ptr for b in res])
+    ctmp = (C.c_void_p * n)(*[b.ptr for b in tmp])
+    e0.record()
+    check(lib.gf2t_square_fold_batch(cres, ctmp, rr.ctypes.data_as(P64), svals.ctypes.data_as(P64),
+                                     kk.ctypes.data_as(P64), n, None))
+    e1.record()
+    ms = e0.elapsed_ms(e1)
+    launch_ms = ms / args.chain_steps
+    gbs = 16 * nw * n / (launch_ms * 1e-3) / 1e9
+    print(json.dumps({"mode": "squaring", "residues": n, "steps": args.chain_steps,
+                      "avg_launch_ms": launch_ms, "GB_per_s": gbs, "frac": gbs / HBM_PEAK_GBS,
+                      "copy_bytes": 16 * nw * n, "copy_ms": copy_ms,
+                      "copy_GB_per_s": 16 * nw * n / (copy_ms * 1e-3) / 1e9}), flush=True)
+
+
+def c4_pairs(seed: int):
+    """64 operand pairs: sizes log-uniform integers in [2^16, 2^27] bits,
+    i.i.d. uniform bits, and a fold s in [1, 68139920] per pair."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(64):
+        ab = int(round(2 ** rng.uniform(16, 27)))
+        bb = int(round(2 ** rng.uniform(16, 27)))
+        s = int(rng.integers(1, SMAX + 1))
+        a = rng.integers(0, 2**64, (ab + 63) // 64, dtype=np.uint64)
+        b = rng.integers(0, 2**64, (bb + 63) // 64, dtype=np.uint64)
+        if ab & 63:
+            a[-1] &= np.uint64((1 << (ab & 63)) - 1)
+        if bb & 63:
+            b[-1] &= np.uint64((1 << (bb & 63)) - 1)
+        out.append((a, ab, b, bb, s))
+    return out
+
+
+def mode_fft(args):
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer, Event
+    device.init(0)
+    nw = (R + 63) // 64
+    pairs = c4_pairs(args.seed)
+    dev = []
+    wsmax = 0
+    for a, ab, b, bb, s in pairs:
+        dev.append((DeviceBuffer.from_array(a), ab, DeviceBuffer.from_array(b), bb, s,
+                    DeviceBuffer.words(nw)))
+        wsmax = max(wsmax, int(lib.gf2t_mul_workspace_bytes(ab, bb)))
+    work = DeviceBuffer(wsmax)
+
+    def run():
+        for da, ab, db, bb, s, do in dev:
+            check(lib.gf2t_mulmod(da.ptr, ab, db.ptr, bb, R, s, do.ptr, work.ptr, None),
+                  "gf2t_mulmod")
+
+    for _ in range(args.warmup):
+        run()
+    device.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(args.steps):
+        run()
+    e1.record()
+    ms = e0.elapsed_ms(e1) / args.steps
+    bits = sum(ab + bb for _, ab, _, bb, _ in pairs)
+    print(json.dumps({"mode": "fft", "products": 64, "ms_per_batch": ms,
+                      "products_per_s": 64 / (ms * 1e-3), "operand_bits": bits}), flush=True)
+
+
 def main():
     args = parse()
+    if args.mode == "squaring":
+        return mode_squaring(args)
+    if args.mode == "fft":
+        return mode_fft(args)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the deep scan shards by independent s values and the sieve by degree buckets; "

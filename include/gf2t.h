@@ -89,6 +89,14 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
                            const uint64_t *r, const uint64_t *s,
                            const uint64_t *steps, int n, void *stream);
 
+/* Slab form: residues i = 0..n-1 at din + i*pitch (pitch >= nw words),
+ * each squared once mod x^r + x^s_i + 1 into dout + i*pitch; ds is a
+ * DEVICE array of the n values s_i (each 1 <= s_i, 2 s_i < r).  The
+ * batched per-degree step of the deep scan (square_A over many s,
+ * src/trinomial_scan.cu:364-366). */
+int gf2t_square_fold_slab(const uint64_t *din, uint64_t *dout, size_t pitch,
+                          uint64_t r, const uint64_t *ds, int n, void *stream);
+
 /* Fold a product of src_bits <= 2r-1 bits (dsrc, words(src_bits) words)
  * into dout (nw words) modulo T.  Replaces TrinGPU::reduce / TsGPU::
  * reduce_into (two k_fold_pass launches, src/gf2_trinomial_cuda.h:63-74,
@@ -152,6 +160,19 @@ int gf2t_mulmod(const uint64_t *da, uint64_t abits, const uint64_t *db,
                 uint64_t bbits, uint64_t r, uint64_t s, uint64_t *dout,
                 void *dwork, void *stream);
 
+/* Slab form of the interval accumulation: for i = 0..n-1,
+ * out_i = a_i * (b_i + [b_add_x] x) mod x^r + x^s_i + 1, with a_i at
+ * da + i*apitch, b_i at db + i*bpitch, out_i at dout + i*opitch (r-bit
+ * operands; dout may equal da), ds a DEVICE array of s_i, dwork of
+ * gf2t_mulmod_slab_workspace_bytes(r, n) bytes.  The batched
+ * TrinGPU::accumulate (ACC <- ACC * (F + x), src/gf2_trinomial_cuda.h:
+ * 172-177) over many s in one set of launches. */
+size_t gf2t_mulmod_slab_workspace_bytes(uint64_t r, int n);
+int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_t *db,
+                     size_t bpitch, int b_add_x, uint64_t r, const uint64_t *ds,
+                     uint64_t *dout, size_t opitch, int n, void *dwork,
+                     void *stream);
+
 /* ------------------------------------------------------------------ *
  * (a) Zech-log coarse sieve
  * ------------------------------------------------------------------ */
@@ -176,6 +197,64 @@ int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax,
 /* Smallest primitive polynomial of degree d (find_primitive_poly,
  * src/gf2_field_setup.h:49-57); 0 if d is out of range. */
 uint64_t gf2t_primitive_poly(int d);
+
+/* ------------------------------------------------------------------ *
+ * Deep scan of sieve survivors (squaring chain + interval products on
+ * the device, interval gcds and factor resolution on host threads)
+ * ------------------------------------------------------------------ */
+#define GF2T_SCAN_FOUND 1      /* "s d p<hex>"                              */
+#define GF2T_SCAN_GAVE_UP 2    /* "s u": maxd reached below the Swan bound  */
+#define GF2T_SCAN_PRIMITIVE 3  /* "s primitive": no factor up to the bound  */
+#define GF2T_SCAN_RANGE 4      /* "s rA-B": factor in [A,B], unresolved     */
+
+typedef struct gf2t_scan_params {
+    uint64_t k0;      /* depth the coarse sieve reached (no factor <= k0) */
+    uint64_t maxd;    /* 0 = Swan bound (r/3 or r/2), else min(maxd, it) */
+    uint64_t L0;      /* first interval length (0 = 64)                  */
+    uint64_t Lmax;    /* interval length cap (0 = 4096)                  */
+    int gcd_threads;  /* host threads for interval gcds (0 = 8)          */
+    int batch;        /* max s values per device cohort (0 = by memory)  */
+    int verbose;
+} gf2t_scan_params;
+
+typedef struct gf2t_scan_result {
+    uint64_t s;
+    int kind;             /* GF2T_SCAN_* */
+    uint64_t d;           /* least factor degree (FOUND)                 */
+    uint64_t rlo, rhi;    /* RANGE bounds                                */
+    uint64_t gcd_degree;  /* degree of the interval gcd that hit         */
+    size_t hex_len;       /* length of the mask hex (FOUND)              */
+} gf2t_scan_result;
+
+typedef struct gf2t_scan_results gf2t_scan_results;
+
+/* Scan every s of svals (1 <= s, 2s < r, r <= 2^23).  Replaces
+ * trinomial_scan's per-s loop (scan_one_s, src/trinomial_scan.cu:240-381)
+ * and tsfactor's (src/tsfactor.cu:1089-1348) with cohorts of s values
+ * advanced together on the device; results follow tsfactor's kinds
+ * (src/tsfactor.cu:2480-2509).  Synchronous. */
+int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
+                  const gf2t_scan_params *params, gf2t_scan_results **out);
+size_t gf2t_scan_results_count(const gf2t_scan_results *res);
+int gf2t_scan_results_get(const gf2t_scan_results *res, size_t i,
+                          gf2t_scan_result *row, char *hexbuf, size_t cap);
+/* the certificate line of result i ("s d p<hex>", "s u", ...) */
+int gf2t_scan_line(const gf2t_scan_results *res, size_t i, char *buf,
+                   size_t cap);
+void gf2t_scan_results_free(gf2t_scan_results *res);
+
+/* Host gcd of two GF(2)[x] word arrays: degree (-1 if both zero) and up
+ * to gcap words of the monic gcd (poly_gcd_naive, src/gf2_gcd.h:48-75). */
+int64_t gf2t_gcd_host(const uint64_t *a, size_t aw, const uint64_t *b,
+                      size_t bw, uint64_t *g, size_t gcap);
+/* Numerically least irreducible factor of t, whose factors all have
+ * degree d (edf_least_ntl / edf_least_enum, src/gf2_gcd_ntl.cpp:44-61,
+ * src/gf2_gcd.h:137-145). */
+int gf2t_edf_least(const uint64_t *t, size_t tw, uint64_t d, uint64_t *out,
+                   size_t cap);
+/* Degree bound of the scan: r/3 under Swan's criterion, else r/2
+ * (src/tsfactor.cu:1101-1107); *swan (may be NULL) gets the criterion. */
+uint64_t gf2t_swan_bound(uint64_t r, uint64_t s, int *swan);
 
 #ifdef __cplusplus
 }

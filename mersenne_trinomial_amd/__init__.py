@@ -12,3 +12,5 @@ from .engine import (CantorBasis, DevFFT, GF2TError, TrinGPU, U64_MAX, bits_to_w
                      square_fold)
 
 __version__ = "0.1.0"
+from . import certs  # noqa: E402,F401
+from .scan import ScanOut, scan, swan_bound  # noqa: E402,F401

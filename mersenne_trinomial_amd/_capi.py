@@ -27,6 +27,16 @@ class GF2TError(RuntimeError):
         self.code = code
 
 
+class ScanParams(C.Structure):
+    _fields_ = [("k0", _u64), ("maxd", _u64), ("L0", _u64), ("Lmax", _u64),
+                ("gcd_threads", C.c_int), ("batch", C.c_int), ("verbose", C.c_int)]
+
+
+class ScanResult(C.Structure):
+    _fields_ = [("s", _u64), ("kind", C.c_int), ("d", _u64), ("rlo", _u64), ("rhi", _u64),
+                ("gcd_degree", _u64), ("hex_len", _sz)]
+
+
 class SieveStats(C.Structure):
     _fields_ = [("d", C.c_int), ("q", _u64), ("classes", _u64), ("marking", _u64),
                 ("heavy", _u64), ("ms_table", C.c_float), ("ms_classes", C.c_float),
@@ -58,6 +68,10 @@ _SIGS = {
     "gf2t_square_fold_steps": (C.c_int, [_vp, _vp, _u64, _u64, _u64, _vp]),
     "gf2t_square_fold_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _p64, _p64, _p64,
                                          C.c_int, _vp]),
+    "gf2t_square_fold_slab": (C.c_int, [_vp, _vp, _sz, _u64, _vp, C.c_int, _vp]),
+    "gf2t_mulmod_slab_workspace_bytes": (_sz, [_u64, C.c_int]),
+    "gf2t_mulmod_slab": (C.c_int, [_vp, _sz, _vp, _sz, C.c_int, _u64, _vp, _vp, _sz, C.c_int,
+                                   _vp, _vp]),
     "gf2t_fold": (C.c_int, [_vp, _u64, _vp, _u64, _u64, _vp]),
     "gf2t_mulx_xor": (C.c_int, [_vp, _vp, _u64, _u64, _vp]),
     "gf2t_set_x": (C.c_int, [_vp, _sz, _vp]),
@@ -75,6 +89,14 @@ _SIGS = {
     "gf2t_mulmod": (C.c_int, [_vp, _u64, _vp, _u64, _u64, _u64, _vp, _vp, _vp]),
     "gf2t_sieve": (C.c_int, [_u64, C.c_int, C.c_int, _u64, _vp, C.POINTER(SieveStats)]),
     "gf2t_primitive_poly": (_u64, [C.c_int]),
+    "gf2t_scan_run": (C.c_int, [_u64, _p64, _sz, C.POINTER(ScanParams), C.POINTER(_vp)]),
+    "gf2t_scan_results_count": (_sz, [_vp]),
+    "gf2t_scan_results_get": (C.c_int, [_vp, _sz, C.POINTER(ScanResult), C.c_char_p, _sz]),
+    "gf2t_scan_line": (C.c_int, [_vp, _sz, C.c_char_p, _sz]),
+    "gf2t_scan_results_free": (None, [_vp]),
+    "gf2t_gcd_host": (C.c_int64, [_p64, _sz, _p64, _sz, _p64, _sz]),
+    "gf2t_edf_least": (C.c_int, [_p64, _sz, _u64, _p64, _sz]),
+    "gf2t_swan_bound": (_u64, [_u64, _u64, C.POINTER(C.c_int)]),
 }
 
 

@@ -19,7 +19,10 @@ LIB = os.path.join(PKG, "libgf2t.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["capi.hip", "square.hip", "fft.hip", "sieve.hip"]
-HOST_SOURCES = ["host.cpp"]
+HOST_SOURCES = ["host_poly.cpp", "scan.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOSTFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-mpclmul", "-msse4.1", "-pthread",
+             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
@@ -41,10 +44,8 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(OBJ, src + ".o")
         jobs.append([HIPCC, *COMMON, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     for src in HOST_SOURCES:
-        path = os.path.join(CSRC, src)
-        if os.path.exists(path):
-            obj = os.path.join(OBJ, src + ".o")
-            jobs.append([HIPCC, *COMMON, "-c", path, "-o", obj])
+        obj = os.path.join(OBJ, src + ".o")
+        jobs.append([CXX, *HOSTFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     objs = [j[-1] for j in jobs]

@@ -78,10 +78,15 @@ GF2T_DEV u64 packed_chunk(const u32 *__restrict__ src, u64 nchunks, u32 lastmask
     return v;
 }
 
+// blockIdx.y selects one transform of a batch: element y lives at
+// f + y*f_stride and packs from pack_src + y*pack_stride (u32 units);
+// pack_xor0 is XORed into element 0 (adds x to the packed operand).
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
         u64 *__restrict__ f, const PassDesc pd, const u32 *__restrict__ pack_src,
-        u64 pack_nchunks, u32 pack_lastmask) {
+        u64 pack_nchunks, u32 pack_lastmask, u64 f_stride, u64 pack_stride, u32 pack_xor0) {
     extern __shared__ u64 lds[];
+    f += (u64)blockIdx.y * f_stride;
+    if (pack_src) pack_src += (u64)blockIdx.y * pack_stride;
     TwTables &tw = *reinterpret_cast<TwTables *>(lds);
     u64 *tile = lds + 4 * 256;
     const int T = pd.c + pd.w;
@@ -105,7 +110,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     // load the tile (low c bits contiguous in memory)
     for (u64 li = threadIdx.x; li < tsize; li += FFT_THREADS) {
         u64 g = global_index(pd, li, rest_lo, rest_hi);
-        tile[li] = pack_src ? packed_chunk(pack_src, pack_nchunks, pack_lastmask, g) : f[g];
+        u64 v = pack_src ? packed_chunk(pack_src, pack_nchunks, pack_lastmask, g) : f[g];
+        if (pack_src && g == 0) v ^= pack_xor0;
+        tile[li] = v;
     }
     __syncthreads();
 
@@ -155,10 +162,13 @@ __global__ void k_pointwise(u64 *__restrict__ fa, const u64 *__restrict__ fb, si
 }
 
 // overlap-add in gather form: output word w collects chunks 2w-1, 2w, 2w+1
+// (blockIdx.y: batch element, F + y*f_stride -> out + y*o_stride)
 __global__ void k_overlap_add(const u64 *__restrict__ F, size_t nchunks, u64 *__restrict__ out,
-                              size_t owords) {
+                              size_t owords, u64 f_stride, u64 o_stride) {
     size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= owords) return;
+    F += (u64)blockIdx.y * f_stride;
+    out += (u64)blockIdx.y * o_stride;
     size_t k = 2 * w;
     u64 v = 0;
     if (k < nchunks) v ^= F[k];
@@ -343,7 +353,8 @@ const gf2t_fft_plan *cached_plan(int m) {
 }
 
 int run_passes(const std::vector<PassDesc> &ps, u64 *df, hipStream_t st, const u32 *pack_src,
-               u64 nchunks, u32 lastmask) {
+               u64 nchunks, u32 lastmask, unsigned nbatch = 1, u64 f_stride = 0,
+               u64 pack_stride = 0, u32 pack_xor0 = 0) {
     static std::once_flag lds_once;
     static hipError_t lds_err = hipSuccess;
     std::call_once(lds_once, [] {
@@ -358,8 +369,8 @@ int run_passes(const std::vector<PassDesc> &ps, u64 *df, hipStream_t st, const u
         unsigned blocks = (unsigned)(1ULL << (pd.m - T));
         size_t lds = (4 * 256 + ((size_t)1 << T)) * sizeof(u64);
         const u32 *src = (i == 0) ? pack_src : nullptr;
-        hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(FFT_THREADS), lds, st, df, pd, src,
-                           nchunks, lastmask);
+        hipLaunchKernelGGL(k_fft_pass, dim3(blocks, nbatch), dim3(FFT_THREADS), lds, st, df, pd,
+                           src, nchunks, lastmask, f_stride, pack_stride, pack_xor0);
     }
     GF2T_CHECK_LAUNCH("k_fft_pass");
     return GF2T_OK;
@@ -463,7 +474,7 @@ int mul_core(const u64 *da, u64 abits, const u64 *db, u64 bbits, u64 *dprod, voi
     rc = run_passes(plan->inv, FA, st, nullptr, 0, 0);
     if (rc) return rc;
     hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0, st, FA,
-                       (size_t)(ca + cb - 1), dprod, ow);
+                       (size_t)(ca + cb - 1), dprod, ow, (u64)0, (u64)0);
     GF2T_CHECK_LAUNCH("k_overlap_add");
     return GF2T_OK;
 }
@@ -501,4 +512,68 @@ extern "C" int gf2t_mulmod(const uint64_t *da, uint64_t abits, const uint64_t *d
     u64 pbits = abits + bbits ? abits + bbits - 1 : 0;
     if (pbits < 1) pbits = 1;
     return gf2t_fold(prod, pbits, dout, r, s, stream);
+}
+
+// ------------------------------------------------------------------ slabs
+namespace {
+__global__ void k_fold_slab(const u64 *__restrict__ p, size_t pw, u64 p_stride, u64 L,
+                            u64 *__restrict__ out, u64 o_stride, size_t nw, u64 r,
+                            const u64 *__restrict__ svec) {
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const u64 s = svec[blockIdx.y];
+    p += (u64)blockIdx.y * p_stride;
+    out += (u64)blockIdx.y * o_stride;
+    out[w] = fold_word(p, pw, L, r, s, w);
+}
+}  // namespace
+
+extern "C" size_t gf2t_mulmod_slab_workspace_bytes(uint64_t r, int n) {
+    u64 ca = (r + 31) >> 5;
+    int m = fft_m_for(2 * ca - 1);
+    size_t per = 2 * ((size_t)1 << m) + 2 * words_of(r) + 2;
+    return per * (size_t)(n > 0 ? n : 0) * sizeof(u64);
+}
+
+extern "C" int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_t *db,
+                                size_t bpitch, int b_add_x, uint64_t r, const uint64_t *ds,
+                                uint64_t *dout, size_t opitch, int n, void *dwork, void *stream) {
+    if (!da || !db || !dout || !ds || !dwork || n < 0 || r < 3) {
+        gf2t::set_error("gf2t_mulmod_slab: bad arguments");
+        return GF2T_EINVAL;
+    }
+    if (n == 0) return GF2T_OK;
+    size_t nw = words_of(r);
+    if (apitch < nw || bpitch < nw || opitch < nw || n > 65535) {
+        gf2t::set_error("gf2t_mulmod_slab: pitch below nw or more than 65535 residues");
+        return GF2T_EINVAL;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc = ensure_basis();
+    if (rc) return rc;
+    u64 ca = (r + 31) >> 5;
+    int m = fft_m_for(2 * ca - 1);
+    if (m > FFT_MAX_M) { gf2t::set_error("gf2t_mulmod_slab: r too large"); return GF2T_ERANGE; }
+    const gf2t_fft_plan *plan = cached_plan(m);
+    size_t N = (size_t)1 << m;
+    u64 *FA = (u64 *)dwork, *FB = FA + N * n, *prod = FB + N * n;
+    size_t ppitch = 2 * nw + 2;
+    u32 lm = (r & 31) ? (u32)((1ULL << (r & 31)) - 1) : 0xFFFFFFFFu;
+    rc = run_passes(plan->fwd, FA, st, (const u32 *)da, ca, lm, (unsigned)n, N, 2 * apitch, 0);
+    if (rc) return rc;
+    rc = run_passes(plan->fwd, FB, st, (const u32 *)db, ca, lm, (unsigned)n, N, 2 * bpitch,
+                    b_add_x ? 2u : 0u);
+    if (rc) return rc;
+    size_t tot = N * (size_t)n;
+    unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_pointwise, dim3(blocks), dim3(256), 0, st, FA, FB, tot);
+    rc = run_passes(plan->inv, FA, st, nullptr, 0, 0, (unsigned)n, N, 0, 0);
+    if (rc) return rc;
+    size_t ow = 2 * nw;   // product of two r-bit operands: 2r-1 bits
+    hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256), n), dim3(256), 0, st, FA,
+                       (size_t)(2 * ca - 1), prod, ow, (u64)N, (u64)ppitch);
+    hipLaunchKernelGGL(k_fold_slab, dim3((unsigned)((nw + 255) / 256), n), dim3(256), 0, st, prod,
+                       ow, (u64)ppitch, 2 * r - 1, dout, (u64)opitch, nw, r, ds);
+    GF2T_CHECK_LAUNCH("gf2t_mulmod_slab");
+    return GF2T_OK;
 }

@@ -1,0 +1,48 @@
+// host_poly.hpp -- GF(2)[x] arithmetic on the host CPU for the parts of
+// the search that stay on the host in the reference too: interval gcds
+// against T, least-degree resolution and equal-degree factorization of
+// the (small) interval gcd, certificate hex (reference: src/gf2_gcd.h:
+// 48-145, src/tsfactor.cu:932-987).  Polynomials are little-endian u64
+// word vectors; the carry-less products use PCLMULQDQ.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+
+namespace gf2t_host {
+
+typedef uint64_t u64;
+typedef std::vector<u64> Poly;
+
+int64_t deg(const Poly &a);
+int64_t deg_w(const u64 *a, size_t n);
+void trim(Poly &a);
+bool is_zero(const Poly &a);
+bool less(const Poly &a, const Poly &b);          // numeric order
+bool equal(const Poly &a, const Poly &b);
+void xor_shift(Poly &a, const Poly &b, u64 sh);   // a ^= b << sh (a grown as needed)
+Poly mul(const Poly &a, const Poly &b);
+Poly sqr(const Poly &a);
+void mod_inplace(Poly &a, const Poly &g);         // a <- a mod g (deg g >= 0)
+Poly divexact(const Poly &a, const Poly &g);      // a / g, remainder must be 0
+Poly gcd(const Poly &a, const Poly &b);           // monic; gcd(0,0) = 0
+Poly mulmod(const Poly &a, const Poly &b, const Poly &g);
+Poly sqrmod(const Poly &a, const Poly &g);
+std::string hex(const Poly &a);                   // lowercase, no leading zero
+Poly from_words(const u64 *w, size_t n);
+Poly trinomial(u64 r, u64 s);
+
+// least factor degree d in (ka-1, kb] of g, whose irreducible factors all
+// have degrees in [ka, kb], and the numerically least degree-d factor.
+// h0 = x^(2^(ka-1)) mod g.  Returns false if nothing is found.
+bool resolve_least(const Poly &g, u64 ka, u64 kb, const Poly &h0, u64 &d, Poly &mask);
+
+// numerically least irreducible factor of t, all of whose irreducible
+// factors have degree exactly d (t squarefree)
+Poly edf_least(const Poly &t, u64 d);
+
+// Swan bound of the reference (src/tsfactor.cu:1101-1107)
+u64 swan_rhigh(u64 r, u64 s, bool *swan);
+
+}  // namespace gf2t_host

@@ -54,6 +54,21 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
     if (2 * i + 1 < d.c.nw) d.out[2 * i + 1] = o1;
 }
 
+// slab: residue y at in + y*pitch, its s in svec[y]
+__global__ __launch_bounds__(SQ_THREADS) void k_square_fold_slab(
+        const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
+        const u64 *__restrict__ svec) {
+    const SqConst c = sq_const(r, svec[blockIdx.y]);
+    size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
+    if (i >= (c.nw + 1) / 2) return;
+    in += (u64)blockIdx.y * pitch;
+    out += (u64)blockIdx.y * pitch;
+    u64 o0, o1;
+    square_fold_pair_fast(in, c, i, o0, o1);
+    out[2 * i] = o0;
+    if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
+}
+
 __global__ __launch_bounds__(SQ_THREADS) void k_fold(
         const u64 *__restrict__ p, size_t pw, u64 L, u64 *__restrict__ out,
         size_t nw, u64 r, u64 s) {
@@ -247,5 +262,20 @@ extern "C" int gf2t_xor_x(uint64_t *dp, void *stream) {
     if (!dp) { gf2t::set_error("gf2t_xor_x: null pointer"); return GF2T_EINVAL; }
     hipLaunchKernelGGL(k_xor_word0, dim3(1), dim3(64), 0, (hipStream_t)stream, dp, 2ULL);
     GF2T_CHECK_LAUNCH("k_xor_word0");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_square_fold_slab(const uint64_t *din, uint64_t *dout, size_t pitch,
+                                     uint64_t r, const uint64_t *ds, int n, void *stream) {
+    size_t nw = words_of(r);
+    if (!din || !dout || !ds || din == dout || r < 3 || pitch < nw || n < 0 || n > 65535) {
+        gf2t::set_error("gf2t_square_fold_slab: bad arguments");
+        return GF2T_EINVAL;
+    }
+    if (n == 0) return GF2T_OK;
+    size_t np = (nw + 1) / 2;
+    hipLaunchKernelGGL(k_square_fold_slab, dim3(blocks_for(np, SQ_THREADS), n), dim3(SQ_THREADS), 0,
+                       (hipStream_t)stream, din, dout, (u64)pitch, r, ds);
+    GF2T_CHECK_LAUNCH("k_square_fold_slab");
     return GF2T_OK;
 }

@@ -66,6 +66,8 @@ def _load():
         "gf2o_mulmod": (C.c_int, [_p64, _u64, _p64, _u64, _u64, _u64, _p64]),
         "gf2o_residue": (_u64, [_p64, _u64, _u64, C.c_int]),
         "gf2o_find_irreducible": (_u64, [C.c_int, _u64]),
+        "gf2o_edf_least": (C.c_size_t, [_p64, C.c_size_t, _u64, _p64, C.c_size_t]),
+        "gf2o_scan_one": (C.c_int, [_u64, _u64, _u64, _u64, _p64, _p64, C.c_size_t, _p64]),
         "gf2o_deg": (_i64, [_p64, C.c_size_t]),
         "gf2o_gcd": (_i64, [_p64, C.c_size_t, _p64, C.c_size_t, _p64, C.c_size_t]),
         "gf2o_verify_factor": (C.c_int, [_u64, _u64, _p64, C.c_size_t]),
@@ -295,6 +297,27 @@ def verify_factor(r, s, P: np.ndarray) -> bool:
     the check-ntl verifier's criterion (src/check-ntl.c:17-58)."""
     P = np.ascontiguousarray(P, dtype=np.uint64)
     return bool(lib().gf2o_verify_factor(r, s, _ptr(P), len(P)))
+
+
+def edf_least(t: np.ndarray, d: int) -> np.ndarray:
+    t = np.ascontiguousarray(t, dtype=np.uint64)
+    cap = words(d + 1)
+    out = np.zeros(cap, dtype=np.uint64)
+    n = lib().gf2o_edf_least(_ptr(t), len(t), d, _ptr(out), cap)
+    if n == 0:
+        raise RuntimeError("no factor found")
+    return out
+
+
+def scan_one(r: int, s: int, k0: int, maxd: int):
+    """(kind, d, mask_hex, gcd_degree): kind 0 = not a survivor,
+    1 = found, 2 = no factor up to maxd."""
+    cap = words(maxd + 1) + 1
+    mask = np.zeros(cap, dtype=np.uint64)
+    d = np.zeros(1, dtype=np.uint64)
+    gd = np.zeros(1, dtype=np.uint64)
+    kind = lib().gf2o_scan_one(r, s, k0, maxd, _ptr(d), _ptr(mask), cap, _ptr(gd))
+    return kind, int(d[0]), (poly_hex(mask) if kind == 1 else None), int(gd[0])
 
 
 def trinomial(r: int, s: int) -> np.ndarray:

@@ -821,3 +821,127 @@ int gf2o_verify_factor(uint64_t r, uint64_t s, const uint64_t *P, size_t pw) {
     free(snap); free(h); free(t);
     return ok;
 }
+
+/* ======================================================================
+ * Equal-degree factorization and the per-degree scan oracle
+ * ==================================================================== */
+/* (a * b) mod p for word arrays of pw words (p of degree dp) */
+static void mulmod_words(const uint64_t *a, const uint64_t *b, const uint64_t *p, int64_t dp,
+                         uint64_t *out, size_t pw) {
+    size_t tw = 2 * pw + 2;
+    uint64_t *t = (uint64_t *)calloc(tw, 8);
+    gf2o_mul_comb(a, (uint64_t)pw * 64, b, (uint64_t)pw * 64, t);
+    pmod_words(t, tw, p, dp);
+    memcpy(out, t, pw * 8);
+    free(t);
+}
+
+static int64_t edf_rec(const uint64_t *t, size_t tw, uint64_t d, uint64_t *best, size_t cap,
+                       int64_t *best_set) {
+    int64_t dt = gf2o_deg(t, tw);
+    if (dt <= 0) return 0;
+    size_t w = WORDS((uint64_t)dt + 1);
+    if ((uint64_t)dt == d) {
+        /* keep the numerically least */
+        int less = !*best_set;
+        if (!less) {
+            for (size_t i = cap; i-- > 0;) {
+                uint64_t x = i < w ? t[i] : 0;
+                if (x != best[i]) { less = x < best[i]; break; }
+            }
+        }
+        if (less) { memset(best, 0, cap * 8); memcpy(best, t, (w < cap ? w : cap) * 8); *best_set = 1; }
+        return 1;
+    }
+    size_t pw = WORDS((uint64_t)dt);
+    uint64_t *a = (uint64_t *)calloc(pw + 1, 8), *tr = (uint64_t *)calloc(pw + 1, 8);
+    uint64_t *p = (uint64_t *)calloc(pw + 1, 8), *g = (uint64_t *)calloc(w + 1, 8);
+    uint64_t *tt = (uint64_t *)calloc(w + 1, 8);
+    memcpy(tt, t, w * 8);
+    int64_t ok = 0;
+    for (uint64_t i = 1; i < 4 * (uint64_t)dt + 64 && !ok; i++) {
+        /* a = x^i + 1 reduced mod t */
+        uint64_t *e = (uint64_t *)calloc(WORDS(i + 1) + pw + 2, 8);
+        e[i >> 6] |= 1ULL << (i & 63);
+        e[0] ^= 1;
+        pmod_words(e, WORDS(i + 1) + pw + 2, tt, dt);
+        memcpy(a, e, pw * 8);
+        free(e);
+        memcpy(tr, a, pw * 8);
+        memcpy(p, a, pw * 8);
+        for (uint64_t k = 1; k < d; k++) {
+            mulmod_words(p, p, tt, dt, p, pw);
+            for (size_t j = 0; j < pw; j++) tr[j] ^= p[j];
+        }
+        int64_t dg = gf2o_gcd(tr, pw, tt, w, g, w + 1);
+        if (dg > 0 && dg < dt) {
+            /* split: g and t / g */
+            size_t gw = WORDS((uint64_t)dg + 1);
+            uint64_t *rem = (uint64_t *)calloc(w + 2, 8), *q = (uint64_t *)calloc(w + 2, 8);
+            memcpy(rem, tt, w * 8);
+            int64_t dr = dt;
+            while (dr >= dg) {
+                uint64_t sh = (uint64_t)(dr - dg);
+                q[sh >> 6] |= 1ULL << (sh & 63);
+                pxor_shift(rem, w + 2, g, dg, sh);
+                dr = gf2o_deg(rem, w + 2);
+            }
+            edf_rec(g, gw, d, best, cap, best_set);
+            edf_rec(q, w + 1, d, best, cap, best_set);
+            free(rem); free(q);
+            ok = 1;
+        }
+    }
+    free(a); free(tr); free(p); free(g); free(tt);
+    return ok;
+}
+
+size_t gf2o_edf_least(const uint64_t *t, size_t tw, uint64_t d, uint64_t *out, size_t cap) {
+    int64_t dt = gf2o_deg(t, tw);
+    if (dt < (int64_t)d || d == 0) return 0;
+    memset(out, 0, cap * 8);
+    if (d <= 24) {
+        for (uint64_t m = (1ULL << d) | 1; m < (2ULL << d); m += 2) {
+            if (!gf2o_is_irreducible_small(m, (int)d)) continue;
+            uint64_t mv[1] = {m};
+            uint64_t g[2] = {0, 0};
+            if (gf2o_gcd(mv, 1, t, tw, g, 2) == (int64_t)d) { out[0] = m; return 1; }
+        }
+        return 0;
+    }
+    int64_t set = 0;
+    edf_rec(t, tw, d, out, cap, &set);
+    return set ? WORDS(d + 1) : 0;
+}
+
+int gf2o_scan_one(uint64_t r, uint64_t s, uint64_t k0, uint64_t maxd, uint64_t *d,
+                  uint64_t *mask, size_t cap, uint64_t *gdeg) {
+    size_t nw = WORDS(r), tw = WORDS(r + 1);
+    uint64_t *T = (uint64_t *)calloc(tw, 8), *F = (uint64_t *)calloc(nw, 8);
+    uint64_t *Fx = (uint64_t *)calloc(nw, 8), *g = (uint64_t *)calloc(tw + 1, 8);
+    T[0] |= 1; T[s >> 6] |= 1ULL << (s & 63); T[r >> 6] |= 1ULL << (r & 63);
+    F[0] = 2;
+    int kind = 2;
+    for (uint64_t k = 1; k <= maxd; k++) {
+        gf2o_square_fold_steps(F, r, s, 1);
+        memcpy(Fx, F, nw * 8);
+        Fx[0] ^= 2;
+        int64_t gd = gf2o_gcd(Fx, nw, T, tw, g, tw + 1);
+        if (gd > 0) {
+            if (k <= k0) { kind = 0; break; }
+            *d = k;
+            if (gdeg) *gdeg = (uint64_t)gd;
+            if ((uint64_t)gd == k) {
+                memset(mask, 0, cap * 8);
+                size_t gw = WORDS((uint64_t)gd + 1);
+                memcpy(mask, g, (gw < cap ? gw : cap) * 8);
+            } else {
+                gf2o_edf_least(g, WORDS((uint64_t)gd + 1), k, mask, cap);
+            }
+            kind = 1;
+            break;
+        }
+    }
+    free(T); free(F); free(Fx); free(g);
+    return kind;
+}

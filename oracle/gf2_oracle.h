@@ -108,6 +108,20 @@ int64_t gf2o_gcd(const uint64_t *a, size_t aw, const uint64_t *b,
 int gf2o_verify_factor(uint64_t r, uint64_t s, const uint64_t *P,
                        size_t pw);
 
+/* numerically least irreducible factor of t whose factors all have
+ * degree d: brute-force enumeration of degree-d masks for d <= 24 (the
+ * reference's edf_least_enum, src/gf2_gcd.h:137-145), trace splitting
+ * with the elements x^i + 1 above that.  Returns words written (0 if
+ * none). */
+size_t gf2o_edf_least(const uint64_t *t, size_t tw, uint64_t d, uint64_t *out, size_t cap);
+
+/* Per-degree deep-scan oracle (reference: oracle_scan,
+ * src/trinomial_scan.cu:393-421): F_k = x^(2^k) mod T, first k with
+ * gcd(F_k + x, T) != 1.  kind: 0 not a survivor (hit at k <= k0),
+ * 1 found (d, least mask in mask[]), 2 none up to maxd.  Returns kind. */
+int gf2o_scan_one(uint64_t r, uint64_t s, uint64_t k0, uint64_t maxd, uint64_t *d,
+                  uint64_t *mask, size_t cap, uint64_t *gdeg);
+
 #ifdef __cplusplus
 }
 #endif

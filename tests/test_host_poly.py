@@ -1,0 +1,101 @@
+"""Host-side GF(2)[x] pieces of the deep scan (gcd, equal-degree
+factorization, Swan bound) and the certificate formats, against the
+oracle.  No GPU: these run on the host CPU through the C ABI."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+P = C.POINTER(C.c_uint64)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from mersenne_trinomial_amd._capi import lib
+    return lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(P)
+
+
+def rand_poly(rng, bits):
+    a = rng.integers(0, 2**64, O.words(bits), dtype=np.uint64)
+    if bits & 63:
+        a[-1] &= np.uint64((1 << (bits & 63)) - 1)
+    a[-1] |= np.uint64(1 << ((bits - 1) & 63))
+    return a
+
+
+def test_gcd_host_vs_oracle(lib):
+    rng = np.random.default_rng(1)
+    for _ in range(30):
+        g = rand_poly(rng, int(rng.integers(2, 300)))
+        a = O.mul_comb(g, len(g) * 64, rand_poly(rng, 2000), 2048)
+        b = O.mul_comb(g, len(g) * 64, rand_poly(rng, 1500), 1536)
+        cap = max(len(a), len(b)) + 1
+        out = np.zeros(cap, dtype=np.uint64)
+        d = lib.gf2t_gcd_host(ptr(a), len(a), ptr(b), len(b), ptr(out), cap)
+        d2, g2 = O.gcd(a, b)
+        assert d == d2
+        assert np.array_equal(out[: len(g2)], g2)
+    # a full-size trinomial gcd at r = 132049 against a random residue
+    r, s = 132049, 1201
+    t = O.trinomial(r, s)
+    a = rand_poly(rng, r - 5)
+    out = np.zeros(len(t) + 1, dtype=np.uint64)
+    d = lib.gf2t_gcd_host(ptr(a), len(a), ptr(t), len(t), ptr(out), len(out))
+    assert d == O.gcd(a, t)[0]
+
+
+def irreducibles(d, count, rng):
+    found = []
+    while len(found) < count:
+        m = int(rng.integers(1 << d, 2 << d)) | 1
+        if O.is_irreducible_small(m, d) and m not in found:
+            found.append(m)
+    return found
+
+
+@pytest.mark.parametrize("d,k", [(5, 2), (13, 3), (24, 2), (31, 2), (40, 3), (61, 2)])
+def test_edf_least(lib, d, k):
+    rng = np.random.default_rng(d * 10 + k)
+    fs = irreducibles(d, k, rng)
+    t = np.array([1], dtype=np.uint64)
+    for f in fs:
+        t = O.mul_comb(t, len(t) * 64, np.array([f], dtype=np.uint64), 64)
+        t = t[: O.words(O.deg(t) + 1)]
+    out = np.zeros(O.words(d + 1) + 1, dtype=np.uint64)
+    assert lib.gf2t_edf_least(ptr(t), len(t), d, ptr(out), len(out)) == 0
+    assert int(out[0]) == min(fs)
+    assert int(O.edf_least(t, d)[0]) == min(fs)
+
+
+def test_swan_bound_values():
+    import mersenne_trinomial_amd as M
+    assert M.swan_bound(132049, 2) == (66024, False)      # s even, s | 2r
+    assert M.swan_bound(132049, 5)[0] == 132049 // 3
+    assert M.swan_bound(132049, 4)[0] == 132049 // 3      # 4 does not divide 2r
+    assert M.swan_bound(4423, 1)[0] == 4423 // 3          # 4423 = 7 (mod 8), (r-s) does not divide 2r
+    assert M.swan_bound(4229, 10)[0] == 4229 // 2         # 4229 = 5 (mod 8): no Swan bound
+
+
+def test_cert_formats(tmp_path):
+    from mersenne_trinomial_amd import certs
+    best = np.full(11, certs.NONE, dtype=np.uint64)
+    best[3] = (2 << 48) | 7
+    best[7] = (5 << 48) | 0x29
+    best[9] = 0
+    lines = certs.sieve_lines(best)
+    assert lines == ["3 2 p7", "7 5 p29"]
+    out = certs.write_sieve_outputs(str(tmp_path / "x"), best, found_bin=True)
+    assert open(out["cert"]).read().split("\n")[:2] == lines
+    assert list(certs.read_survivors(out["survivors"])) == [1, 2, 4, 5, 6, 8, 10]
+    back = certs.read_found_bin(out["found_bin"], 10)
+    assert np.array_equal(back[1:], best[1:])
+    assert certs.parse_line("474 41 p3991ef16129") == (474, "p", 41, "3991ef16129")
+    assert certs.parse_line("12 u") == (12, "u", None, None)
+    seeded = certs.best_from_survivors([2, 5], 6, 30)
+    assert seeded[2] == certs.NONE and seeded[5] == certs.NONE and seeded[3] == np.uint64(29 << 48)
