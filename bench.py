@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-depth", type=int, default=24)
+    ap.add_argument("--sq-ppt", type=int, default=0, help="override the squaring pairs/lane")
+    ap.add_argument("--sq-order", type=int, default=-1, help="override the squaring tile order")
+    ap.add_argument("--sq-nt", type=int, default=-1, help="override streaming stores (0/1)")
     ap.add_argument("--mode", choices=["full", "squaring", "fft"], default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
@@ -94,12 +97,18 @@ def mode_squaring(args):
     big_a = DeviceBuffer.words(nw * n)
     big_b = DeviceBuffer.words(nw * n)
     e0, e1 = Event(), Event()
-    check(lib.gf2t_memcpy_d2d(big_b.ptr, big_a.ptr, 8 * nw * n, None))
+    nbytes = 8 * nw * n
+    nbytes16 = nbytes & ~15
+    check(lib.gf2t_stream_copy(big_b.ptr, big_a.ptr, nbytes16, None))
     device.synchronize()
     e0.record()
-    check(lib.gf2t_memcpy_d2d(big_b.ptr, big_a.ptr, 8 * nw * n, None))
+    check(lib.gf2t_stream_copy(big_b.ptr, big_a.ptr, nbytes16, None))
     e1.record()
     copy_ms = e0.elapsed_ms(e1)
+    e0.record()
+    check(lib.gf2t_memcpy_d2d(big_b.ptr, big_a.ptr, nbytes, None))
+    e1.record()
+    rt_copy_ms = e0.elapsed_ms(e1)
     del big_a, big_b
     rng = np.random.default_rng(args.seed)
     svals = np.sort(rng.choice(np.arange(3, SMAX + 1), size=n, replace=False)).astype(np.uint64)
@@ -118,9 +127,12 @@ def mode_squaring(args):
     launch_ms = ms / args.chain_steps
     gbs = 16 * nw * n / (launch_ms * 1e-3) / 1e9
     print(json.dumps({"mode": "squaring", "residues": n, "steps": args.chain_steps,
+                      "sq_ppt": args.sq_ppt, "sq_order": args.sq_order, "sq_nt": args.sq_nt,
                       "avg_launch_ms": launch_ms, "GB_per_s": gbs, "frac": gbs / HBM_PEAK_GBS,
-                      "copy_bytes": 16 * nw * n, "copy_ms": copy_ms,
-                      "copy_GB_per_s": 16 * nw * n / (copy_ms * 1e-3) / 1e9}), flush=True)
+                      "copy_bytes_moved": 2 * nbytes16, "copy_ms": copy_ms,
+                      "copy_GB_per_s": 2 * nbytes16 / (copy_ms * 1e-3) / 1e9,
+                      "runtime_copy_GB_per_s": 2 * nbytes / (rt_copy_ms * 1e-3) / 1e9}),
+          flush=True)
 
 
 def c4_pairs(seed: int):
@@ -176,8 +188,19 @@ def mode_fft(args):
                       "products_per_s": 64 / (ms * 1e-3), "operand_bits": bits}), flush=True)
 
 
+def apply_tuning(args):
+    from mersenne_trinomial_amd._capi import check, lib
+    if args.sq_ppt:
+        check(lib.gf2t_tune(b"sq_ppt", args.sq_ppt))
+    if args.sq_order >= 0:
+        check(lib.gf2t_tune(b"sq_order", args.sq_order))
+    if args.sq_nt >= 0:
+        check(lib.gf2t_tune(b"sq_nt", args.sq_nt))
+
+
 def main():
     args = parse()
+    apply_tuning(args)
     if args.mode == "squaring":
         return mode_squaring(args)
     if args.mode == "fft":

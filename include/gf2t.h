@@ -61,6 +61,10 @@ int gf2t_event_create(void **event);
 int gf2t_event_destroy(void *event);
 int gf2t_event_record(void *event, void *stream);
 int gf2t_event_elapsed_ms(void *start, void *stop, float *ms);
+/* 16-byte vector copy kernel (dst, src 16-byte aligned, bytes % 16 == 0):
+ * the stream-copy reference the HBM roofline figures are calibrated
+ * against; no reference counterpart. */
+int gf2t_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ------------------------------------------------------------------ *
  * (b) Residue ring GF(2)[x]/(x^r + x^s + 1)
@@ -88,6 +92,15 @@ int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r,
 int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
                            const uint64_t *r, const uint64_t *s,
                            const uint64_t *steps, int n, void *stream);
+
+/* Tuning knobs of the kernels (the autotuner's handle; the reference
+ * reads GF2C_* environment variables, src/gf2_cantor_cuda.h:24-61):
+ *   "sq_ppt"   output pairs per lane in the batched squaring (1, 2, 4)
+ *   "sq_order" 1 = XCD-aware tile order in the batched squaring, 0 = linear
+ *   "sq_nt"    1 = non-temporal (streaming) stores in the batched squaring
+ *              and in gf2t_stream_copy
+ * Environment defaults: GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0), GF2T_SQ_NT (1). */
+int gf2t_tune(const char *key, int64_t value);
 
 /* Slab form: residues i = 0..n-1 at din + i*pitch (pitch >= nw words),
  * each squared once mod x^r + x^s_i + 1 into dout + i*pitch; ds is a

@@ -64,6 +64,8 @@ _SIGS = {
     "gf2t_event_destroy": (C.c_int, [_vp]),
     "gf2t_event_record": (C.c_int, [_vp, _vp]),
     "gf2t_event_elapsed_ms": (C.c_int, [_vp, _vp, C.POINTER(C.c_float)]),
+    "gf2t_stream_copy": (C.c_int, [_vp, _vp, _sz, _vp]),
+    "gf2t_tune": (C.c_int, [C.c_char_p, C.c_int64]),
     "gf2t_square_fold": (C.c_int, [_vp, _vp, _u64, _u64, _vp]),
     "gf2t_square_fold_steps": (C.c_int, [_vp, _vp, _u64, _u64, _u64, _vp]),
     "gf2t_square_fold_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _p64, _p64, _p64,

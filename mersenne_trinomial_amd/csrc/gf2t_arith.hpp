@@ -220,6 +220,52 @@ GF2T_HD void square_fold_pair_fast(const u64 *in, const SqConst &C, size_t i, u6
     }
 }
 
+// ---- XCD-aware visiting order of the output tiles of one residue
+// Window 2 of output pair i and window 3 of pair i + delta read the same
+// input words, and so do windows 4 and 5 (delta = off2 - off3 ~ s/128
+// pairs).  Launches hand workgroups to the 8 XCDs round-robin (XCD =
+// block id mod 8), each XCD with its own L2.  So block x goes to XCD
+// x mod 8, which walks a contiguous stretch of a sequence in which every
+// chunk of W tiles is followed by the chunk D*W ~ delta tiles later: the
+// second read of an input region then comes from the same XCD's L2
+// while the first is still resident.  tile_of() is a bijection between
+// the blocks [0, 8P) that are not idle and the tiles [0, T).
+struct TileOrder {
+    u64 T, W, C, D, P;
+};
+GF2T_HD TileOrder tile_order(u64 T, u64 delta_pairs, u64 pairs_per_tile) {
+    TileOrder o;
+    o.T = T;
+    o.W = 16;
+    o.C = (T + o.W - 1) / o.W;
+    u64 dt = (delta_pairs + pairs_per_tile / 2) / pairs_per_tile;   // tiles
+    u64 D = (dt + o.W / 2) / o.W;                                     // chunks
+    o.D = D ? D : 1;
+    o.P = (o.C * o.W + 7) / 8;
+    return o;
+}
+GF2T_HD int64_t tile_of(const TileOrder &o, u64 x) {
+    if (x >= 8 * o.P) return -1;               // grid wider than this residue needs
+    u64 pos = (x % 8) * o.P + x / 8;
+    if (pos >= o.C * o.W) return -1;
+    u64 cpos = pos / o.W, j = pos % o.W;
+    u64 lo = o.C / o.D, big = o.C % o.D;   // big classes hold lo + 1 chunks
+    u64 rho, q;
+    if (cpos < big * (lo + 1)) {
+        rho = cpos / (lo + 1);
+        q = cpos % (lo + 1);
+    } else {
+        u64 c2 = cpos - big * (lo + 1);
+        rho = big + c2 / lo;
+        q = c2 % lo;
+    }
+    u64 tile = (rho + q * o.D) * o.W + j;
+    return tile < o.T ? (int64_t)tile : -1;
+}
+GF2T_HD u64 sq_delta_pairs(const SqConst &c) {
+    return c.w2.off >= c.w3.off ? c.w2.off - c.w3.off : 0;
+}
+
 // 64-bit window of p (pw words) at bit pos, restricted to bits [lo, hi)
 GF2T_HD u64 pwin64(const u64 *p, size_t pw, u64 pos, u64 lo, u64 hi) {
     if (pos >= hi || pos + 64 <= lo) return 0;

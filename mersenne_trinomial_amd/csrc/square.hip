@@ -21,6 +21,8 @@
 #include <vector>
 #include <algorithm>
 #include <numeric>
+#include <cstring>
+#include <cstdlib>
 
 namespace {
 
@@ -40,18 +42,38 @@ struct SqDesc {
     const u64 *in;
     u64 *out;
     SqConst c;
+    const int *order;   // block -> tile (-1 idle), or null for identity
 };
 
-// batched: blockIdx.y selects the residue, blockIdx.x the 512-word tile
+// batched: blockIdx.y selects the residue, blockIdx.x a tile of
+// SQ_THREADS*PPT output pairs (pairs j*SQ_THREADS + lane for j < PPT)
+template <int PPT, bool NT>
 __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
         const SqDesc *__restrict__ desc) {
     const SqDesc d = desc[blockIdx.y];
-    size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
-    if (i >= (d.c.nw + 1) / 2) return;
-    u64 o0, o1;
-    square_fold_pair_fast(d.in, d.c, i, o0, o1);
-    d.out[2 * i] = o0;
-    if (2 * i + 1 < d.c.nw) d.out[2 * i + 1] = o1;
+    int t = (int)blockIdx.x;
+    if (d.order) {
+        t = d.order[blockIdx.x];
+        if (t < 0) return;
+    }
+    const size_t np = (d.c.nw + 1) / 2;
+    const size_t base = (size_t)t * SQ_THREADS * PPT + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        size_t i = base + (size_t)j * SQ_THREADS;
+        if (i < np) {
+            u64 o0, o1;
+            square_fold_pair_fast(d.in, d.c, i, o0, o1);
+            if (NT) {
+                // streaming store: F^2 is not re-read before the next launch
+                __builtin_nontemporal_store(o0, d.out + 2 * i);
+                if (2 * i + 1 < d.c.nw) __builtin_nontemporal_store(o1, d.out + 2 * i + 1);
+            } else {
+                d.out[2 * i] = o0;
+                if (2 * i + 1 < d.c.nw) d.out[2 * i + 1] = o1;
+            }
+        }
+    }
 }
 
 // slab: residue y at in + y*pitch, its s in svec[y]
@@ -104,6 +126,49 @@ __global__ void k_xor_word0(u64 *p, u64 v) {
 
 bool valid_rs(u64 r, u64 s) { return r >= 3 && s >= 1 && 2 * s < r; }
 
+// tuning knobs of the batched squaring (gf2t_tune / GF2T_SQ_PPT, GF2T_SQ_ORDER)
+int g_sq_ppt = -1, g_sq_order = -1, g_sq_nt = -1;
+bool sq_nt() {
+    if (g_sq_nt < 0) {
+        const char *e = getenv("GF2T_SQ_NT");
+        g_sq_nt = e ? atoi(e) : 1;
+    }
+    return g_sq_nt != 0;
+}
+void sq_tuning(int &ppt, int &order) {
+    if (g_sq_ppt < 0) {
+        const char *e = getenv("GF2T_SQ_PPT");
+        g_sq_ppt = e ? atoi(e) : 2;
+    }
+    if (g_sq_order < 0) {
+        const char *e = getenv("GF2T_SQ_ORDER");
+        g_sq_order = e ? atoi(e) : 0;
+    }
+    ppt = (g_sq_ppt == 2 || g_sq_ppt == 4) ? g_sq_ppt : 1;
+    order = g_sq_order ? 1 : 0;
+}
+
+// 16-byte vector copy, four loads in flight per lane
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_copy(const ulonglong2 *__restrict__ src,
+                                                     ulonglong2 *__restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        ulonglong2 v[4] = {src[i], src[i + stride], src[i + 2 * stride], src[i + 3 * stride]};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (NT) {
+                __builtin_nontemporal_store(v[k].x, &dst[i + k * stride].x);
+                __builtin_nontemporal_store(v[k].y, &dst[i + k * stride].y);
+            } else {
+                dst[i + k * stride] = v[k];
+            }
+        }
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 unsigned blocks_for(size_t work, int threads) {
     return (unsigned)((work + threads - 1) / threads);
 }
@@ -117,9 +182,8 @@ extern "C" int gf2t_square_fold(const uint64_t *din, uint64_t *dout, uint64_t r,
         return GF2T_EINVAL;
     }
     SqConst c = sq_const(r, s);
-    size_t np = (c.nw + 1) / 2;
-    hipLaunchKernelGGL(k_square_fold, dim3(blocks_for(np, SQ_THREADS)), dim3(SQ_THREADS), 0,
-                       (hipStream_t)stream, din, dout, c);
+    hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((c.nw + 1) / 2, SQ_THREADS)),
+                       dim3(SQ_THREADS), 0, (hipStream_t)stream, din, dout, c);
     GF2T_CHECK_LAUNCH("k_square_fold");
     return GF2T_OK;
 }
@@ -131,12 +195,12 @@ extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r
         return GF2T_EINVAL;
     }
     SqConst c = sq_const(r, s);
-    size_t nw = c.nw, np = (nw + 1) / 2;
+    size_t nw = c.nw;
     hipStream_t st = (hipStream_t)stream;
     u64 *a = dres, *b = dtmp;
     for (u64 k = 0; k < steps; k++) {
-        hipLaunchKernelGGL(k_square_fold, dim3(blocks_for(np, SQ_THREADS)), dim3(SQ_THREADS),
-                           0, st, a, b, c);
+        hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((nw + 1) / 2, SQ_THREADS)),
+                           dim3(SQ_THREADS), 0, st, a, b, c);
         std::swap(a, b);
     }
     GF2T_CHECK_LAUNCH("k_square_fold");
@@ -145,13 +209,34 @@ extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r
 }
 
 namespace {
-// descriptor tables for the batched chain, cached and grown on demand
+// descriptor and tile-order tables for the batched chain, cached and
+// grown on demand
 struct BatchScratch {
     SqDesc *d = nullptr;
-    size_t cap = 0;
+    int *ord = nullptr;
+    size_t cap = 0, ocap = 0;
     int dev = -1;
 } g_batch;
+
+template <int PPT>
+void launch_batch(unsigned gx, unsigned active, const SqDesc *tab, hipStream_t st) {
+    if (sq_nt())
+        hipLaunchKernelGGL((k_square_fold_batch<PPT, true>), dim3(gx, active), dim3(SQ_THREADS), 0,
+                           st, tab);
+    else
+        hipLaunchKernelGGL((k_square_fold_batch<PPT, false>), dim3(gx, active), dim3(SQ_THREADS),
+                           0, st, tab);
+}
 }  // namespace
+
+extern "C" int gf2t_tune(const char *key, int64_t value) {
+    if (!key) return GF2T_EINVAL;
+    if (!strcmp(key, "sq_ppt")) { g_sq_ppt = (int)value; return GF2T_OK; }
+    if (!strcmp(key, "sq_order")) { g_sq_order = (int)value; return GF2T_OK; }
+    if (!strcmp(key, "sq_nt")) { g_sq_nt = (int)value; return GF2T_OK; }
+    gf2t::set_error("gf2t_tune: unknown key '%s'", key);
+    return GF2T_EINVAL;
+}
 
 extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
                                       const uint64_t *r, const uint64_t *s,
@@ -171,6 +256,8 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
             return GF2T_EINVAL;
         }
     hipStream_t st = (hipStream_t)stream;
+    int ppt, order;
+    sq_tuning(ppt, order);
     // order residues by remaining steps, longest first: the residues
     // still active at step k are a prefix, so each launch covers exactly
     // the live chains (uneven frontiers, per-s early exit)
@@ -180,22 +267,66 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
     int dev = 0;
     GF2T_HIP(hipGetDevice(&dev));
     size_t need = 2 * (size_t)n;
-    if (g_batch.cap < need || g_batch.dev != dev) {
-        if (g_batch.d && g_batch.dev == dev) (void)hipFree(g_batch.d);
-        g_batch.d = nullptr;
-        GF2T_HIP(hipMalloc(&g_batch.d, need * sizeof(SqDesc)));
-        g_batch.cap = need;
-        g_batch.dev = dev;
-    }
-    std::vector<SqDesc> h(need);
-    size_t maxpairs = 0;
+    // per-residue tile orders
+    std::vector<int> otab;
+    std::vector<size_t> ooff(n, 0);
+    size_t gx = 0;
+    const u64 tile_pairs = (u64)SQ_THREADS * ppt;
     for (int k = 0; k < n; k++) {
         int i = ord[k];
         SqConst c = sq_const(r[i], s[i]);
-        u64 nw = c.nw;
-        h[k] = SqDesc{dres[i], dtmp[i], c};          // even steps
-        h[n + k] = SqDesc{dtmp[i], dres[i], c};      // odd steps
-        maxpairs = std::max(maxpairs, (size_t)(nw + 1) / 2);
+        u64 T = ((c.nw + 1) / 2 + tile_pairs - 1) / tile_pairs;
+        if (order) {
+            TileOrder o = tile_order(T, sq_delta_pairs(c), tile_pairs);
+            ooff[k] = otab.size();
+            for (u64 x = 0; x < 8 * o.P; x++) otab.push_back((int)tile_of(o, x));
+            gx = std::max<size_t>(gx, 8 * o.P);
+        } else {
+            gx = std::max<size_t>(gx, T);
+        }
+    }
+    if (g_batch.dev != dev) {
+        g_batch = BatchScratch();
+        g_batch.dev = dev;
+    }
+    if (g_batch.cap < need) {
+        if (g_batch.d) (void)hipFree(g_batch.d);
+        g_batch.d = nullptr;
+        GF2T_HIP(hipMalloc(&g_batch.d, need * sizeof(SqDesc)));
+        g_batch.cap = need;
+    }
+    if (order && g_batch.ocap < otab.size()) {
+        if (g_batch.ord) (void)hipFree(g_batch.ord);
+        g_batch.ord = nullptr;
+        GF2T_HIP(hipMalloc(&g_batch.ord, otab.size() * sizeof(int)));
+        g_batch.ocap = otab.size();
+    }
+    // a block index beyond a residue's own table must idle: pad each
+    // residue's table to gx entries
+    if (order) {
+        std::vector<int> padded((size_t)n * gx, -1);
+        for (int k = 0; k < n; k++) {
+            size_t len = (k + 1 < n ? ooff[k + 1] : otab.size()) - ooff[k];
+            std::copy(otab.begin() + ooff[k], otab.begin() + ooff[k] + len,
+                      padded.begin() + (size_t)k * gx);
+        }
+        otab.swap(padded);
+        if (g_batch.ocap < otab.size()) {
+            if (g_batch.ord) (void)hipFree(g_batch.ord);
+            g_batch.ord = nullptr;
+            GF2T_HIP(hipMalloc(&g_batch.ord, otab.size() * sizeof(int)));
+            g_batch.ocap = otab.size();
+        }
+        GF2T_HIP(hipMemcpyAsync(g_batch.ord, otab.data(), otab.size() * sizeof(int),
+                                hipMemcpyHostToDevice, st));
+    }
+    std::vector<SqDesc> h(need);
+    for (int k = 0; k < n; k++) {
+        int i = ord[k];
+        SqConst c = sq_const(r[i], s[i]);
+        const int *ot = order ? g_batch.ord + (size_t)k * gx : nullptr;
+        h[k] = SqDesc{dres[i], dtmp[i], c, ot};          // even steps
+        h[n + k] = SqDesc{dtmp[i], dres[i], c, ot};      // odd steps
     }
     GF2T_HIP(hipMemcpyAsync(g_batch.d, h.data(), need * sizeof(SqDesc), hipMemcpyHostToDevice, st));
     u64 kmax = steps[ord[0]];
@@ -203,15 +334,16 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
     for (u64 k = 0; k < kmax; k++) {
         while (active > 0 && steps[ord[active - 1]] <= k) active--;
         const SqDesc *tab = g_batch.d + ((k & 1) ? n : 0);
-        hipLaunchKernelGGL(k_square_fold_batch, dim3(blocks_for(maxpairs, SQ_THREADS), active),
-                           dim3(SQ_THREADS), 0, st, tab);
+        if (ppt == 4) launch_batch<4>((unsigned)gx, active, tab, st);
+        else if (ppt == 2) launch_batch<2>((unsigned)gx, active, tab, st);
+        else launch_batch<1>((unsigned)gx, active, tab, st);
     }
     GF2T_CHECK_LAUNCH("k_square_fold_batch");
     for (int i = 0; i < n; i++)
         if (steps[i] & 1)
             GF2T_HIP(hipMemcpyAsync(dres[i], dtmp[i], words_of(r[i]) * 8,
                                     hipMemcpyDeviceToDevice, st));
-    // the host table must outlive the async copy
+    // the host tables must outlive the async copies
     GF2T_HIP(hipStreamSynchronize(st));
     return GF2T_OK;
 }
@@ -273,9 +405,26 @@ extern "C" int gf2t_square_fold_slab(const uint64_t *din, uint64_t *dout, size_t
         return GF2T_EINVAL;
     }
     if (n == 0) return GF2T_OK;
-    size_t np = (nw + 1) / 2;
-    hipLaunchKernelGGL(k_square_fold_slab, dim3(blocks_for(np, SQ_THREADS), n), dim3(SQ_THREADS), 0,
+    unsigned gx = blocks_for((nw + 1) / 2, SQ_THREADS);
+    hipLaunchKernelGGL(k_square_fold_slab, dim3(gx, n), dim3(SQ_THREADS), 0,
                        (hipStream_t)stream, din, dout, (u64)pitch, r, ds);
     GF2T_CHECK_LAUNCH("k_square_fold_slab");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_stream_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!dst || !src || (bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15)) {
+        gf2t::set_error("gf2t_stream_copy: 16-byte aligned pointers and size required");
+        return GF2T_EINVAL;
+    }
+    size_t n = bytes / 16;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 32);
+    if (sq_nt())
+        hipLaunchKernelGGL(k_stream_copy<true>, dim3(blocks ? blocks : 1), dim3(256), 0,
+                           (hipStream_t)stream, (const ulonglong2 *)src, (ulonglong2 *)dst, n);
+    else
+        hipLaunchKernelGGL(k_stream_copy<false>, dim3(blocks ? blocks : 1), dim3(256), 0,
+                           (hipStream_t)stream, (const ulonglong2 *)src, (ulonglong2 *)dst, n);
+    GF2T_CHECK_LAUNCH("k_stream_copy");
     return GF2T_OK;
 }

@@ -29,6 +29,11 @@ u64 kl_gf64_mul(u64 a, u64 b) { return gf64_mul(a, b); }
 u64 kl_clmul32(u32 a, u32 b) { return clmul32(a, b); }
 u32 kl_fmul(u32 a, u32 b, u64 q, int d) { return fmul(a, b, q, d); }
 u64 kl_range_mask64(int lc, int hc) { return range_mask64(lc, hc); }
+int64_t kl_tile_of(u64 T, u64 delta_pairs, u64 ppt, u64 x, u64 *nblocks) {
+    TileOrder o = tile_order(T, delta_pairs, ppt);
+    *nblocks = 8 * o.P;
+    return tile_of(o, x);
+}
 u32 kl_gcd32(u32 a, u32 b) { return gcd32(a, b); }
 u32 kl_inv32(u32 a, u32 m) { return inv32(a, m); }
 u64 kl_mod_mersenne(u64 x, int d) { return mod_mersenne(x, d, ((u64)1 << d) - 1); }

@@ -68,7 +68,21 @@ def test_full_size_random_s(eng):
         assert np.array_equal(eng.square_fold(a, R136, s, 2), O.square_fold_steps(a, R136, s, 2)), s
 
 
-def test_batch_uneven_frontiers_mixed_r(eng):
+@pytest.fixture(params=[(1, 0, 0), (2, 0, 0), (4, 0, 1), (1, 1, 0), (2, 1, 1), (4, 1, 0)],
+                ids=lambda p: f"ppt{p[0]}-order{p[1]}-nt{p[2]}")
+def tuned(request, eng):
+    from mersenne_trinomial_amd._capi import check, lib
+    ppt, order, nt = request.param
+    check(lib.gf2t_tune(b"sq_ppt", ppt))
+    check(lib.gf2t_tune(b"sq_order", order))
+    check(lib.gf2t_tune(b"sq_nt", nt))
+    yield request.param
+    check(lib.gf2t_tune(b"sq_ppt", 2))
+    check(lib.gf2t_tune(b"sq_order", 0))
+    check(lib.gf2t_tune(b"sq_nt", 1))
+
+
+def test_batch_uneven_frontiers_mixed_r(eng, tuned):
     from mersenne_trinomial_amd._capi import check, lib
     from mersenne_trinomial_amd.device import DeviceBuffer
     import ctypes as C
@@ -91,7 +105,7 @@ def test_batch_uneven_frontiers_mixed_r(eng):
         assert np.array_equal(b.download(O.words(r)), O.square_fold_steps(a, r, s, k)), (r, s, k)
 
 
-def test_batch_full_size(eng):
+def test_batch_full_size(eng, tuned):
     from mersenne_trinomial_amd._capi import check, lib
     from mersenne_trinomial_amd.device import DeviceBuffer
     import ctypes as C

@@ -32,6 +32,8 @@ def kl(tmp_path_factory):
     lib.kl_fmul.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int]
     lib.kl_range_mask64.restype = C.c_uint64
     lib.kl_range_mask64.argtypes = [C.c_int, C.c_int]
+    lib.kl_tile_of.restype = C.c_int64
+    lib.kl_tile_of.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, P]
     lib.kl_gcd32.restype = C.c_uint32
     lib.kl_gcd32.argtypes = [C.c_uint32, C.c_uint32]
     lib.kl_inv32.restype = C.c_uint32
@@ -192,3 +194,25 @@ def test_minpoly_bm_equals_product(kl):
             assert O.is_irreducible_small(want, d)
             n += 1
         assert n > 0
+
+
+def test_tile_order_is_a_bijection(kl):
+    nb = np.zeros(1, dtype=np.uint64)
+    for T in (1, 7, 16, 17, 100, 4159, 8318):
+        for delta in (0, 1, 255, 256, 257, 4000, 40000, 532343, 10**7):
+            kl.kl_tile_of(T, delta, 256, 0, ptr(nb))
+            tiles = [kl.kl_tile_of(T, delta, 256, x, ptr(nb)) for x in range(int(nb[0]) + 40)]
+            got = sorted(t for t in tiles if t >= 0)
+            assert got == list(range(T)), (T, delta)
+
+
+def test_square_fold_fast_with_tile_order_covers_output(kl):
+    # the kernels' squaring is per-pair; the order only permutes tiles.
+    # Re-run the fast path in tile order on the host and compare.
+    rng = np.random.default_rng(4)
+    for r, s in [(132049, 66024), (132049, 1), (756839, 300000)]:
+        a = rand_res(rng, r)
+        want = O.square_fold(a, r, s)
+        out = np.zeros(O.words(r), np.uint64)
+        kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
+        assert np.array_equal(out, want)
