@@ -1,0 +1,230 @@
+// gf2t_engine.hpp -- C++ engine structs with the reference's names and
+// member semantics, implemented over the C ABI (gf2t.h), so code written
+// against src/gf2_cantor_engine.h, src/gf2_cantor_cuda.h and
+// src/gf2_trinomial_cuda.h recompiles against the MI355X library.
+// Like the reference, a failed call prints the error and exits(1)
+// (CUCHK, src/gf2_cantor_cuda.h:65-69); use gf2t.h directly for status
+// returns.
+#ifndef GF2T_ENGINE_HPP
+#define GF2T_ENGINE_HPP
+#include "gf2t.h"
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace gf2t {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+inline void check(int rc, const char *what) {
+    if (rc != GF2T_OK) {
+        fprintf(stderr, "gf2t error %d in %s: %s\n", rc, what, gf2t_last_error());
+        exit(1);
+    }
+}
+
+inline size_t bits_to_words(u64 bits) { return (size_t)((bits + 63) >> 6); }
+inline size_t mul_out_words(u64 abits, u64 bbits) { return gf2t_mul_out_words(abits, bbits); }
+
+// A device allocation of n words (RAII).
+struct DevWords {
+    u64 *p = nullptr;
+    size_t n = 0;
+    DevWords() {}
+    explicit DevWords(size_t words) { alloc(words); }
+    DevWords(const DevWords &) = delete;
+    DevWords &operator=(const DevWords &) = delete;
+    ~DevWords() { release(); }
+    void alloc(size_t words) {
+        release();
+        void *q = nullptr;
+        check(gf2t_malloc(&q, (words ? words : 1) * 8), "gf2t_malloc");
+        p = (u64 *)q;
+        n = words;
+    }
+    void release() {
+        if (p) gf2t_free(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const u64 *h, size_t words) {
+        check(gf2t_memcpy_h2d(p, h, words * 8, nullptr), "h2d");
+    }
+    void download(u64 *h, size_t words) const {
+        check(gf2t_memcpy_d2h(h, p, words * 8, nullptr), "d2h");
+    }
+};
+
+// GF(2^64) = GF(2)[x]/(x^64 + x^4 + x^3 + x + 1), bit-serial (host checks)
+inline u64 gf64_mul_host(u64 a, u64 b) {
+    u64 r = 0;
+    while (b) {
+        if (b & 1) r ^= a;
+        u64 hi = a >> 63;
+        a <<= 1;
+        if (hi) a ^= 0x1BULL;
+        b >>= 1;
+    }
+    return r;
+}
+
+// src/gf2_cantor_engine.h:74-121
+struct CantorBasis {
+    u64 b[64];
+    int len = 0;
+    void build() { check(gf2t_cantor_basis(b, &len), "gf2t_cantor_basis"); }
+    bool verify() const {
+        if (len < 1 || b[0] != 1) return false;
+        for (int i = 1; i < len; i++)
+            if ((gf64_mul_host(b[i], b[i]) ^ b[i]) != b[i - 1]) return false;
+        u64 pv[64] = {0};
+        for (int i = 0; i < len; i++) {
+            u64 v = b[i];
+            for (int bit = 63; bit >= 0 && v; bit--) {
+                if (!((v >> bit) & 1)) continue;
+                if (pv[bit]) v ^= pv[bit];
+                else { pv[bit] = v; v = 0; }
+            }
+            if (v) return false;
+            bool placed = false;
+            for (int bit = 0; bit < 64; bit++) placed |= pv[bit] != 0;
+            if (!placed) return false;
+        }
+        int rank = 0;
+        for (int bit = 0; bit < 64; bit++) rank += pv[bit] != 0;
+        return rank == len;
+    }
+};
+
+// Device transform (CudaFFT, src/gf2_cantor_cuda.h:361-521).
+struct CudaFFT {
+    int m = 0;
+    size_t n = 0;
+    gf2t_fft_plan *plan = nullptr;
+    void init(int m_, const CantorBasis &) {
+        fini();
+        m = m_;
+        n = (size_t)1 << m;
+        check(gf2t_fft_plan_create(m, &plan), "gf2t_fft_plan_create");
+    }
+    void fini() {
+        if (plan) gf2t_fft_plan_destroy(plan);
+        plan = nullptr;
+    }
+    void fwd(u64 *df) const { check(gf2t_fft_fwd(plan, df, nullptr), "gf2t_fft_fwd"); }
+    void inv(u64 *df) const { check(gf2t_fft_inv(plan, df, nullptr), "gf2t_fft_inv"); }
+};
+
+// Host-buffer transform with the CantorFFT interface
+// (src/gf2_cantor_engine.h:125-188); it runs on the device.
+struct CantorFFT {
+    int m = 0;
+    size_t n = 0;
+    CudaFFT dev;
+    void init(int m_, const CantorBasis &B) {
+        m = m_;
+        n = (size_t)1 << m;
+        dev.init(m, B);
+    }
+    void fwd(u64 *f) const { run(f, true); }
+    void inv(u64 *f) const { run(f, false); }
+    void run(u64 *f, bool forward) const {
+        DevWords d(n);
+        d.upload(f, n);
+        if (forward) dev.fwd(d.p);
+        else dev.inv(d.p);
+        d.download(f, n);
+    }
+};
+
+// out (mul_out_words words) = a * b over GF(2)[x] (gf2x_fft_mul,
+// src/gf2_cantor_engine.h:232-242); host buffers in and out.
+inline void gf2x_fft_mul(const u64 *a, u64 abits, const u64 *b, u64 bbits, u64 *out,
+                         const CantorBasis &) {
+    size_t ow = mul_out_words(abits, bbits);
+    DevWords da(bits_to_words(abits)), db(bits_to_words(bbits)), dout(ow);
+    da.upload(a, bits_to_words(abits));
+    db.upload(b, bits_to_words(bbits));
+    void *work = nullptr;
+    size_t wb = gf2t_mul_workspace_bytes(abits, bbits);
+    if (wb) check(gf2t_malloc(&work, wb), "gf2t_malloc");
+    check(gf2t_mul(da.p, abits, db.p, bbits, dout.p, work, nullptr), "gf2t_mul");
+    dout.download(out, ow);
+    if (work) gf2t_free(work);
+}
+
+// Arithmetic in GF(2)[x]/(x^r + x^s + 1) (TrinGPU,
+// src/gf2_trinomial_cuda.h:87-178).  dT0 holds the last full product
+// (2r-1 bits) after mul_dev, as in the reference.
+struct TrinGPU {
+    u64 r = 0, s = 0;
+    size_t nw = 0, nw2 = 0;
+    int m = 0;
+    CantorBasis Bb;
+    DevWords A, ACC, T0, T1, W;
+    u64 *dA = nullptr, *dACC = nullptr, *dT0 = nullptr, *dT1 = nullptr;
+
+    void init(u64 r_) {
+        r = r_;
+        s = 0;
+        nw = bits_to_words(r);
+        nw2 = 2 * nw;
+        Bb.build();
+        size_t ca = (size_t)((r + 31) >> 5);
+        m = 1;
+        while (((size_t)1 << m) < 2 * ca - 1) m++;
+        A.alloc(nw);
+        ACC.alloc(nw);
+        T0.alloc(nw2 + 2);
+        T1.alloc(nw);
+        W.alloc((gf2t_mul_workspace_bytes(r, r) + 7) / 8);
+        dA = A.p; dACC = ACC.p; dT0 = T0.p; dT1 = T1.p;
+    }
+    void set_s(u64 s_) {
+        if (!(s_ >= 1 && 2 * s_ < r)) {
+            fprintf(stderr, "need 1 <= s <= (r-1)/2\n");
+            exit(1);
+        }
+        s = s_;
+    }
+    void set_x() { check(gf2t_set_x(dA, nw, nullptr), "gf2t_set_x"); }
+    void reset_acc() { check(gf2t_set_one(dACC, nw, nullptr), "gf2t_set_one"); }
+    void reduce(u64 *dst_rbits, u64 L) {
+        check(gf2t_fold(dT0, L, dst_rbits, r, s, nullptr), "gf2t_fold");
+    }
+    void square_A() {
+        check(gf2t_square_fold(dA, dT1, r, s, nullptr), "gf2t_square_fold");
+        std::swap(dA, dT1);
+    }
+    void mul_dev(const u64 *dx, const u64 *dy) {
+        check(gf2t_mul(dx, r, dy, r, dT0, W.p, nullptr), "gf2t_mul");
+    }
+    void accumulate() {
+        check(gf2t_xor_x(dA, nullptr), "gf2t_xor_x");
+        check(gf2t_mulmod(dACC, r, dA, r, r, s, dACC, W.p, nullptr), "gf2t_mulmod");
+        check(gf2t_xor_x(dA, nullptr), "gf2t_xor_x");
+    }
+    void fini() {
+        A.release(); ACC.release(); T0.release(); T1.release(); W.release();
+        dA = dACC = dT0 = dT1 = nullptr;
+    }
+};
+
+// Coarse sieve of all s in [1, smax] over degrees min_depth..depth into a
+// host best[] (run_gpu, src/coarse_sieve.cu:1119-1225); best seeds the
+// minimum when it already holds smax+1 entries.
+inline void run_gpu(u64 r, int depth, u64 smax, std::vector<u64> &best, int min_depth = 2,
+                    std::vector<gf2t_sieve_stats> *stats = nullptr) {
+    if (best.size() != smax + 1) best.assign(smax + 1, ~0ULL);
+    DevWords d(smax + 1);
+    d.upload(best.data(), smax + 1);
+    std::vector<gf2t_sieve_stats> st(33);
+    check(gf2t_sieve(r, min_depth, depth, smax, d.p, st.data()), "gf2t_sieve");
+    d.download(best.data(), smax + 1);
+    if (stats) *stats = st;
+}
+
+}  // namespace gf2t
+#endif

@@ -16,10 +16,13 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgf2t.so")
+BIN = os.path.join(PKG, "bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["capi.hip", "square.hip", "fft.hip", "sieve.hip"]
 HOST_SOURCES = ["host_poly.cpp", "scan.cpp"]
+# command-line drivers with the reference's argv forms and output files
+CLIS = ["coarse_sieve", "trinomial_stage2", "trinomial_scan"]
 CXX = os.environ.get("CXX", "g++")
 HOSTFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-mpclmul", "-msse4.1", "-pthread",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
@@ -33,7 +36,7 @@ def _run(cmd):
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         sys.stderr.write(" ".join(cmd) + "\n" + p.stdout + p.stderr)
-        raise RuntimeError(f"compile failed: {cmd[-1]}")
+        raise RuntimeError(f"compile failed: {cmd}")
     return p
 
 
@@ -52,6 +55,17 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    clis = []
+    for name in CLIS:
+        out = os.path.join(BIN, name)
+        clis.append([CXX, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+                     os.path.join(PKG, "cli", name + ".cpp"), "-o", out + ".tmp", "-L" + PKG,
+                     "-lgf2t", "-Wl,-rpath,$ORIGIN/.."])
+    with ThreadPoolExecutor(max_workers=len(clis)) as ex:
+        list(ex.map(_run, clis))
+    for name in CLIS:
+        os.replace(os.path.join(BIN, name + ".tmp"), os.path.join(BIN, name))
     if verbose:
         print("built", LIB)
     return LIB
