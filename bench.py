@@ -31,6 +31,13 @@ sys.path.insert(0, ROOT)
 R = 136279841
 SMAX = R // 2
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one k_square_fold_batch launch moves for 256 residues at
+# r = 136279841 (default tuning), from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# runs scaled by the factors a copy kernel of known size gives (x2.0 for
+# FETCH_SIZE, x1.0 for WRITE_SIZE): 6,736,154,912 B fetched + 4,570,116,256 B
+# written.  The command lines and raw counts are kept in
+# profiles/k_square_fold_batch_r136279841_256res.md.
+SQ_TRAFFIC_256 = 6736154912 + 4570116256
 
 
 def parse():
@@ -200,16 +207,17 @@ def apply_tuning(args):
 
 def main():
     args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the deep scan shards by independent s values, the sieve by class ranges "
+              "with one elementwise-min exchange of best[])", file=sys.stderr)
+        sys.exit(2)
     apply_tuning(args)
+    default_tuning = not (args.sq_ppt or args.sq_order >= 0 or args.sq_nt >= 0)
     if args.mode == "squaring":
         return mode_squaring(args)
     if args.mode == "fft":
         return mode_fft(args)
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the deep scan shards by independent s values and the sieve by degree buckets; "
-              "see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
 
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib
@@ -305,7 +313,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": SQ_TRAFFIC_256 if (n == 256 and default_tuning) else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": launch_ms,
         },

@@ -98,6 +98,11 @@ def words(bits: int) -> int:
     return (bits + 63) >> 6
 
 
+def mul_out_words(abits: int, bbits: int) -> int:
+    """Words of a product buffer (mul_out_words, src/gf2_cantor_engine.h:200-202)."""
+    return lib().gf2o_mul_out_words(abits, bbits)
+
+
 # ---------------------------------------------------------------- field
 def fmul(a, b, q, d):
     return lib().gf2o_fmul(a, b, q, d)

@@ -1,0 +1,74 @@
+"""The HIP path against the golden fixtures made from the reference's own
+code and against its published figures (tests/golden)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from golden_data import SIEVE_FIXTURES, cantor, known, product_1m_inputs, sieve_lines, words
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import device
+    device.init(0)
+    return M
+
+
+def cert(best):
+    out = []
+    for s in np.nonzero(best != np.uint64(0xFFFFFFFFFFFFFFFF))[0]:
+        if s:
+            k = int(best[s])
+            out.append(f"{s} {k >> 48} p{k & 0xFFFFFFFFFFFF:x}")
+    return out
+
+
+@pytest.mark.parametrize("r,depth,smax", SIEVE_FIXTURES)
+def test_sieve_vs_reference_fixture(eng, r, depth, smax):
+    best, _ = eng.run_gpu(r, depth, smax)
+    assert cert(best) == sieve_lines(r, depth, smax)
+
+
+def test_cantor_basis(eng):
+    B = eng.CantorBasis()
+    B.build()
+    assert B.verify()
+    assert np.array_equal(np.array(B.b[:64], dtype=np.uint64), words(cantor()["basis"]))
+
+
+def test_fft_forward_vectors(eng):
+    for t in cantor()["fft_fwd"]:
+        f = words(t["in"])
+        assert np.array_equal(eng.fft_fwd(f, t["m"]), words(t["fwd"])), t["m"]
+        assert np.array_equal(eng.fft_fwd(words(t["fwd"]), t["m"], inverse=True), f)
+
+
+def test_product_vectors(eng):
+    for p in cantor()["products"]:
+        got = eng.mul(words(p["a"]), p["abits"], words(p["b"]), p["bbits"])
+        assert np.array_equal(got, words(p["out"])), (p["abits"], p["bbits"])
+
+
+def test_product_1m_digest(eng):
+    a, b, big = product_1m_inputs()
+    out = eng.mul(a, big["abits"], b, big["bbits"])
+    assert hashlib.md5(out.tobytes()).hexdigest() == big["out_md5"]
+
+
+def test_full_range_heavy_counts_match_reference_log(eng):
+    # every degree's count of progressions longer than 1024 marks, over all
+    # s <= r/2 at r=136279841, as printed by the reference's run
+    k = known()["r136279841_full_range"]
+    _, st = eng.run_gpu(k["r"], 32, k["smax"])
+    got = {s.d: s.heavy for s in st}
+    for d, n in k["heavy"].items():
+        if int(d) <= 32:
+            assert got[int(d)] == n, d
+    classes = {s.d: s.classes for s in st}
+    for d, n in k["classes"].items():
+        if int(d) <= 32:
+            assert classes[int(d)] == n, d

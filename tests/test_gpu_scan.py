@@ -32,7 +32,9 @@ def oracle_lines(r, svals, k0, maxd):
 def test_scan_r4423_vs_oracle(eng, L0, Lmax):
     r, k0, maxd = 4423, 5, 200
     want = oracle_lines(r, range(2, 351), k0, maxd)
-    assert len(want) > 60
+    # src/README.md:107-112: 104 survivors, 102 factors and 2 "u"
+    assert len(want) == 104
+    assert sum(v.endswith(" u") for v in want.values()) == 2
     svals = sorted(want)
     got = eng.scan(r, svals, k0=k0, maxd=maxd, L0=L0, Lmax=Lmax)
     lines = {o.s: o.line() for o in got}
