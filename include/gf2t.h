@@ -99,7 +99,11 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  *   "sq_order" 1 = XCD-aware tile order in the batched squaring, 0 = linear
  *   "sq_nt"    1 = non-temporal (streaming) stores in the batched squaring
  *              and in gf2t_stream_copy
- * Environment defaults: GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0), GF2T_SQ_NT (1). */
+ *   "gcd_dev_bits" products inside the host gcds (gf2t_gcd_host, the
+ *              deep scan) go to the device when both operands have at
+ *              least this many bits; 0 = host Karatsuba only
+ * Environment defaults: GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0), GF2T_SQ_NT (1),
+ * GF2T_GCD_DEV_BITS (2^18, set by gf2t_init; 0 before it). */
 int gf2t_tune(const char *key, int64_t value);
 
 /* Slab form: residues i = 0..n-1 at din + i*pitch (pitch >= nw words),
@@ -186,6 +190,21 @@ int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_t *db,
                      uint64_t *dout, size_t opitch, int n, void *dwork,
                      void *stream);
 
+/* Host-buffer product service: the device side of the reference's
+ * GpuMulHook (TsGpuMul::mul / mat2_apply, src/tsfactor.cu:467-607;
+ * interface src/gf2_gcd.h:155-172).  Host pointers in and out; the
+ * library keeps one mutex-guarded stream and device buffers (grown on
+ * demand, kept until exit), so any host thread may call.  Synchronous.
+ *   gf2t_mul_host:  hout (gf2t_mul_out_words(abits, bbits) words) = a*b.
+ *   gf2t_mat2_host: (oa, ob) = (m0 a + m1 b, m2 a + m3 b) with one set of
+ *   six forward and two inverse transforms; outw >= gf2t_mul_out_words(
+ *   max mbits, max(abits, bbits)), both outputs fully written. */
+int gf2t_mul_host(const uint64_t *ha, uint64_t abits, const uint64_t *hb,
+                  uint64_t bbits, uint64_t *hout);
+int gf2t_mat2_host(const uint64_t *const hm[4], const uint64_t mbits[4],
+                   const uint64_t *ha, uint64_t abits, const uint64_t *hb,
+                   uint64_t bbits, uint64_t *hoa, uint64_t *hob, size_t outw);
+
 /* ------------------------------------------------------------------ *
  * (a) Zech-log coarse sieve
  * ------------------------------------------------------------------ */
@@ -257,7 +276,11 @@ int gf2t_scan_line(const gf2t_scan_results *res, size_t i, char *buf,
 void gf2t_scan_results_free(gf2t_scan_results *res);
 
 /* Host gcd of two GF(2)[x] word arrays: degree (-1 if both zero) and up
- * to gcap words of the monic gcd (poly_gcd_naive, src/gf2_gcd.h:48-75). */
+ * to gcap words of the monic gcd.  Half-gcd recursion with Karatsuba
+ * products, and device FFT products above the "gcd_dev_bits" threshold
+ * (poly_gcd_naive / poly_gcd_ntl / poly_gcd_hybrid, src/gf2_gcd.h:48-75,
+ * src/gf2_gcd_ntl.cpp:32-42, :354-452).  Returns GF2T_EHIP (negative)
+ * if a device product fails. */
 int64_t gf2t_gcd_host(const uint64_t *a, size_t aw, const uint64_t *b,
                       size_t bw, uint64_t *g, size_t gcap);
 /* Numerically least irreducible factor of t, whose factors all have

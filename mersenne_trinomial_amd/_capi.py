@@ -97,6 +97,8 @@ _SIGS = {
     "gf2t_scan_line": (C.c_int, [_vp, _sz, C.c_char_p, _sz]),
     "gf2t_scan_results_free": (None, [_vp]),
     "gf2t_gcd_host": (C.c_int64, [_p64, _sz, _p64, _sz, _p64, _sz]),
+    "gf2t_mul_host": (C.c_int, [_p64, _u64, _p64, _u64, _p64]),
+    "gf2t_mat2_host": (C.c_int, [C.POINTER(_p64), _p64, _p64, _u64, _p64, _u64, _p64, _p64, _sz]),
     "gf2t_edf_least": (C.c_int, [_p64, _sz, _u64, _p64, _sz]),
     "gf2t_swan_bound": (_u64, [_u64, _u64, C.POINTER(C.c_int)]),
 }

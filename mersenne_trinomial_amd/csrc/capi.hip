@@ -2,6 +2,7 @@
 // memory, streams and events.  Kernels live in square.hip, fft.hip and
 // sieve.hip.
 #include "gf2t_internal.hpp"
+#include "host_poly.hpp"
 #include "../../include/gf2t.h"
 #include <cstdarg>
 #include <cstdio>
@@ -42,6 +43,11 @@ extern "C" int gf2t_init(int device) {
     }
     GF2T_HIP(hipSetDevice(device));
     GF2T_HIP(hipFree(nullptr));   // create the context now
+    // with a device present, large gcd products go to the device
+    if (!gf2t_host::device_product_bits()) {
+        const char *e = getenv("GF2T_GCD_DEV_BITS");
+        gf2t_host::set_device_product_bits(e ? strtoull(e, nullptr, 10) : (1ULL << 18));
+    }
     return GF2T_OK;
 }
 

@@ -577,3 +577,146 @@ extern "C" int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_
     GF2T_CHECK_LAUNCH("gf2t_mulmod_slab");
     return GF2T_OK;
 }
+
+// --------------------------------------------- host-buffer product service
+// The device side of the reference's GpuMulHook (TsGpuMul, src/tsfactor.cu:
+// 467-607; interface src/gf2_gcd.h:155-172): host vectors in and out, one
+// mutex-guarded context per process with a private stream and device
+// buffers grown on demand (never shrunk).
+namespace {
+__global__ void k_mat2_pointwise(u64 *__restrict__ f0, const u64 *__restrict__ f1,
+                                 u64 *__restrict__ f2, const u64 *__restrict__ f3,
+                                 const u64 *__restrict__ fa, const u64 *__restrict__ fb, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        u64 a = fa[i], b = fb[i];
+        f0[i] = gf64_mul(f0[i], a) ^ gf64_mul(f1[i], b);
+        f2[i] = gf64_mul(f2[i], a) ^ gf64_mul(f3[i], b);
+    }
+}
+
+struct HostMulCtx {
+    std::mutex mu;
+    hipStream_t st = nullptr;
+    u64 *dbuf = nullptr;
+    size_t dcap = 0;   // words
+} g_hm;
+
+int hm_reserve(size_t words) {
+    if (!g_hm.st) GF2T_HIP(hipStreamCreate(&g_hm.st));
+    if (words <= g_hm.dcap) return GF2T_OK;
+    if (g_hm.dbuf) GF2T_HIP(hipFree(g_hm.dbuf));
+    g_hm.dbuf = nullptr;
+    g_hm.dcap = 0;
+    size_t want = words + words / 4;
+    if (hipMalloc(&g_hm.dbuf, want * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        gf2t::set_error("gf2t host product service: cannot allocate %zu bytes", want * 8);
+        return GF2T_ENOMEM;
+    }
+    g_hm.dcap = want;
+    return GF2T_OK;
+}
+
+u32 last_mask(u64 bits) { return (bits & 31) ? (u32)((1ULL << (bits & 31)) - 1) : 0xFFFFFFFFu; }
+}  // namespace
+
+extern "C" int gf2t_mul_host(const uint64_t *ha, uint64_t abits, const uint64_t *hb,
+                             uint64_t bbits, uint64_t *hout) {
+    if ((!ha && abits) || (!hb && bbits) || !hout) {
+        gf2t::set_error("gf2t_mul_host: null pointer");
+        return GF2T_EINVAL;
+    }
+    size_t ow = gf2t_mul_out_words(abits, bbits);
+    if (!abits || !bbits) {
+        std::memset(hout, 0, ow * 8);
+        return GF2T_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_hm.mu);
+    size_t aw = words_of(abits), bw = words_of(bbits);
+    size_t ww = (gf2t_mul_workspace_bytes(abits, bbits) + 7) / 8;
+    int rc = hm_reserve(aw + bw + ow + ww + 64);   // + room to align the workspace
+    if (rc) return rc;
+    u64 *da = g_hm.dbuf, *db = da + aw, *dout = db + bw, *dw = dout + ow;
+    dw = (u64 *)(((uintptr_t)dw + 255) & ~(uintptr_t)255);
+    hipStream_t st = g_hm.st;
+    GF2T_HIP(hipMemcpyAsync(da, ha, aw * 8, hipMemcpyHostToDevice, st));
+    GF2T_HIP(hipMemcpyAsync(db, hb, bw * 8, hipMemcpyHostToDevice, st));
+    rc = mul_core(da, abits, db, bbits, dout, dw, st);
+    if (rc) return rc;
+    GF2T_HIP(hipMemcpyAsync(hout, dout, ow * 8, hipMemcpyDeviceToHost, st));
+    GF2T_HIP(hipStreamSynchronize(st));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_mat2_host(const uint64_t *const hm[4], const uint64_t mbits[4],
+                              const uint64_t *ha, uint64_t abits, const uint64_t *hb,
+                              uint64_t bbits, uint64_t *hoa, uint64_t *hob, size_t outw) {
+    if (!hm || !mbits || !hoa || !hob) {
+        gf2t::set_error("gf2t_mat2_host: null pointer");
+        return GF2T_EINVAL;
+    }
+    u64 mb = 0;
+    for (int i = 0; i < 4; i++) {
+        if (!hm[i] && mbits[i]) { gf2t::set_error("gf2t_mat2_host: null entry"); return GF2T_EINVAL; }
+        mb = std::max<u64>(mb, mbits[i]);
+    }
+    u64 vb = std::max(abits, bbits);
+    if ((!ha && abits) || (!hb && bbits)) { gf2t::set_error("gf2t_mat2_host: null operand"); return GF2T_EINVAL; }
+    if (outw < gf2t_mul_out_words(mb, vb)) {
+        gf2t::set_error("gf2t_mat2_host: outw %zu below %zu", outw, gf2t_mul_out_words(mb, vb));
+        return GF2T_EINVAL;
+    }
+    std::memset(hoa, 0, outw * 8);
+    std::memset(hob, 0, outw * 8);
+    if (!mb || !vb) return GF2T_OK;
+    u64 cm = (mb + 31) >> 5, cv = (vb + 31) >> 5;
+    int m = fft_m_for(cm + cv - 1);
+    if (m > FFT_MAX_M) { gf2t::set_error("gf2t_mat2_host: product too large"); return GF2T_ERANGE; }
+    std::lock_guard<std::mutex> lk(g_hm.mu);
+    int rc = ensure_basis();
+    if (rc) return rc;
+    const gf2t_fft_plan *plan = cached_plan(m);
+    size_t n = (size_t)1 << m, ow = gf2t_mul_out_words(mb, vb);
+    size_t in_w[6] = {words_of(mbits[0]), words_of(mbits[1]), words_of(mbits[2]),
+                      words_of(mbits[3]), words_of(abits), words_of(bbits)};
+    size_t stage = 0;
+    for (size_t w : in_w) stage += w + 4;
+    rc = hm_reserve(6 * n + 2 * ow + stage + 64);
+    if (rc) return rc;
+    hipStream_t st = g_hm.st;
+    u64 *F[6];
+    for (int i = 0; i < 6; i++) F[i] = g_hm.dbuf + (size_t)i * n;
+    u64 *dout = g_hm.dbuf + 6 * n, *dst = dout + 2 * ow;
+    const u64 *src[6] = {hm[0], hm[1], hm[2], hm[3], ha, hb};
+    const u64 bits[6] = {mbits[0], mbits[1], mbits[2], mbits[3], abits, bbits};
+    for (int i = 0; i < 6; i++) {
+        u64 c = (bits[i] + 31) >> 5;
+        if (!c) {
+            GF2T_HIP(hipMemsetAsync(F[i], 0, n * 8, st));
+            continue;
+        }
+        GF2T_HIP(hipMemcpyAsync(dst, src[i], in_w[i] * 8, hipMemcpyHostToDevice, st));
+        rc = run_passes(plan->fwd, F[i], st, (const u32 *)dst, c, last_mask(bits[i]));
+        if (rc) return rc;
+        dst += in_w[i] + 4;
+    }
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_mat2_pointwise, dim3(blocks), dim3(256), 0, st, F[0], F[1], F[2], F[3],
+                       F[4], F[5], n);
+    rc = run_passes(plan->inv, F[0], st, nullptr, 0, 0);
+    if (rc) return rc;
+    rc = run_passes(plan->inv, F[2], st, nullptr, 0, 0);
+    if (rc) return rc;
+    size_t nch = (size_t)(cm + cv - 1);
+    hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0, st, F[0],
+                       nch, dout, ow, (u64)0, (u64)0);
+    hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0, st, F[2],
+                       nch, dout + ow, ow, (u64)0, (u64)0);
+    GF2T_CHECK_LAUNCH("gf2t_mat2_host");
+    GF2T_HIP(hipMemcpyAsync(hoa, dout, ow * 8, hipMemcpyDeviceToHost, st));
+    GF2T_HIP(hipMemcpyAsync(hob, dout + ow, ow * 8, hipMemcpyDeviceToHost, st));
+    GF2T_HIP(hipStreamSynchronize(st));
+    return GF2T_OK;
+}

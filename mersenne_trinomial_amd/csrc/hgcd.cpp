@@ -1,0 +1,324 @@
+// hgcd.cpp -- subquadratic gcd in GF(2)[x] (half-gcd recursion), the role
+// of the reference's poly_gcd_ntl / poly_gcd_hybrid (src/gf2_gcd_ntl.cpp:
+// 32-42, :354-452): products below a size threshold run as Karatsuba over
+// PCLMULQDQ on the calling host thread, products above it on the device
+// through the Cantor-FFT product service (gf2t_mul_host, gf2t_mat2_host).
+//
+// Remainder sequence of (a, b), deg a > deg b:  r0 = a, r1 = b,
+// r(i+1) = r(i-1) + q(i) r(i).  One step is the matrix [[0,1],[1,q]]
+// acting on the column (r(i-1), r(i)); a product of steps M maps (a, b) to
+// a pair of consecutive remainders.  hgcd(a, b) returns the M that reaches
+// the pair (c, d) with deg c >= ceil(deg a / 2) > deg d.  Its first half
+// recurses on the top coefficients of (a, b) only (the quotients while
+// the degree stays above that half depend on nothing else), applies the
+// result to the full pair, takes one explicit division step and recurses
+// once more on the top of the new pair.
+#include "host_poly.hpp"
+#include "gf2t.h"
+#include <wmmintrin.h>
+#include <smmintrin.h>
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace gf2t_host {
+
+namespace {
+
+std::atomic<u64> g_dev_bits{0};   // 0: host products only
+
+constexpr size_t KBASE = 24;           // words: schoolbook below this
+constexpr int64_t HGCD_BASE = 1536;    // bits: step-by-step half-gcd below this
+constexpr int64_t GCD_BASE = 4096;     // bits: plain Euclid below this
+
+inline void clmul(u64 a, u64 b, u64 &lo, u64 &hi) {
+    __m128i p = _mm_clmulepi64_si128(_mm_cvtsi64_si128((long long)a),
+                                     _mm_cvtsi64_si128((long long)b), 0);
+    lo = (u64)_mm_cvtsi128_si64(p);
+    hi = (u64)_mm_extract_epi64(p, 1);
+}
+
+// out[0, na+nb) ^= a[0,na) * b[0,nb)
+void school(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
+    for (size_t i = 0; i < na; i++) {
+        u64 ai = a[i];
+        if (!ai) continue;
+        for (size_t j = 0; j < nb; j++) {
+            u64 lo, hi;
+            clmul(ai, b[j], lo, hi);
+            out[i + j] ^= lo;
+            out[i + j + 1] ^= hi;
+        }
+    }
+}
+
+// out[0, 2n) = a[0,n) * b[0,n); ws holds at least 4n words per level
+void kara(const u64 *a, const u64 *b, size_t n, u64 *out, u64 *ws) {
+    if (n <= KBASE) {
+        std::memset(out, 0, 2 * n * 8);
+        school(a, n, b, n, out);
+        return;
+    }
+    size_t h = (n + 1) / 2, l = n - h;
+    u64 *sa = ws, *sb = ws + h, *z1 = ws + 2 * h, *next = ws + 4 * h;
+    for (size_t i = 0; i < h; i++) {
+        sa[i] = a[i] ^ (i < l ? a[h + i] : 0);
+        sb[i] = b[i] ^ (i < l ? b[h + i] : 0);
+    }
+    kara(a, b, h, out, next);                 // z0 -> out[0, 2h)
+    kara(a + h, b + h, l, out + 2 * h, next); // z2 -> out[2h, 2n)
+    kara(sa, sb, h, z1, next);                // z1 (2h words)
+    for (size_t i = 0; i < 2 * h; i++) z1[i] ^= out[i];
+    for (size_t i = 0; i < 2 * l; i++) z1[i] ^= out[2 * h + i];
+    for (size_t i = 0; i < 2 * h; i++) out[h + i] ^= z1[i];
+}
+
+size_t words(const Poly &a) {
+    int64_t d = deg(a);
+    return d < 0 ? 0 : (size_t)(d >> 6) + 1;
+}
+
+void check_rc(int rc, const char *what) {
+    if (rc != GF2T_OK)
+        throw std::runtime_error(std::string(what) + ": " + gf2t_last_error());
+}
+
+Poly host_mul(const u64 *a, size_t na, const u64 *b, size_t nb) {
+    if (na < nb) {
+        std::swap(a, b);
+        std::swap(na, nb);
+    }
+    Poly out(na + nb + 1, 0);
+    if (nb <= KBASE) {
+        school(a, na, b, nb, out.data());
+        trim(out);
+        return out;
+    }
+    // a in blocks of nb words, each block times b by Karatsuba
+    std::vector<u64> ws(8 * nb + 64), blk(nb), part(2 * nb);
+    for (size_t off = 0; off < na; off += nb) {
+        size_t len = std::min(nb, na - off);
+        std::memcpy(blk.data(), a + off, len * 8);
+        if (len < nb) std::memset(blk.data() + len, 0, (nb - len) * 8);
+        kara(blk.data(), b, nb, part.data(), ws.data());
+        size_t lim = std::min(2 * nb, out.size() - off);
+        for (size_t i = 0; i < lim; i++) out[off + i] ^= part[i];
+    }
+    trim(out);
+    return out;
+}
+
+Poly add(const Poly &a, const Poly &b) {
+    Poly out = a.size() >= b.size() ? a : b;
+    const Poly &o = a.size() >= b.size() ? b : a;
+    for (size_t i = 0; i < o.size(); i++) out[i] ^= o[i];
+    trim(out);
+    return out;
+}
+
+// a div x^k
+Poly shr(const Poly &a, u64 k) {
+    size_t ws = (size_t)(k >> 6), n = words(a);
+    unsigned bs = (unsigned)(k & 63);
+    if (ws >= n) return Poly();
+    Poly out(n - ws);
+    for (size_t i = 0; i < out.size(); i++) {
+        u64 lo = a[ws + i] >> bs;
+        u64 hi = (bs && ws + i + 1 < n) ? a[ws + i + 1] << (64 - bs) : 0;
+        out[i] = lo | hi;
+    }
+    trim(out);
+    return out;
+}
+
+// a mod x^k
+Poly low(const Poly &a, u64 k) {
+    size_t n = std::min(words(a), (size_t)((k + 63) >> 6));
+    Poly out(a.begin(), a.begin() + n);
+    if (n && (k & 63) && n == (size_t)((k + 63) >> 6)) out[n - 1] &= (1ULL << (k & 63)) - 1;
+    trim(out);
+    return out;
+}
+
+void divmod(const Poly &a, const Poly &b, Poly &q, Poly &r) {
+    int64_t db = deg(b), dr = deg(a);
+    if (db < 0) throw std::runtime_error("gcd: division by zero");
+    r = a;
+    r.resize(std::max(r.size(), words(b)) + 2, 0);
+    q.assign(dr >= db ? (size_t)((dr - db) >> 6) + 1 : 1, 0);
+    size_t bw = words(b);
+    while (dr >= db) {
+        u64 sh = (u64)(dr - db);
+        q[sh >> 6] |= 1ULL << (sh & 63);
+        xor_shift_words(r.data(), b.data(), bw, sh);
+        dr = deg_w(r.data(), (size_t)(dr >> 6) + 1);
+    }
+    trim(q);
+    trim(r);
+}
+
+// (x, y) -> (M00 x + M01 y, M10 x + M11 y)
+void apply(const HMat &M, const Poly &x, const Poly &y, Poly &ox, Poly &oy) {
+    int64_t dm = std::max(std::max(deg(M.m00), deg(M.m01)), std::max(deg(M.m10), deg(M.m11)));
+    int64_t dv = std::max(deg(x), deg(y));
+    u64 lim = g_dev_bits.load(std::memory_order_relaxed);
+    if (lim && dm + 1 >= (int64_t)lim && dv + 1 >= (int64_t)lim) {
+        const Poly *e[4] = {&M.m00, &M.m01, &M.m10, &M.m11};
+        const uint64_t *hm[4];
+        uint64_t mb[4];
+        for (int i = 0; i < 4; i++) {
+            hm[i] = e[i]->data();
+            mb[i] = (uint64_t)(deg(*e[i]) + 1);
+        }
+        size_t ow = gf2t_mul_out_words((u64)dm + 1, (u64)dv + 1);
+        ox.assign(ow, 0);
+        oy.assign(ow, 0);
+        check_rc(gf2t_mat2_host(hm, mb, x.data(), (u64)(deg(x) + 1), y.data(),
+                                (u64)(deg(y) + 1), ox.data(), oy.data(), ow),
+                 "gf2t_mat2_host");
+        trim(ox);
+        trim(oy);
+        return;
+    }
+    ox = add(mul_fast(M.m00, x), mul_fast(M.m01, y));
+    oy = add(mul_fast(M.m10, x), mul_fast(M.m11, y));
+}
+
+// step-by-step half-gcd for small inputs
+void hgcd_base(const Poly &a, const Poly &b, int64_t m, HMat &M, Poly &c, Poly &d) {
+    M = HMat{Poly{1}, Poly(), Poly(), Poly{1}};
+    c = a;
+    d = b;
+    Poly q, r;
+    while (deg(d) >= m) {
+        divmod(c, d, q, r);
+        Poly n10 = add(M.m00, mul_fast(q, M.m10));
+        Poly n11 = add(M.m01, mul_fast(q, M.m11));
+        M.m00.swap(M.m10);
+        M.m01.swap(M.m11);
+        M.m10.swap(n10);
+        M.m11.swap(n11);
+        c.swap(d);
+        d.swap(r);
+    }
+}
+
+}  // namespace
+
+void xor_shift_words(u64 *a, const u64 *b, size_t bw, u64 sh) {
+    size_t ws = (size_t)(sh >> 6);
+    unsigned bs = (unsigned)(sh & 63);
+    if (!bs) {
+        for (size_t j = 0; j < bw; j++) a[j + ws] ^= b[j];
+        return;
+    }
+    u64 carry = 0;
+    for (size_t j = 0; j < bw; j++) {
+        u64 v = b[j];
+        a[j + ws] ^= (v << bs) | carry;
+        carry = v >> (64 - bs);
+    }
+    a[bw + ws] ^= carry;
+}
+
+void set_device_product_bits(u64 bits) { g_dev_bits.store(bits); }
+u64 device_product_bits() { return g_dev_bits.load(); }
+
+Poly mul_fast(const Poly &a, const Poly &b) {
+    size_t na = words(a), nb = words(b);
+    if (!na || !nb) return Poly();
+    u64 lim = g_dev_bits.load(std::memory_order_relaxed);
+    int64_t da = deg(a), db = deg(b);
+    if (lim && da + 1 >= (int64_t)lim && db + 1 >= (int64_t)lim) {
+        Poly out(gf2t_mul_out_words((u64)da + 1, (u64)db + 1), 0);
+        check_rc(gf2t_mul_host(a.data(), (u64)da + 1, b.data(), (u64)db + 1, out.data()),
+                 "gf2t_mul_host");
+        trim(out);
+        return out;
+    }
+    return host_mul(a.data(), na, b.data(), nb);
+}
+
+void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d) {
+    int64_t n = deg(a);
+    int64_t m = (n + 1) / 2;
+    if (deg(b) < m) {
+        M = HMat{Poly{1}, Poly(), Poly(), Poly{1}};
+        c = a;
+        d = b;
+        trim(c);
+        trim(d);
+        return;
+    }
+    if (n < HGCD_BASE) {
+        hgcd_base(a, b, m, M, c, d);
+        return;
+    }
+    // first half: the top n - m + 1 coefficients
+    HMat R;
+    Poly c0, d0;
+    hgcd(shr(a, (u64)m), shr(b, (u64)m), R, c0, d0);
+    Poly a1, b1;
+    apply(R, low(a, (u64)m), low(b, (u64)m), a1, b1);
+    xor_shift(a1, c0, (u64)m);
+    xor_shift(b1, d0, (u64)m);
+    trim(a1);
+    trim(b1);
+    if (deg(b1) < m) {
+        M = std::move(R);
+        c = std::move(a1);
+        d = std::move(b1);
+        return;
+    }
+    // one division step: (a1, b1) -> (b1, a1 mod b1)
+    Poly q, rem;
+    divmod(a1, b1, q, rem);
+    HMat R2{R.m10, R.m11, add(R.m00, mul_fast(q, R.m10)), add(R.m01, mul_fast(q, R.m11))};
+    if (deg(rem) < m) {
+        M = std::move(R2);
+        c = std::move(b1);
+        d = std::move(rem);
+        return;
+    }
+    // second half on the top 2(l - m) + 1 coefficients of (b1, rem)
+    int64_t l = deg(b1), k = 2 * m - l;
+    HMat S;
+    Poly c3, d3;
+    hgcd(shr(b1, (u64)k), shr(rem, (u64)k), S, c3, d3);
+    apply(S, low(b1, (u64)k), low(rem, (u64)k), c, d);
+    xor_shift(c, c3, (u64)k);
+    xor_shift(d, d3, (u64)k);
+    trim(c);
+    trim(d);
+    // M = S * R2, by columns: S applied to (R2.m00, R2.m10) and (R2.m01, R2.m11)
+    apply(S, R2.m00, R2.m10, M.m00, M.m10);
+    apply(S, R2.m01, R2.m11, M.m01, M.m11);
+}
+
+Poly gcd_fast(const Poly &a0, const Poly &b0) {
+    Poly a = a0, b = b0;
+    trim(a);
+    trim(b);
+    if (deg(a) < deg(b)) std::swap(a, b);
+    if (deg(b) >= 0 && deg(a) == deg(b)) {   // make deg a > deg b
+        Poly t = add(a, b);
+        a.swap(b);
+        b.swap(t);
+        if (deg(a) < deg(b)) std::swap(a, b);
+    }
+    while (deg(b) >= GCD_BASE) {
+        HMat M;
+        Poly c, d;
+        hgcd(a, b, M, c, d);
+        if (deg(d) < 0) return c;
+        Poly q, r;
+        divmod(c, d, q, r);
+        a.swap(d);
+        b.swap(r);
+    }
+    return gcd(a, b);
+}
+
+}  // namespace gf2t_host

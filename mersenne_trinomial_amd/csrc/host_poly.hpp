@@ -45,4 +45,22 @@ Poly edf_least(const Poly &t, u64 d);
 // Swan bound of the reference (src/tsfactor.cu:1101-1107)
 u64 swan_rhigh(u64 r, u64 s, bool *swan);
 
+// ---- subquadratic products and gcd (hgcd.cpp)
+// a[...] ^= b[0, bw) << sh; a must hold bw + sh/64 + 1 words
+void xor_shift_words(u64 *a, const u64 *b, size_t bw, u64 sh);
+// Products where both operands have at least `bits` bits go to the
+// device product service; 0 keeps every product on the host.
+void set_device_product_bits(u64 bits);
+u64 device_product_bits();
+Poly mul_fast(const Poly &a, const Poly &b);       // Karatsuba / device FFT
+// The 2x2 step matrix of a stretch of the remainder sequence: it maps
+// (a, b) to (m00 a + m01 b, m10 a + m11 b).
+struct HMat {
+    Poly m00, m01, m10, m11;
+};
+// deg a > deg b >= -1: M and (c, d) = M (a, b), consecutive remainders
+// of (a, b) with deg c >= ceil(deg a / 2) > deg d
+void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d);
+Poly gcd_fast(const Poly &a, const Poly &b);      // monic; same as gcd()
+
 }  // namespace gf2t_host

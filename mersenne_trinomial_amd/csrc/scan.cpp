@@ -26,7 +26,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <functional>
+#include <mutex>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -49,20 +52,32 @@ double now_s() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// runs fn(0..n-1) on up to `threads` threads; the first exception a
+// call throws stops the remaining work and is rethrown here
 void parallel_for(size_t n, int threads, const std::function<void(size_t)> &fn) {
     if (n == 0) return;
     int t = std::max(1, std::min<int>(threads, (int)n));
     std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr err;
+    std::mutex err_mu;
     std::vector<std::thread> pool;
     for (int i = 0; i < t; i++)
         pool.emplace_back([&] {
             for (;;) {
                 size_t k = next.fetch_add(1);
-                if (k >= n) break;
-                fn(k);
+                if (k >= n || failed.load()) break;
+                try {
+                    fn(k);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lk(err_mu);
+                    if (!err) err = std::current_exception();
+                    failed.store(true);
+                }
             }
         });
     for (auto &th : pool) th.join();
+    if (err) std::rethrow_exception(err);
 }
 
 struct DevMem {
@@ -135,7 +150,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         std::vector<Poly> g(n);
         parallel_for((size_t)n, P.gcd_threads > 0 ? P.gcd_threads : 8, [&](size_t i) {
             Poly a = from_words(acc_host.data() + i * pitch, nw);
-            g[i] = gcd(a, trinomial(r, live[i].s));
+            g[i] = gcd_fast(trinomial(r, live[i].s), a);
         });
         // resolve hits (all earlier intervals of these s were clear)
         std::vector<int> hit;
@@ -220,9 +235,9 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
         gf2t::set_error("gf2t_scan_run: k0 (the depth the sieve reached) must be >= 1");
         return GF2T_EINVAL;
     }
-    if (r > ((uint64_t)1 << 23)) {
-        gf2t::set_error("gf2t_scan_run: r=%llu needs a subquadratic gcd; the host gcd here is "
-                        "O(r^2) and limited to r <= 2^23", (unsigned long long)r);
+    if (r >= ((uint64_t)1 << 31)) {
+        gf2t::set_error("gf2t_scan_run: r=%llu beyond the product size limit",
+                        (unsigned long long)r);
         return GF2T_ERANGE;
     }
     auto *res = new gf2t_scan_results();
@@ -263,7 +278,12 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
         for (size_t off = 0; off < G.live.size(); off += cap) {
             std::vector<Live> part(G.live.begin() + off,
                                    G.live.begin() + std::min(G.live.size(), off + cap));
-            rc = run_cohort(r, P, G.rhigh, G.limited, part, res->rows, res->hex);
+            try {
+                rc = run_cohort(r, P, G.rhigh, G.limited, part, res->rows, res->hex);
+            } catch (const std::exception &e) {
+                gf2t::set_error("gf2t_scan_run: %s", e.what());
+                rc = GF2T_EHIP;
+            }
             if (rc) { delete res; return rc; }
         }
     for (size_t i = 0; i < n; i++) res->rows[i].hex_len = res->hex[i].size();
@@ -309,7 +329,13 @@ extern "C" void gf2t_scan_results_free(gf2t_scan_results *res) { delete res; }
 
 extern "C" int64_t gf2t_gcd_host(const uint64_t *a, size_t aw, const uint64_t *b, size_t bw,
                                  uint64_t *g, size_t gcap) {
-    Poly G = gcd(from_words(a, aw), from_words(b, bw));
+    Poly G;
+    try {
+        G = gcd_fast(from_words(a, aw), from_words(b, bw));
+    } catch (const std::exception &e) {
+        gf2t::set_error("gf2t_gcd_host: %s", e.what());
+        return GF2T_EHIP;
+    }
     int64_t d = deg(G);
     if (g && gcap) {
         memset(g, 0, gcap * 8);

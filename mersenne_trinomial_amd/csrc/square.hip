@@ -17,6 +17,7 @@
 // F^2); the second read of the upper input half by window 3 hits in
 // L2 / Infinity Cache.
 #include "gf2t_internal.hpp"
+#include "host_poly.hpp"
 #include "../../include/gf2t.h"
 #include <vector>
 #include <algorithm>
@@ -234,6 +235,10 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
     if (!strcmp(key, "sq_ppt")) { g_sq_ppt = (int)value; return GF2T_OK; }
     if (!strcmp(key, "sq_order")) { g_sq_order = (int)value; return GF2T_OK; }
     if (!strcmp(key, "sq_nt")) { g_sq_nt = (int)value; return GF2T_OK; }
+    if (!strcmp(key, "gcd_dev_bits")) {
+        gf2t_host::set_device_product_bits(value > 0 ? (uint64_t)value : 0);
+        return GF2T_OK;
+    }
     gf2t::set_error("gf2t_tune: unknown key '%s'", key);
     return GF2T_EINVAL;
 }

@@ -77,3 +77,12 @@ def test_scan_sieve_survivors_r132049(eng):
             assert (kind, d, hx) == (1, o.d, o.hex), o.line()
             checked_min += 1
     assert checked_min >= 5
+
+
+def test_scan_full_size_r136279841_s474(eng):
+    # the deep scan at the full residue size: interval (32, 64] of
+    # x^136279841 + x^474 + 1 holds its least factor, degree 41
+    # (src/tsfactor_handoff.md:337-338); the interval gcd runs as a
+    # half-gcd with device products
+    got = eng.scan(136279841, [474], k0=32, maxd=64, L0=64, Lmax=64)
+    assert [o.line() for o in got] == ["474 41 p3991ef16129"]

@@ -50,6 +50,46 @@ def test_gcd_host_vs_oracle(lib):
     assert d == O.gcd(a, t)[0]
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_half_gcd_vs_euclid(lib, seed):
+    # sizes that cross the half-gcd base case several times over, planted
+    # common factors of every size, coprime pairs, equal degrees
+    rng = np.random.default_rng(100 + seed)
+    for trial in range(40):
+        gb = int(rng.integers(1, 6000))
+        g = rand_poly(rng, gb)
+        ab, bb = int(rng.integers(1, 30000)), int(rng.integers(1, 30000))
+        if trial % 5 == 4:
+            bb = ab
+        a = O.mul_comb(g, gb, rand_poly(rng, ab), ab)
+        b = O.mul_comb(g, gb, rand_poly(rng, bb), bb) if trial % 3 else rand_poly(rng, bb)
+        cap = max(len(a), len(b)) + 1
+        out = np.zeros(cap, dtype=np.uint64)
+        d = lib.gf2t_gcd_host(ptr(a), len(a), ptr(b), len(b), ptr(out), cap)
+        d2, g2 = O.gcd(a, b)
+        n = O.words(d2 + 1)
+        assert d == d2, (seed, trial)
+        assert np.array_equal(out[:n], g2[:n]) and not np.any(out[n:])
+
+
+def test_half_gcd_trinomial_planted_factor(lib):
+    # T = x^756839 + x^s + 1 against (residue with a planted small factor
+    # of T): the gcd is that factor
+    rng = np.random.default_rng(9)
+    r = 756839
+    best, _ = O.sieve(r, 14, 2000)
+    s = int(np.nonzero(best[1:] != O.NONE)[0][-1]) + 1
+    f = O.poly_from_hex(f"{int(best[s]) & 0xFFFFFFFFFFFF:x}")
+    d = int(best[s]) >> 48
+    t = O.trinomial(r, s)
+    b = O.mul_comb(f, d + 1, rand_poly(rng, r - 100), r - 100)
+    out = np.zeros(len(t) + 1, dtype=np.uint64)
+    got = lib.gf2t_gcd_host(ptr(t), len(t), ptr(b), len(b), ptr(out), len(out))
+    want, gw = O.gcd(t, b)
+    assert got == want and want >= d
+    assert np.array_equal(out[:len(gw)], gw)
+
+
 def irreducibles(d, count, rng):
     found = []
     while len(found) < count:
