@@ -190,6 +190,42 @@ int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_t *db,
                      uint64_t *dout, size_t opitch, int n, void *dwork,
                      void *stream);
 
+/* ------------------------------------------------------------------ *
+ * Blocked distinct-degree factorization on slabs (tsfactor's h-table:
+ * TsGPU::init_h_at / square_all_step / horner_C, src/tsfactor.cu:
+ * 71-126, :278-300, :302-307, :345-365), for a cohort of s values.
+ * h-table of n residues: slot j (j < m) of residue i at
+ * dh + (i*m + j)*slot_pitch; slot j holds e_(j+1)(x^(2^(k+1)), ...,
+ * x^(2^(k+m))) for the block of degrees k+1..k+m.  ds: DEVICE array of
+ * the n values s_i.  Slab arrays: residue i at base + i*pitch.
+ * ------------------------------------------------------------------ */
+/* Pascal build of the h-table for block start kstart (degrees kstart..
+ * kstart+m-1), into dh; dh_alt is an equal-sized scratch table.
+ * 1 <= m <= 63. */
+int gf2t_ddf_init_slab(uint64_t *dh, uint64_t *dh_alt, size_t slot_pitch, int m,
+                       uint64_t r, const uint64_t *ds, uint64_t kstart, int n,
+                       void *stream);
+/* n residues (pitch words apart) squared once; residue y uses
+ * ds[y / group] -- with group = m this advances a whole h-table by one
+ * degree (square_all_step).  No limit on n. */
+int gf2t_square_fold_slab_g(const uint64_t *din, uint64_t *dout, size_t pitch,
+                            uint64_t r, const uint64_t *ds, size_t n, int group,
+                            void *stream);
+/* dc_i = x^m + sum_j h_(i,j) x^(m-1-j) mod T_i = prod_t (x + x^(2^(k+1+t)))
+ * (horner_C: k_horner_fused + k_horner_tail); 2m < r. */
+int gf2t_horner_slab(const uint64_t *dh, size_t slot_pitch, int m, uint64_t r,
+                     const uint64_t *ds, uint64_t *dc, size_t cpitch, int n,
+                     void *stream);
+/* slab helpers: dst_i ^= x src_i mod T_i (k_mulx_xor); p_i[0] ^= v;
+ * p_i = 1; dst_i = src_i */
+int gf2t_mulx_xor_slab(uint64_t *ddst, size_t dpitch, const uint64_t *dsrc,
+                       size_t spitch, uint64_t r, const uint64_t *ds, int n,
+                       void *stream);
+int gf2t_xor_word0_slab(uint64_t *dp, size_t pitch, uint64_t v, int n, void *stream);
+int gf2t_set_one_slab(uint64_t *dp, size_t pitch, size_t nw, int n, void *stream);
+int gf2t_copy_slab(uint64_t *ddst, size_t dpitch, const uint64_t *dsrc,
+                   size_t spitch, size_t nw, int n, void *stream);
+
 /* Host-buffer product service: the device side of the reference's
  * GpuMulHook (TsGpuMul::mul / mat2_apply, src/tsfactor.cu:467-607;
  * interface src/gf2_gcd.h:155-172).  Host pointers in and out; the
@@ -247,6 +283,9 @@ typedef struct gf2t_scan_params {
     int gcd_threads;  /* host threads for interval gcds (0 = 8)          */
     int batch;        /* max s values per device cohort (0 = by memory)  */
     int verbose;
+    int block_m;      /* degrees per product: 1 = one product per degree
+                         (trinomial_scan), m > 1 = blocked h-table scheme
+                         (tsfactor), 0 = chosen from r                  */
 } gf2t_scan_params;
 
 typedef struct gf2t_scan_result {

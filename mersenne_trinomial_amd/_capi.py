@@ -29,7 +29,8 @@ class GF2TError(RuntimeError):
 
 class ScanParams(C.Structure):
     _fields_ = [("k0", _u64), ("maxd", _u64), ("L0", _u64), ("Lmax", _u64),
-                ("gcd_threads", C.c_int), ("batch", C.c_int), ("verbose", C.c_int)]
+                ("gcd_threads", C.c_int), ("batch", C.c_int), ("verbose", C.c_int),
+                ("block_m", C.c_int)]
 
 
 class ScanResult(C.Structure):
@@ -98,6 +99,13 @@ _SIGS = {
     "gf2t_scan_results_free": (None, [_vp]),
     "gf2t_gcd_host": (C.c_int64, [_p64, _sz, _p64, _sz, _p64, _sz]),
     "gf2t_mul_host": (C.c_int, [_p64, _u64, _p64, _u64, _p64]),
+    "gf2t_ddf_init_slab": (C.c_int, [_vp, _vp, _sz, C.c_int, _u64, _vp, _u64, C.c_int, _vp]),
+    "gf2t_square_fold_slab_g": (C.c_int, [_vp, _vp, _sz, _u64, _vp, _sz, C.c_int, _vp]),
+    "gf2t_horner_slab": (C.c_int, [_vp, _sz, C.c_int, _u64, _vp, _vp, _sz, C.c_int, _vp]),
+    "gf2t_mulx_xor_slab": (C.c_int, [_vp, _sz, _vp, _sz, _u64, _vp, C.c_int, _vp]),
+    "gf2t_xor_word0_slab": (C.c_int, [_vp, _sz, _u64, C.c_int, _vp]),
+    "gf2t_set_one_slab": (C.c_int, [_vp, _sz, _sz, C.c_int, _vp]),
+    "gf2t_copy_slab": (C.c_int, [_vp, _sz, _vp, _sz, _sz, C.c_int, _vp]),
     "gf2t_mat2_host": (C.c_int, [C.POINTER(_p64), _p64, _p64, _u64, _p64, _u64, _p64, _p64, _sz]),
     "gf2t_edf_least": (C.c_int, [_p64, _sz, _u64, _p64, _sz]),
     "gf2t_swan_bound": (_u64, [_u64, _u64, C.POINTER(C.c_int)]),

@@ -199,12 +199,71 @@ static Poly x_plus(const Poly &h) {
     return t;
 }
 
+Modulus::Modulus(const Poly &g) : g_(g) {
+    trim(g_);
+    n_ = deg(g_);
+    barrett_ = n_ >= 256;
+    if (barrett_) {
+        Poly x2n((size_t)((2 * n_) >> 6) + 1, 0);
+        x2n[(size_t)((2 * n_) >> 6)] = 1ULL << ((2 * n_) & 63);
+        // mu = x^(2n) div g by long division
+        Poly r = x2n;
+        int64_t dr = 2 * n_;
+        mu_.assign((size_t)(n_ >> 6) + 1, 0);
+        size_t gw = (size_t)(n_ >> 6) + 1;
+        r.resize(r.size() + 2, 0);
+        while (dr >= n_) {
+            u64 sh = (u64)(dr - n_);
+            mu_[sh >> 6] |= 1ULL << (sh & 63);
+            xor_shift_words(r.data(), g_.data(), gw, sh);
+            dr = deg_w(r.data(), (size_t)(dr >> 6) + 1);
+        }
+        trim(mu_);
+    }
+}
+
+static Poly shr_bits(const Poly &a, u64 k) {
+    size_t ws = (size_t)(k >> 6);
+    unsigned bs = (unsigned)(k & 63);
+    if (ws >= a.size()) return Poly();
+    Poly out(a.size() - ws);
+    for (size_t i = 0; i < out.size(); i++) {
+        u64 lo = a[ws + i] >> bs;
+        u64 hi = (bs && ws + i + 1 < a.size()) ? a[ws + i + 1] << (64 - bs) : 0;
+        out[i] = lo | hi;
+    }
+    trim(out);
+    return out;
+}
+
+Poly Modulus::reduce(const Poly &a0) const {
+    Poly a = a0;
+    trim(a);
+    int64_t da = deg(a);
+    if (da < n_) return a;
+    if (!barrett_ || da >= 2 * n_) {
+        mod_inplace(a, g_);
+        return a;
+    }
+    // q = ((a div x^n) * mu) div x^n; a - q g has degree < n
+    Poly q = shr_bits(mul_fast(shr_bits(a, (u64)n_), mu_), (u64)n_);
+    Poly qg = mul_fast(q, g_);
+    if (a.size() < qg.size()) a.resize(qg.size(), 0);
+    for (size_t i = 0; i < qg.size(); i++) a[i] ^= qg[i];
+    trim(a);
+    if (deg(a) >= n_) mod_inplace(a, g_);
+    return a;
+}
+
+Poly Modulus::sqr(const Poly &a) const { return reduce(gf2t_host::sqr(a)); }
+Poly Modulus::mul(const Poly &a, const Poly &b) const { return reduce(mul_fast(a, b)); }
+
 bool resolve_least(const Poly &g, u64 ka, u64 kb, const Poly &h0, u64 &d, Poly &mask) {
-    Poly h = h0;
-    mod_inplace(h, g);
+    Modulus G(g);
+    Poly h = G.reduce(h0);
     for (u64 k = ka; k <= kb; k++) {
-        h = sqrmod(h, g);
-        Poly t = gcd(x_plus(h), g);
+        h = G.sqr(h);
+        Poly t = gcd_fast(x_plus(h), g);
         if (deg(t) > 0) {
             d = k;
             mask = ((u64)deg(t) == k) ? t : edf_least(t, k);
@@ -242,18 +301,19 @@ Poly edf_least(const Poly &t0, u64 d) {
     if (dt <= 0) return Poly();
     if ((u64)dt == d) return t;
     u64 state = 0x243F6A8885A308D3ULL ^ (u64)dt;
+    Modulus T(t);
     for (int tries = 0; tries < 200; tries++) {
         Poly a = pr_elem(state, dt);
         if (deg(a) < 1) continue;
         Poly tr = a, p = a;
         for (u64 i = 1; i < d; i++) {
-            p = sqrmod(p, t);
+            p = T.sqr(p);
             Poly acc = tr;
             xor_shift(acc, p, 0);
             trim(acc);
             tr.swap(acc);
         }
-        Poly u = gcd(tr, t);
+        Poly u = gcd_fast(tr, t);
         int64_t du = deg(u);
         if (du > 0 && du < dt) {
             Poly v = divexact(t, u);

@@ -33,6 +33,21 @@ std::string hex(const Poly &a);                   // lowercase, no leading zero
 Poly from_words(const u64 *w, size_t n);
 Poly trinomial(u64 r, u64 s);
 
+// Reduction modulo a fixed g: Barrett (mu = x^(2n) div g, n = deg g)
+// with Karatsuba / device products for large g, shift-and-add otherwise.
+class Modulus {
+public:
+    explicit Modulus(const Poly &g);
+    Poly reduce(const Poly &a) const;
+    Poly sqr(const Poly &a) const;
+    Poly mul(const Poly &a, const Poly &b) const;
+    const Poly &poly() const { return g_; }
+private:
+    Poly g_, mu_;
+    int64_t n_;
+    bool barrett_;
+};
+
 // least factor degree d in (ka-1, kb] of g, whose irreducible factors all
 // have degrees in [ka, kb], and the numerically least degree-d factor.
 // h0 = x^(2^(ka-1)) mod g.  Returns false if nothing is found.

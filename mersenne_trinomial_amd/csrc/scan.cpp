@@ -102,67 +102,108 @@ struct Live {
     u64 s;
 };
 
+// x^(2^k) mod g on the host (g small: the interval gcd)
+Poly frob_mod(const Poly &g, u64 k) {
+    Modulus G(g);
+    Poly h = G.reduce(Poly{2});
+    for (u64 i = 0; i < k; i++) h = G.sqr(h);
+    return h;
+}
+
+// degrees per product: per degree the blocked scheme costs m slab
+// squarings plus 1/m of a slab product
+int block_for(const gf2t_scan_params &P, u64 r) {
+    int m = P.block_m > 0 ? P.block_m : (r < (1u << 14) ? 4 : 16);
+    if (m > 63) m = 63;
+    while (m > 1 && 2 * (u64)m >= r) m /= 2;
+    return m;
+}
+
+// One cohort of s values sharing a degree bound rhigh, advanced together.
+// block_m == 1: per degree one squaring of F and one product ACC *= F + x
+// (the reference's trinomial_scan loop, src/trinomial_scan.cu:345-380);
+// block_m > 1: the blocked scheme of ddf.hip (tsfactor's h-table,
+// src/tsfactor.cu:1242-1341), one product per m degrees.  Intervals end
+// with a host gcd(ACC, T_s) per residue.
 int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                std::vector<Live> live, std::vector<gf2t_scan_result> &rows,
                std::vector<std::string> &hexes) {
     const size_t nw = (size_t)((r + 63) >> 6);
     const size_t pitch = (nw + 31) & ~(size_t)31;
     const int B = (int)live.size();
-    DevMem F0, F1, ACC, SNAP, SV, WORK;
-    SCAN_CHECK(F0.alloc(pitch * B * 8));
-    SCAN_CHECK(F1.alloc(pitch * B * 8));
+    const int m = block_for(P, r);
+    const size_t tab = (m > 1 ? (size_t)m : 1) * pitch;   // words per residue in F / h-table
+    DevMem T0, T1, ACC, C, SV, WORK;
+    SCAN_CHECK(T0.alloc(tab * B * 8));
+    SCAN_CHECK(T1.alloc(tab * B * 8));
     SCAN_CHECK(ACC.alloc(pitch * B * 8));
-    SCAN_CHECK(SNAP.alloc(pitch * B * 8));
+    if (m > 1) SCAN_CHECK(C.alloc(pitch * B * 8));
     SCAN_CHECK(SV.alloc(B * 8));
     SCAN_CHECK(WORK.alloc(gf2t_mulmod_slab_workspace_bytes(r, B)));
-    std::vector<u64> host((size_t)B * pitch, 0), one((size_t)B * pitch, 0);
-    for (int i = 0; i < B; i++) {
-        host[(size_t)i * pitch] = 2;   // F = x
-        one[(size_t)i * pitch] = 1;    // ACC = 1
-    }
-    SCAN_CHECK(gf2t_memcpy_h2d(F0.p, host.data(), host.size() * 8, nullptr));
     std::vector<u64> svals(B);
     for (int i = 0; i < B; i++) svals[i] = live[i].s;
     SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, nullptr));
-    uint64_t *cur = F0.w(), *oth = F1.w();
-    for (u64 k = 0; k < P.k0; k++) {
-        SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), B, nullptr));
-        std::swap(cur, oth);
+    uint64_t *cur = T0.w(), *oth = T1.w();
+    if (m > 1) {
+        // h-table for the first block, degrees k0+1 .. k0+m
+        SCAN_CHECK(gf2t_ddf_init_slab(cur, oth, pitch, m, r, SV.w(), P.k0 + 1, B, nullptr));
+    } else {
+        std::vector<u64> host((size_t)B * pitch, 0);
+        for (int i = 0; i < B; i++) host[(size_t)i * pitch] = 2;   // F = x
+        SCAN_CHECK(gf2t_memcpy_h2d(cur, host.data(), host.size() * 8, nullptr));
+        for (u64 k = 0; k < P.k0; k++) {
+            SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), B, nullptr));
+            std::swap(cur, oth);
+        }
     }
     int n = B;
     u64 k = P.k0, L = P.L0 ? P.L0 : 64;
     const u64 Lmax = P.Lmax ? P.Lmax : 4096;
+    const int threads = P.gcd_threads > 0 ? P.gcd_threads : 8;
     std::vector<u64> acc_host;
     while (n > 0) {
         u64 ka = k + 1;
         if (ka > rhigh) break;
-        u64 kb = std::min(k + L, rhigh);
-        SCAN_CHECK(gf2t_memcpy_d2d(SNAP.p, cur, (size_t)n * pitch * 8, nullptr));
-        SCAN_CHECK(gf2t_memcpy_h2d(ACC.p, one.data(), (size_t)n * pitch * 8, nullptr));
-        for (u64 deg_k = ka; deg_k <= kb; deg_k++) {
-            SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), n, nullptr));
-            std::swap(cur, oth);
-            SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, cur, pitch, 1, r, SV.w(), ACC.w(), pitch,
-                                        n, WORK.p, nullptr));
+        u64 kend = std::min(k + L, rhigh);
+        u64 kb = kend;
+        SCAN_CHECK(gf2t_set_one_slab(ACC.w(), pitch, nw, n, nullptr));
+        if (m > 1) {
+            // whole blocks: the last may run past rhigh; factors found
+            // beyond it are not reported
+            u64 nblk = (kend - k + (u64)m - 1) / (u64)m;
+            kb = k + nblk * (u64)m;
+            for (u64 b = 0; b < nblk; b++) {
+                SCAN_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, n, nullptr));
+                SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(),
+                                            pitch, n, WORK.p, nullptr));
+                for (int t = 0; t < m; t++) {
+                    SCAN_CHECK(gf2t_square_fold_slab_g(cur, oth, pitch, r, SV.w(),
+                                                       (size_t)n * m, m, nullptr));
+                    std::swap(cur, oth);
+                }
+            }
+        } else {
+            for (u64 deg_k = ka; deg_k <= kb; deg_k++) {
+                SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), n, nullptr));
+                std::swap(cur, oth);
+                SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, cur, pitch, 1, r, SV.w(), ACC.w(),
+                                            pitch, n, WORK.p, nullptr));
+            }
         }
         acc_host.assign((size_t)n * pitch, 0);
         SCAN_CHECK(gf2t_memcpy_d2h(acc_host.data(), ACC.p, acc_host.size() * 8, nullptr));
         std::vector<Poly> g(n);
-        parallel_for((size_t)n, P.gcd_threads > 0 ? P.gcd_threads : 8, [&](size_t i) {
+        parallel_for((size_t)n, threads, [&](size_t i) {
             Poly a = from_words(acc_host.data() + i * pitch, nw);
             g[i] = gcd_fast(trinomial(r, live[i].s), a);
         });
-        // resolve hits (all earlier intervals of these s were clear)
+        // resolve hits: every earlier interval of these s was clear, so
+        // each irreducible factor of g has its degree in [ka, kb]
         std::vector<int> hit;
         for (int i = 0; i < n; i++)
             if (deg(g[i]) > 0) hit.push_back(i);
-        std::vector<u64> snap_host;
-        if (!hit.empty()) {
-            snap_host.assign((size_t)n * pitch, 0);
-            SCAN_CHECK(gf2t_memcpy_d2h(snap_host.data(), SNAP.p, snap_host.size() * 8, nullptr));
-        }
         std::vector<char> done(n, 0);
-        parallel_for(hit.size(), P.gcd_threads > 0 ? P.gcd_threads : 8, [&](size_t h) {
+        parallel_for(hit.size(), threads, [&](size_t h) {
             int i = hit[h];
             gf2t_scan_result &R = rows[live[i].idx];
             const Poly &gi = g[i];
@@ -175,9 +216,9 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                 mask = gi;
                 ok = true;
             } else {
-                Poly h0 = from_words(snap_host.data() + (size_t)i * pitch, nw);
-                ok = resolve_least(gi, ka, kb, h0, d, mask);
+                ok = resolve_least(gi, ka, kb, frob_mod(gi, ka - 1), d, mask);
             }
+            if (ok && d > rhigh) return;             // beyond the bound: not reported
             if (ok) {
                 R.kind = GF2T_SCAN_FOUND;
                 R.d = d;
@@ -185,7 +226,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             } else {
                 R.kind = GF2T_SCAN_RANGE;
                 R.rlo = ka;
-                R.rhi = kb;
+                R.rhi = std::min(kb, rhigh);
             }
             R.gcd_degree = gd;
             done[i] = 1;
@@ -198,23 +239,23 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                     done[i] = 1;
                 }
         // compact the survivors of this interval to the front of the slabs
-        int m = 0;
+        int nl = 0;
         for (int i = 0; i < n; i++) {
             if (done[i]) continue;
-            if (m != i) {
-                SCAN_CHECK(gf2t_memcpy_d2d(cur + (size_t)m * pitch, cur + (size_t)i * pitch,
-                                           nw * 8, nullptr));
-                live[m] = live[i];
-                svals[m] = svals[i];
+            if (nl != i) {
+                SCAN_CHECK(gf2t_memcpy_d2d(cur + (size_t)nl * tab, cur + (size_t)i * tab,
+                                           tab * 8, nullptr));
+                live[nl] = live[i];
+                svals[nl] = svals[i];
             }
-            m++;
+            nl++;
         }
-        if (m != n && m > 0)
-            SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)m * 8, nullptr));
+        if (nl != n && nl > 0)
+            SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)nl * 8, nullptr));
         if (P.verbose)
             fprintf(stderr, "  interval (%llu,%llu]: %d live, %zu hits, %d continue\n",
-                    (unsigned long long)k, (unsigned long long)kb, n, hit.size(), m);
-        n = m;
+                    (unsigned long long)k, (unsigned long long)kb, n, hit.size(), nl);
+        n = nl;
         k = kb;
         L = std::min(2 * L, Lmax);
     }
@@ -268,7 +309,9 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
         it->live.push_back(Live{i, s});
     }
     size_t nw = (size_t)((r + 63) >> 6);
-    size_t per = 4 * ((nw + 31) & ~(size_t)31) * 8 + gf2t_mulmod_slab_workspace_bytes(r, 1);
+    const size_t pitch = (nw + 31) & ~(size_t)31;
+    const size_t per = (2 * (size_t)block_for(P, r) + 2) * pitch * 8 +
+                       gf2t_mulmod_slab_workspace_bytes(r, 1);
     size_t freeb = 0, totb = 0;
     int rc = gf2t_mem_info(&freeb, &totb);
     if (rc) { delete res; return rc; }
