@@ -296,6 +296,19 @@ GF2T_HD u64 mod_mersenne(u64 x, int d, u64 M) {
     while (x > M) x = (x & M) + (x >> d);
     return x == M ? 0 : x;
 }
+// j is strictly below all its other rotations within d bits (d <= 32,
+// M = 2^d - 1): a binary Lyndon word read most significant bit first,
+// i.e. the representative of its class {j 2^i mod M} that lies in no
+// proper subfield.  For d >= 2 such j is odd and below 2^(d-1).
+GF2T_HD bool is_lyndon32(u32 j, int d, u32 M) {
+    u32 jj = j;
+    for (int i = 1; i < d; i++) {
+        jj = ((jj << 1) | (jj >> (d - 1))) & M;
+        if (jj <= j) return false;
+    }
+    return true;
+}
+
 GF2T_HD u32 gcd32(u32 a, u32 b) {
     while (b) { u32 t = a % b; a = b; b = t; }
     return a;

@@ -33,6 +33,7 @@
 #include <vector>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace {
 
@@ -166,25 +167,175 @@ __global__ void k_log_build(DegreeParams P, u32 *__restrict__ L) {
     }
 }
 
-// strict minimum of its rotations within d bits
-GF2T_DEV bool is_lyndon(u64 j, int d, u64 M) {
-    u64 jj = j;
-    for (int i = 1; i < d; i++) {
-        jj = ((jj << 1) | (jj >> (d - 1))) & M;
-        if (jj <= j) return false;
+// ---- log table by radix partition (d >= LP_MIN_D) --------------------
+// The direct build above scatters 4-byte stores over 4*2^d bytes.  For
+// large d the table is built in three coalesced passes instead:
+//   pass 1: each block generates E consecutive (g^i, i), sorts them in LDS
+//           by the top B1 bits of g^i and appends each bucket's run to
+//           that bucket's region (one cursor atomic per bucket per block);
+//   pass 2: the same by the next B2 bits, inside each level-1 bucket;
+//   pass 3: one block per sub-bucket of 2^LP_SB values places its entries
+//           into an LDS copy of that slice of L and writes it out whole.
+// Every value 1..2^d-1 occurs exactly once (g generates the group), so
+// every region's size is known in advance.  Entries are u64:
+// (g^i mod 2^(d-B1)) << 32 | i.
+constexpr int LP_MIN_D = 26;
+constexpr int LP_SB = 15;            // 2^15 values per placed slice (128 KiB)
+constexpr int LP_E = 8192;           // entries per block in passes 1 and 2
+constexpr int LP_T = 256;            // threads per block in passes 1 and 2
+
+struct LpShape {
+    int B1, B2;                      // partition bits, level 1 and 2
+};
+
+__global__ __launch_bounds__(LP_T) void k_lp_part1(DegreeParams P, LpShape S,
+                                                   unsigned long long *__restrict__ out,
+                                                   unsigned *__restrict__ cursor,
+                                                   unsigned *__restrict__ bad) {
+    __shared__ unsigned cnt[256], off[256], gbase[256];
+    __shared__ unsigned long long stage[LP_E];
+    const unsigned nb = 1u << S.B1;
+    const int shift = P.d - S.B1;
+    const u64 cap = (u64)1 << shift;
+    const u64 i0 = (u64)blockIdx.x * LP_E;
+    for (unsigned b = threadIdx.x; b < nb; b += LP_T) cnt[b] = 0;
+    __syncthreads();
+    // each thread: 32 consecutive exponents by the LFSR step a <- a*g
+    constexpr int PER = LP_E / LP_T;
+    const u64 my0 = i0 + (u64)threadIdx.x * PER;
+    unsigned rank[PER];
+    u32 av[PER];
+    const u32 top = 1u << (P.d - 1);
+    const u32 qlow = (u32)P.q;
+    u32 a = my0 < P.M ? antilog(P, my0) : 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        av[k] = a;
+        rank[k] = 0;
+        if (my0 + k < P.M) rank[k] = atomicAdd(&cnt[a >> shift], 1u);
+        a = (a << 1) ^ ((a & top) ? qlow : 0u);
     }
-    return true;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (unsigned b = 0; b < nb; b++) { off[b] = acc; acc += cnt[b]; }
+    }
+    __syncthreads();
+    for (unsigned b = threadIdx.x; b < nb; b += LP_T)
+        gbase[b] = cnt[b] ? atomicAdd(&cursor[b], cnt[b]) : 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++)
+        if (my0 + k < P.M)
+            stage[off[av[k] >> shift] + rank[k]] = ((unsigned long long)av[k] << 32) | (u32)(my0 + k);
+    __syncthreads();
+    const u64 nvalid = (i0 + LP_E <= P.M) ? LP_E : P.M - i0;
+    for (u64 k = threadIdx.x; k < nvalid; k += LP_T) {
+        unsigned long long e = stage[k];
+        u32 v = (u32)(e >> 32);
+        unsigned b = v >> shift;
+        u64 pos = (u64)gbase[b] + (k - off[b]);
+        if (pos >= cap) { atomicOr(bad, 4u); continue; }
+        out[(u64)b * cap + pos] = ((unsigned long long)(v & (u32)(cap - 1)) << 32) | (u32)e;
+    }
 }
 
+// level 2: inside level-1 bucket blockIdx.y, chunk blockIdx.x of LP_E
+// entries, by bits [LP_SB, LP_SB + B2) of the stored value
+__global__ __launch_bounds__(LP_T) void k_lp_part2(DegreeParams P, LpShape S,
+                                                   const unsigned long long *__restrict__ in,
+                                                   unsigned long long *__restrict__ out,
+                                                   unsigned *__restrict__ cursor,
+                                                   unsigned *__restrict__ bad) {
+    __shared__ unsigned cnt[512], off[512], gbase[512];
+    __shared__ unsigned long long stage[LP_E];
+    const unsigned nb2 = 1u << S.B2;
+    const u64 cap1 = (u64)1 << (P.d - S.B1);
+    const u64 cap2 = (u64)1 << LP_SB;
+    const unsigned b1 = blockIdx.y;
+    const u64 count = b1 ? cap1 : cap1 - 1;     // value 0 never occurs
+    const u64 k0 = (u64)blockIdx.x * LP_E;
+    if (k0 >= count) return;
+    const u64 nvalid = (k0 + LP_E <= count) ? LP_E : count - k0;
+    const unsigned long long *src = in + (u64)b1 * cap1 + k0;
+    for (unsigned b = threadIdx.x; b < nb2; b += LP_T) cnt[b] = 0;
+    __syncthreads();
+    constexpr int PER = LP_E / LP_T;
+    unsigned rank[PER];
+    unsigned long long ev[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        u64 idx = (u64)k * LP_T + threadIdx.x;
+        ev[k] = idx < nvalid ? src[idx] : 0;
+        rank[k] = idx < nvalid ? atomicAdd(&cnt[(unsigned)(ev[k] >> (32 + LP_SB)) & (nb2 - 1)], 1u) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (unsigned b = 0; b < nb2; b++) { off[b] = acc; acc += cnt[b]; }
+    }
+    __syncthreads();
+    for (unsigned b = threadIdx.x; b < nb2; b += LP_T)
+        gbase[b] = cnt[b] ? atomicAdd(&cursor[(u64)b1 * nb2 + b], cnt[b]) : 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        u64 idx = (u64)k * LP_T + threadIdx.x;
+        if (idx < nvalid) stage[off[(unsigned)(ev[k] >> (32 + LP_SB)) & (nb2 - 1)] + rank[k]] = ev[k];
+    }
+    __syncthreads();
+    for (u64 k = threadIdx.x; k < nvalid; k += LP_T) {
+        unsigned long long e = stage[k];
+        unsigned b = (unsigned)(e >> (32 + LP_SB)) & (nb2 - 1);
+        u64 pos = (u64)gbase[b] + (k - off[b]);
+        if (pos >= cap2) { atomicOr(bad, 4u); continue; }
+        out[((u64)b1 * nb2 + b) * cap2 + pos] = e;
+    }
+}
+
+// one block per slice z of 2^LP_SB values: L[z*2^LP_SB + v] for every v
+__global__ __launch_bounds__(1024) void k_lp_place(const unsigned long long *__restrict__ in,
+                                                   u32 *__restrict__ L) {
+    extern __shared__ u32 tile[];
+    const u64 cap = (u64)1 << LP_SB;
+    const unsigned z = blockIdx.x;
+    const u64 count = z ? cap : cap - 1;
+    const unsigned long long *src = in + (u64)z * cap;
+    if (z == 0 && threadIdx.x == 0) tile[0] = 0;
+    for (u64 k = threadIdx.x; k < count; k += blockDim.x) {
+        unsigned long long e = src[k];
+        tile[(u32)(e >> 32) & (u32)(cap - 1)] = (u32)e;
+    }
+    __syncthreads();
+    u32 *dst = L + (u64)z * cap;
+    for (u64 k = threadIdx.x; k < cap; k += blockDim.x) dst[k] = tile[k];
+}
+
+__global__ void k_fill_u32(unsigned *p, u64 n) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+LpShape lp_shape(int d) {
+    LpShape S;
+    S.B1 = std::min(8, d - LP_SB);
+    S.B2 = d - LP_SB - S.B1;
+    return S;
+}
+
+// A Lyndon word of length >= 2 starts with 0 and ends with 1, so only the
+// odd j below 2^(d-1) are candidates: candidate c tests j = 2c + 1,
+// c < 2^(d-2).
 __global__ void k_lyndon(int d, u64 M, u32 *__restrict__ surv, unsigned long long *__restrict__ nsurv,
                          u64 cap) {
     const int lane = threadIdx.x & 63;
     const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 ncand = (u64)1 << (d - 2);
+    const u32 M32 = (u32)M;
     // wave-uniform loop: every lane of a wave takes part in each ballot
-    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < M;
+    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < ncand;
          wbase += stride) {
-        u64 j = wbase + lane;
-        bool keep = j >= 1 && j < M && is_lyndon(j, d, M);
+        u64 c = wbase + lane;
+        u32 j = (u32)(2 * c + 1);
+        bool keep = c < ncand && is_lyndon32(j, d, M32);
         // 64-lane ballot compaction: one atomic per wave
         unsigned long long m = __ballot(keep);
         unsigned long long base = 0;
@@ -192,7 +343,7 @@ __global__ void k_lyndon(int d, u64 M, u32 *__restrict__ surv, unsigned long lon
         base = __shfl(base, 0);
         if (keep) {
             u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
-            if (pos < cap) surv[pos] = (u32)j;
+            if (pos < cap) surv[pos] = j;
         }
     }
 }
@@ -316,9 +467,23 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
         gf2t::set_error("gf2t_sieve: need 2 <= dmin <= dmax <= 32, smax >= 1, dbest");
         return GF2T_EINVAL;
     }
-    DevBuf bL, bSurv, bQueue, bHeavy, bTab, bCnt;
+    DevBuf bL, bSurv, bQueue, bHeavy, bTab, bCnt, bP1, bP2, bCur;
     int rc;
     if ((rc = bL.ensure(((size_t)1 << dmax) * sizeof(u32)))) return rc;
+    if (dmax >= LP_MIN_D) {
+        LpShape S = lp_shape(dmax);
+        if ((rc = bP1.ensure(((size_t)1 << dmax) * 8))) return rc;
+        if (S.B2 > 0 && (rc = bP2.ensure(((size_t)1 << dmax) * 8))) return rc;
+        if ((rc = bCur.ensure(((size_t)1 << (S.B1 + S.B2)) * 4 + 256 * 4))) return rc;
+        static std::once_flag lds_once;
+        static hipError_t lds_err = hipSuccess;
+        std::call_once(lds_once, [] {
+            lds_err = hipFuncSetAttribute((const void *)k_lp_place,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)(((size_t)1 << LP_SB) * sizeof(u32)));
+        });
+        GF2T_HIP(lds_err);
+    }
     if ((rc = bTab.ensure(2 * 65536 * sizeof(u32)))) return rc;
     if ((rc = bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
     if ((rc = bCnt.ensure(64))) return rc;
@@ -375,13 +540,41 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
         Heavy *heavy = (Heavy *)bHeavy.p;
 
         GF2T_HIP(hipEventRecord(ev[0], 0));
-        u64 nchunks = (M + LOG_CHUNK - 1) / LOG_CHUNK;
-        hipLaunchKernelGGL(k_log_build, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, 0,
-                           P, L);
-        GF2T_CHECK_LAUNCH("k_log_build");
+        if (d >= LP_MIN_D) {
+            const LpShape S = lp_shape(d);
+            const unsigned nb1 = 1u << S.B1, nb2 = 1u << S.B2;
+            unsigned *cur1 = (unsigned *)bCur.p, *cur2 = cur1 + 256;
+            const u64 ncur = 256 + (u64)nb1 * nb2;
+            hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((ncur + 255) / 256)), dim3(256), 0, 0,
+                               cur1, ncur);
+            hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
+            unsigned long long *p1 = (unsigned long long *)bP1.p;
+            unsigned long long *p2 = S.B2 ? (unsigned long long *)bP2.p : p1;
+            hipLaunchKernelGGL(k_lp_part1, dim3((unsigned)((M + LP_E - 1) / LP_E)), dim3(LP_T), 0,
+                               0, P, S, p1, cur1, cnt32 + 1);
+            if (S.B2)
+                hipLaunchKernelGGL(k_lp_part2,
+                                   dim3((unsigned)((((u64)1 << (d - S.B1)) + LP_E - 1) / LP_E), nb1),
+                                   dim3(LP_T), 0, 0, P, S, p1, p2, cur2, cnt32 + 1);
+            hipLaunchKernelGGL(k_lp_place, dim3(nb1 * nb2), dim3(1024),
+                               ((size_t)1 << LP_SB) * sizeof(u32), 0, p2, L);
+            GF2T_CHECK_LAUNCH("log table partition");
+            unsigned chk[2];
+            GF2T_HIP(hipMemcpy(chk, cnt32, 8, hipMemcpyDeviceToHost));
+            if (chk[1]) {
+                gf2t::set_error("gf2t_sieve: d=%d: log table partition overflow", d);
+                status = GF2T_EINTERNAL;
+                break;
+            }
+        } else {
+            u64 nchunks = (M + LOG_CHUNK - 1) / LOG_CHUNK;
+            hipLaunchKernelGGL(k_log_build, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0,
+                               0, P, L);
+            GF2T_CHECK_LAUNCH("k_log_build");
+        }
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
         GF2T_HIP(hipEventRecord(ev[1], 0));
-        u64 lyn_blocks = std::min<u64>((M + 255) / 256, 1u << 20);
+        u64 lyn_blocks = std::min<u64>((((u64)1 << (d - 2)) + 255) / 256, 1u << 20);
         hipLaunchKernelGGL(k_lyndon, dim3((unsigned)lyn_blocks), dim3(256), 0, 0, d, M, surv,
                            cnt64, (u64)(ncls + 64));
         GF2T_CHECK_LAUNCH("k_lyndon");

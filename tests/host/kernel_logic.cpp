@@ -38,4 +38,18 @@ u32 kl_gcd32(u32 a, u32 b) { return gcd32(a, b); }
 u32 kl_inv32(u32 a, u32 m) { return inv32(a, m); }
 u64 kl_mod_mersenne(u64 x, int d) { return mod_mersenne(x, d, ((u64)1 << d) - 1); }
 u64 kl_minpoly_bm(u32 beta, u64 q, int d, u32 tmask) { return minpoly_bm(beta, q, d, tmask); }
+// the sieve's class enumeration: candidates j = 2c + 1, c < 2^(d-2)
+u64 kl_count_classes(int d) {
+    u32 M = (u32)(((u64)1 << d) - 1);
+    u64 n = 0;
+    for (u64 c = 0; c < ((u64)1 << (d - 2)); c++) n += is_lyndon32((u32)(2 * c + 1), d, M);
+    return n;
+}
+// the same over every j in [1, M) (no candidate restriction)
+u64 kl_count_classes_all(int d) {
+    u32 M = (u32)(((u64)1 << d) - 1);
+    u64 n = 0;
+    for (u64 j = 1; j < M; j++) n += is_lyndon32((u32)j, d, M);
+    return n;
+}
 }

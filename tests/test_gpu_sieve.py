@@ -96,3 +96,13 @@ def test_headline_full_range_d32_matches_reference_log(eng):
     # the two published known answers
     assert int(best[33]) == (5 << 48) | 0x29
     assert int(best[474]) == O.NONE   # its least factor has degree 41
+
+
+@pytest.mark.parametrize("r,depth,smax", [(132049, 26, 66024), (3021377, 27, 200000)])
+def test_partitioned_log_table_degrees_vs_oracle(eng, r, depth, smax):
+    # degrees >= 26 build the log table by radix partition; the marks must
+    # equal the oracle's direct-table sieve
+    best, st = eng.run_gpu(r, depth, smax)
+    ob, _ = O.sieve(r, depth, smax, nthreads=8)
+    assert np.array_equal(best, ob)
+    assert all(s.classes == O.count_irreducible(s.d) for s in st)

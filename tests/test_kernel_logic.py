@@ -42,6 +42,10 @@ def kl(tmp_path_factory):
     lib.kl_mod_mersenne.argtypes = [C.c_uint64, C.c_int]
     lib.kl_minpoly_bm.restype = C.c_uint64
     lib.kl_minpoly_bm.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_uint32]
+    lib.kl_count_classes.restype = C.c_uint64
+    lib.kl_count_classes.argtypes = [C.c_int]
+    lib.kl_count_classes_all.restype = C.c_uint64
+    lib.kl_count_classes_all.argtypes = [C.c_int]
     return lib
 
 
@@ -216,3 +220,12 @@ def test_square_fold_fast_with_tile_order_covers_output(kl):
         out = np.zeros(O.words(r), np.uint64)
         kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
         assert np.array_equal(out, want)
+
+
+def test_class_enumeration_counts(kl):
+    # the sieve tests only odd j below 2^(d-1); the class count must still
+    # be the number of degree-d irreducibles (Moebius), as over all j
+    for d in range(2, 25):
+        assert kl.kl_count_classes(d) == O.count_irreducible(d), d
+    for d in range(2, 19):
+        assert kl.kl_count_classes_all(d) == O.count_irreducible(d), d
