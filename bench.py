@@ -54,7 +54,14 @@ def parse():
     ap.add_argument("--sq-ppt", type=int, default=0, help="override the squaring pairs/lane")
     ap.add_argument("--sq-order", type=int, default=-1, help="override the squaring tile order")
     ap.add_argument("--sq-nt", type=int, default=-1, help="override streaming stores (0/1)")
-    ap.add_argument("--mode", choices=["full", "squaring", "fft"], default="full",
+    ap.add_argument("--scan-r", type=int, default=R, help="scan mode: trinomial degree r")
+    ap.add_argument("--scan-s", type=str, default="474",
+                    help="scan mode: comma-separated s values")
+    ap.add_argument("--scan-k0", type=int, default=32, help="scan mode: depth already sieved")
+    ap.add_argument("--scan-maxd", type=int, default=64, help="scan mode: degree bound")
+    ap.add_argument("--scan-block", type=str, default="16,1",
+                    help="scan mode: comma-separated block sizes m to run (1 = plain)")
+    ap.add_argument("--mode", choices=["full", "squaring", "fft", "scan"], default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
                          "fft: the Cantor-product batch (64 seeded operand pairs of 2^16..2^27 "
@@ -195,6 +202,24 @@ def mode_fft(args):
                       "products_per_s": 64 / (ms * 1e-3), "operand_bits": bits}), flush=True)
 
 
+def mode_scan(args):
+    """Deep scan of the given s values (one cohort) once per block size,
+    with the scan's own phase times (device, gcd, resolution) on stderr."""
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import device
+    device.init(0)
+    svals = [int(x) for x in args.scan_s.split(",") if x]
+    for bm in [int(x) for x in args.scan_block.split(",") if x]:
+        t0 = time.perf_counter()
+        out = M.scan(args.scan_r, svals, k0=args.scan_k0, maxd=args.scan_maxd, L0=64,
+                     Lmax=4096, verbose=1, block_m=bm)
+        dt = time.perf_counter() - t0
+        print(json.dumps({"mode": "scan", "r": args.scan_r, "k0": args.scan_k0,
+                          "maxd": args.scan_maxd, "block_m": bm, "n_s": len(svals),
+                          "seconds": dt, "s_per_second": len(svals) / dt,
+                          "lines": [o.line() for o in out][:16]}), flush=True)
+
+
 def apply_tuning(args):
     from mersenne_trinomial_amd._capi import check, lib
     if args.sq_ppt:
@@ -218,6 +243,8 @@ def main():
         return mode_squaring(args)
     if args.mode == "fft":
         return mode_fft(args)
+    if args.mode == "scan":
+        return mode_scan(args)
 
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib

@@ -212,6 +212,56 @@ struct TrinGPU {
     }
 };
 
+// Device product service with the member functions of the reference's
+// GpuMulHook (src/gf2_gcd.h:155-172): mul() is the full product, and
+// mat2_apply() the 2x2 matrix-vector product in one set of six forward
+// and two inverse transforms.  Both return false (decline) on a library
+// error, so a caller falls back to its own products.  Code that includes
+// src/gf2_gcd.h binds it as
+//   struct MI355XMulHook : GpuMulHook {
+//       gf2t::MulService svc;
+//       bool mul(const u64 *a, size_t aw, u64 abits, const u64 *b, size_t bw,
+//                u64 bbits, std::vector<u64> &out) override {
+//           return svc.mul(a, aw, abits, b, bw, bbits, out);
+//       }
+//       bool mat2_apply(const Op m[4], const Op &a, const Op &b,
+//                       std::vector<u64> &oa, std::vector<u64> &ob) override {
+//           gf2t::MulService::Op mm[4] = {{m[0].w, m[0].nw, m[0].bits}, ...};
+//           return svc.mat2_apply(mm, {a.w, a.nw, a.bits}, {b.w, b.nw, b.bits}, oa, ob);
+//       }
+//   };
+struct MulService {
+    struct Op {
+        const u64 *w;
+        size_t nw;
+        u64 bits;
+    };
+    bool mul(const u64 *a, size_t aw, u64 abits, const u64 *b, size_t bw, u64 bbits,
+             std::vector<u64> &out) const {
+        if (abits > 64 * (u64)aw || bbits > 64 * (u64)bw) return false;
+        out.assign(mul_out_words(abits, bbits), 0);
+        return gf2t_mul_host(a, abits, b, bbits, out.data()) == GF2T_OK;
+    }
+    bool mat2_apply(const Op m[4], const Op &a, const Op &b, std::vector<u64> &oa,
+                    std::vector<u64> &ob) const {
+        const uint64_t *hm[4];
+        uint64_t mb[4], maxm = 0;
+        for (int i = 0; i < 4; i++) {
+            if (m[i].bits > 64 * (u64)m[i].nw) return false;
+            hm[i] = m[i].w;
+            mb[i] = m[i].bits;
+            if (mb[i] > maxm) maxm = mb[i];
+        }
+        if (a.bits > 64 * (u64)a.nw || b.bits > 64 * (u64)b.nw) return false;
+        u64 maxv = a.bits > b.bits ? a.bits : b.bits;
+        size_t ow = mul_out_words(maxm, maxv);
+        oa.assign(ow, 0);
+        ob.assign(ow, 0);
+        return gf2t_mat2_host(hm, mb, a.w, a.bits, b.w, b.bits, oa.data(), ob.data(), ow) ==
+               GF2T_OK;
+    }
+};
+
 // Coarse sieve of all s in [1, smax] over degrees min_depth..depth into a
 // host best[] (run_gpu, src/coarse_sieve.cu:1119-1225); best seeds the
 // minimum when it already holds smax+1 entries.

@@ -144,6 +144,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     for (int i = 0; i < B; i++) svals[i] = live[i].s;
     SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, nullptr));
     uint64_t *cur = T0.w(), *oth = T1.w();
+    double t_dev = 0, t_gcd = 0, t_res = 0, t0 = now_s();
     if (m > 1) {
         // h-table for the first block, degrees k0+1 .. k0+m
         SCAN_CHECK(gf2t_ddf_init_slab(cur, oth, pitch, m, r, SV.w(), P.k0 + 1, B, nullptr));
@@ -192,6 +193,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         }
         acc_host.assign((size_t)n * pitch, 0);
         SCAN_CHECK(gf2t_memcpy_d2h(acc_host.data(), ACC.p, acc_host.size() * 8, nullptr));
+        double t1 = now_s();
+        t_dev += t1 - t0;
         std::vector<Poly> g(n);
         parallel_for((size_t)n, threads, [&](size_t i) {
             Poly a = from_words(acc_host.data() + i * pitch, nw);
@@ -202,6 +205,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         std::vector<int> hit;
         for (int i = 0; i < n; i++)
             if (deg(g[i]) > 0) hit.push_back(i);
+        double t2 = now_s();
+        t_gcd += t2 - t1;
         std::vector<char> done(n, 0);
         parallel_for(hit.size(), threads, [&](size_t h) {
             int i = hit[h];
@@ -231,6 +236,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             R.gcd_degree = gd;
             done[i] = 1;
         });
+        t0 = now_s();
+        t_res += t0 - t2;
         if (kb >= rhigh)
             for (int i = 0; i < n; i++)
                 if (!done[i]) {
@@ -253,8 +260,10 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         if (nl != n && nl > 0)
             SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)nl * 8, nullptr));
         if (P.verbose)
-            fprintf(stderr, "  interval (%llu,%llu]: %d live, %zu hits, %d continue\n",
-                    (unsigned long long)k, (unsigned long long)kb, n, hit.size(), nl);
+            fprintf(stderr, "  interval (%llu,%llu]: %d live, %zu hits, %d continue"
+                    " (m=%d; device %.3fs, gcd %.3fs, resolve %.3fs so far)\n",
+                    (unsigned long long)k, (unsigned long long)kb, n, hit.size(), nl, m, t_dev,
+                    t_gcd, t_res);
         n = nl;
         k = kb;
         L = std::min(2 * L, Lmax);
