@@ -61,7 +61,10 @@ def parse():
     ap.add_argument("--scan-maxd", type=int, default=64, help="scan mode: degree bound")
     ap.add_argument("--scan-block", type=str, default="16,1",
                     help="scan mode: comma-separated block sizes m to run (1 = plain)")
-    ap.add_argument("--mode", choices=["full", "squaring", "fft", "scan"], default="full",
+    ap.add_argument("--out", type=str, default="",
+                    help="pipeline mode: write the certificate lines to this file")
+    ap.add_argument("--mode", choices=["full", "squaring", "fft", "scan", "pipeline"],
+                    default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
                          "fft: the Cantor-product batch (64 seeded operand pairs of 2^16..2^27 "
@@ -220,6 +223,41 @@ def mode_scan(args):
                           "lines": [o.line() for o in out][:16]}), flush=True)
 
 
+def mode_pipeline(args):
+    """The whole search for one r: the coarse sieve over every s <= (r-1)/2
+    to --depth, then the deep scan of its survivors (k0 = depth) to the
+    Swan bound.  With r = 132049 and depth 20 this is the plain-method
+    configuration (66,024 certificate lines).  Prints one JSON line."""
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import certs, device
+    device.init(0)
+    r = args.scan_r
+    smax = (r - 1) // 2
+    t0 = time.perf_counter()
+    best, _ = M.run_gpu(r, args.depth, smax)
+    t1 = time.perf_counter()
+    lines = {int(ln.split()[0]): ln for ln in certs.sieve_lines(best)}
+    surv = [int(x) for x in certs.survivors(best)]
+    out = M.scan(r, surv, k0=args.depth, maxd=0, block_m=0, gcd_threads=16, verbose=1)
+    t2 = time.perf_counter()
+    kinds = {}
+    for o in out:
+        lines[o.s] = o.line()
+        kinds[o.kind] = kinds.get(o.kind, 0) + 1
+    text = "".join(lines[s] + "\n" for s in sorted(lines))
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    import hashlib
+    print(json.dumps({"mode": "pipeline", "r": r, "depth": args.depth, "s_total": smax,
+                      "lines": len(lines), "sieve_certified": len(lines) - len(surv),
+                      "survivors": len(surv), "scan_found": kinds.get(1, 0),
+                      "scan_u": kinds.get(2, 0), "scan_primitive": kinds.get(3, 0),
+                      "scan_range": kinds.get(4, 0), "sieve_s": t1 - t0, "scan_s": t2 - t1,
+                      "seconds": t2 - t0, "certified_per_s": len(lines) / (t2 - t0),
+                      "md5": hashlib.md5(text.encode()).hexdigest()}), flush=True)
+
+
 def apply_tuning(args):
     from mersenne_trinomial_amd._capi import check, lib
     if args.sq_ppt:
@@ -245,6 +283,8 @@ def main():
         return mode_fft(args)
     if args.mode == "scan":
         return mode_scan(args)
+    if args.mode == "pipeline":
+        return mode_pipeline(args)
 
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib

@@ -258,17 +258,34 @@ Poly Modulus::reduce(const Poly &a0) const {
 Poly Modulus::sqr(const Poly &a) const { return reduce(gf2t_host::sqr(a)); }
 Poly Modulus::mul(const Poly &a, const Poly &b) const { return reduce(mul_fast(a, b)); }
 
+// The first k in [ka, kb] with gcd(x^(2^k) + x, g) != 1 is the least
+// factor degree.  Blocks of 32 degrees are tested with one gcd of the
+// product of their (x^(2^k) + x) mod g; only a block that hits is
+// rescanned one degree at a time.
 bool resolve_least(const Poly &g, u64 ka, u64 kb, const Poly &h0, u64 &d, Poly &mask) {
     Modulus G(g);
     Poly h = G.reduce(h0);
-    for (u64 k = ka; k <= kb; k++) {
-        h = G.sqr(h);
-        Poly t = gcd_fast(x_plus(h), g);
-        if (deg(t) > 0) {
-            d = k;
-            mask = ((u64)deg(t) == k) ? t : edf_least(t, k);
-            return !mask.empty();
+    const u64 B = 32;
+    for (u64 k0 = ka; k0 <= kb; k0 += B) {
+        const u64 k1 = std::min(kb, k0 + B - 1);
+        const Poly start = h;
+        Poly P{1};
+        for (u64 k = k0; k <= k1; k++) {
+            h = G.sqr(h);
+            P = G.mul(P, x_plus(h));
         }
+        if (deg(gcd_fast(P, g)) <= 0) continue;
+        h = start;
+        for (u64 k = k0; k <= k1; k++) {
+            h = G.sqr(h);
+            Poly t = gcd_fast(x_plus(h), g);
+            if (deg(t) > 0) {
+                d = k;
+                mask = ((u64)deg(t) == k) ? t : edf_least(t, k);
+                return !mask.empty();
+            }
+        }
+        return false;   // the block's product hit but no single degree did
     }
     return false;
 }

@@ -1,0 +1,64 @@
+"""The plain-method configuration end to end on the device: x^132049 +
+x^s + 1 for every s in 1..66024, coarse sieve to degree 20, then the deep
+scan of the 5,791 survivors to the Swan bound (r/3, r/2 for s = 2).
+66,024 certificate lines, checked against the reference's sieve digest,
+the oracle's verifier (divisibility and irreducibility), and the oracle's
+per-degree scan on a sample."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+R = 132049
+
+
+@pytest.fixture(scope="module")
+def c1():
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import certs, device
+    device.init(0)
+    best, _ = M.run_gpu(R, 20, 66024)
+    lines = {int(ln.split()[0]): ln for ln in certs.sieve_lines(best)}
+    surv = [int(x) for x in certs.survivors(best)]
+    for o in M.scan(R, surv, k0=20, maxd=0, gcd_threads=16):
+        lines[o.s] = o.line()
+    return best, surv, lines
+
+
+def test_every_s_has_one_line(c1):
+    best, surv, lines = c1
+    assert sorted(lines) == list(range(1, 66025))
+    assert len(surv) == 5791
+
+
+def test_sieve_part_is_the_reference_digest(c1):
+    best, surv, lines = c1
+    text = "".join(lines[s] + "\n" for s in sorted(lines) if best[s] != O.NONE)
+    assert hashlib.md5(text.encode()).hexdigest() == "737739bafe9d565781900c8f45828440"
+
+
+def test_scan_factors_verified(c1):
+    best, surv, lines = c1
+    found = [lines[s] for s in surv if len(lines[s].split()) == 3]
+    assert len(found) >= 5700
+    for ln in found:
+        s, d, hx = ln.split()
+        if int(d) <= 3000:
+            f = O.poly_from_hex(hx[1:])
+            assert O.deg(f) == int(d)
+            assert O.verify_factor(R, int(s), f), ln
+    others = [lines[s] for s in surv if len(lines[s].split()) != 3]
+    assert all(ln.endswith(" primitive") for ln in others)
+
+
+def test_least_degree_sample_vs_oracle_scan(c1):
+    best, surv, lines = c1
+    rng = np.random.default_rng(5)
+    small = [s for s in surv if len(lines[s].split()) == 3 and int(lines[s].split()[1]) <= 40]
+    for s in rng.choice(small, 6, replace=False):
+        d = int(lines[int(s)].split()[1])
+        kind, d2, hx, _ = O.scan_one(R, int(s), 20, d)
+        assert (kind, d2) == (1, d) and lines[int(s)] == f"{s} {d} p{hx}"
