@@ -20,14 +20,48 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <future>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 namespace gf2t_host {
 
 namespace {
 
 std::atomic<u64> g_dev_bits{0};   // 0: host products only
+
+// Helper threads for independent products inside one gcd: a process-wide
+// budget of (hardware threads - 1), taken without blocking; a product
+// that finds the budget empty runs on the calling thread.
+std::atomic<int> g_helpers{-1};
+bool take_helper() {
+    int cur = g_helpers.load();
+    if (cur < 0) {
+        int n = (int)std::thread::hardware_concurrency() - 1;
+        g_helpers.compare_exchange_strong(cur, n > 0 ? n : 0);
+        cur = g_helpers.load();
+    }
+    while (cur > 0)
+        if (g_helpers.compare_exchange_weak(cur, cur - 1)) return true;
+    return false;
+}
+void give_helper() { g_helpers.fetch_add(1); }
+
+// run f on a helper thread when one is free, else now
+template <class F>
+std::future<void> spawn(F &&f) {
+    if (take_helper())
+        return std::async(std::launch::async, [f = std::forward<F>(f)]() mutable {
+            struct Give { ~Give() { give_helper(); } } g;
+            f();
+        });
+    f();
+    std::promise<void> p;
+    p.set_value();
+    return p.get_future();
+}
+constexpr size_t PAR_WORDS = 512;   // products with both operands this large may go parallel
 
 constexpr size_t KBASE = 24;           // words: schoolbook below this
 constexpr int64_t HGCD_BASE = 1536;    // bits: step-by-step half-gcd below this
@@ -67,9 +101,20 @@ void kara(const u64 *a, const u64 *b, size_t n, u64 *out, u64 *ws) {
         sa[i] = a[i] ^ (i < l ? a[h + i] : 0);
         sb[i] = b[i] ^ (i < l ? b[h + i] : 0);
     }
-    kara(a, b, h, out, next);                 // z0 -> out[0, 2h)
-    kara(a + h, b + h, l, out + 2 * h, next); // z2 -> out[2h, 2n)
-    kara(sa, sb, h, z1, next);                // z1 (2h words)
+    if (n >= 2 * PAR_WORDS) {
+        // the three half products are independent; each helper gets its
+        // own workspace
+        std::vector<u64> ws0(8 * h + 64), ws2(8 * h + 64);
+        auto f0 = spawn([&] { kara(a, b, h, out, ws0.data()); });
+        auto f2 = spawn([&] { kara(a + h, b + h, l, out + 2 * h, ws2.data()); });
+        kara(sa, sb, h, z1, next);
+        f0.get();
+        f2.get();
+    } else {
+        kara(a, b, h, out, next);                 // z0 -> out[0, 2h)
+        kara(a + h, b + h, l, out + 2 * h, next); // z2 -> out[2h, 2n)
+        kara(sa, sb, h, z1, next);                // z1 (2h words)
+    }
     for (size_t i = 0; i < 2 * h; i++) z1[i] ^= out[i];
     for (size_t i = 0; i < 2 * l; i++) z1[i] ^= out[2 * h + i];
     for (size_t i = 0; i < 2 * h; i++) out[h + i] ^= z1[i];
@@ -182,26 +227,135 @@ void apply(const HMat &M, const Poly &x, const Poly &y, Poly &ox, Poly &oy) {
         trim(oy);
         return;
     }
+    if (std::min<int64_t>(dm, dv) >= 64 * (int64_t)PAR_WORDS) {
+        Poly p0, p1, p2, p3;
+        auto f1 = spawn([&] { p1 = mul_fast(M.m01, y); });
+        auto f2 = spawn([&] { p2 = mul_fast(M.m10, x); });
+        auto f3 = spawn([&] { p3 = mul_fast(M.m11, y); });
+        p0 = mul_fast(M.m00, x);
+        f1.get();
+        f2.get();
+        f3.get();
+        ox = add(p0, p1);
+        oy = add(p2, p3);
+        return;
+    }
     ox = add(mul_fast(M.m00, x), mul_fast(M.m01, y));
     oy = add(mul_fast(M.m10, x), mul_fast(M.m11, y));
 }
 
-// step-by-step half-gcd for small inputs
+// ---- word-level base case --------------------------------------------
+inline int deg64(u64 v) { return v ? 63 - __builtin_clzll(v) : -1; }
+
+inline u64 clmul_lo(u64 a, u64 b) {
+    u64 lo, hi;
+    clmul(a, b, lo, hi);
+    return lo;
+}
+
+// bits [p, p + 64) of a (zero beyond its words)
+inline u64 window64(const Poly &a, u64 p) {
+    size_t w = (size_t)(p >> 6);
+    unsigned off = (unsigned)(p & 63);
+    u64 lo = w < a.size() ? a[w] >> off : 0;
+    u64 hi = (off && w + 1 < a.size()) ? a[w + 1] << (64 - off) : 0;
+    return lo | hi;
+}
+
+struct W2 {
+    u64 m00, m01, m10, m11;
+};
+
+// Remainder steps of single words (deg c <= 63, deg c > deg d) while
+// deg d >= stop; returns the step matrix (entries stay below 64 bits
+// because each has degree deg(c0) - deg(current c) <= 63 - stop).
+W2 hgcd_word(u64 c, u64 d, int stop, int &steps) {
+    W2 W{1, 0, 0, 1};
+    steps = 0;
+    while (d && deg64(d) >= stop) {
+        const int dd = deg64(d);
+        u64 q = 0, r = c;
+        for (int dr = deg64(r); dr >= dd; dr = deg64(r)) {
+            q |= 1ULL << (dr - dd);
+            r ^= d << (dr - dd);
+        }
+        u64 n10 = W.m00 ^ clmul_lo(q, W.m10), n11 = W.m01 ^ clmul_lo(q, W.m11);
+        W.m00 = W.m10;
+        W.m01 = W.m11;
+        W.m10 = n10;
+        W.m11 = n11;
+        c = d;
+        d = r;
+        steps++;
+    }
+    return W;
+}
+
+// out = w0 x + w1 y (word polynomials times multiword ones)
+void lin2(u64 w0, const Poly &x, u64 w1, const Poly &y, Poly &out) {
+    size_t n = std::max(x.size(), y.size()) + 1;
+    out.assign(n, 0);
+    for (size_t i = 0; i < x.size(); i++) {
+        u64 lo, hi;
+        clmul(w0, x[i], lo, hi);
+        out[i] ^= lo;
+        out[i + 1] ^= hi;
+    }
+    for (size_t i = 0; i < y.size(); i++) {
+        u64 lo, hi;
+        clmul(w1, y[i], lo, hi);
+        out[i] ^= lo;
+        out[i + 1] ^= hi;
+    }
+    trim(out);
+}
+
+// half-gcd for small inputs (deg a < HGCD_BASE): rounds of single-word
+// half-gcds on the top 64 coefficients.  A window's quotients are those
+// of the full pair while the window remainders keep at least half the
+// window's degree (the same rule as the recursion), so each round applies
+// a correct prefix of the remainder sequence, about 32 degrees long.  A
+// round that cannot step (a quotient longer than the window) takes one
+// explicit division.
 void hgcd_base(const Poly &a, const Poly &b, int64_t m, HMat &M, Poly &c, Poly &d) {
     M = HMat{Poly{1}, Poly(), Poly(), Poly{1}};
     c = a;
     d = b;
-    Poly q, r;
+    trim(c);
+    trim(d);
+    Poly t0, t1, t2, t3, q, r;
     while (deg(d) >= m) {
-        divmod(c, d, q, r);
-        Poly n10 = add(M.m00, mul_fast(q, M.m10));
-        Poly n11 = add(M.m01, mul_fast(q, M.m11));
-        M.m00.swap(M.m10);
-        M.m01.swap(M.m11);
-        M.m10.swap(n10);
-        M.m11.swap(n11);
-        c.swap(d);
-        d.swap(r);
+        const int64_t dc = deg(c);
+        const int64_t sh = std::max<int64_t>(0, dc - 63);
+        const u64 cw = window64(c, (u64)sh), dw = window64(d, (u64)sh);
+        const int stop = sh == 0 ? (int)m
+                                 : (int)std::max<int64_t>((dc - sh + 1) / 2, m - sh);
+        int steps = 0;
+        W2 W = stop <= 63 ? hgcd_word(cw, dw, std::max(stop, 0), steps) : W2{1, 0, 0, 1};
+        if (steps == 0) {
+            divmod(c, d, q, r);
+            Poly n10 = add(M.m00, mul_fast(q, M.m10));
+            Poly n11 = add(M.m01, mul_fast(q, M.m11));
+            M.m00.swap(M.m10);
+            M.m01.swap(M.m11);
+            M.m10.swap(n10);
+            M.m11.swap(n11);
+            c.swap(d);
+            d.swap(r);
+            continue;
+        }
+        lin2(W.m00, c, W.m01, d, t0);
+        lin2(W.m10, c, W.m11, d, t1);
+        c.swap(t0);
+        d.swap(t1);
+        lin2(W.m00, M.m00, W.m01, M.m10, t0);
+        lin2(W.m00, M.m01, W.m01, M.m11, t1);
+        lin2(W.m10, M.m00, W.m11, M.m10, t2);
+        lin2(W.m10, M.m01, W.m11, M.m11, t3);
+        M.m00.swap(t0);
+        M.m01.swap(t1);
+        M.m10.swap(t2);
+        M.m11.swap(t3);
     }
 }
 
