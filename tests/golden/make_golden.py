@@ -107,3 +107,9 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref is not built (the reference sources are needed)")
     sieve_fixtures()
     cantor_fixtures()
+
+
+# The C2 digest in known_answers.json ("sieve_r3021377_d32_s1510688") comes
+# from a run too long for this script (136.7 min on one core):
+#     oracle/_ref/sieve_ref run 3021377 32 1510688 > c2.cert
+# then md5sum c2.cert, wc -l c2.cert and the count of lines per degree.

@@ -72,3 +72,18 @@ def test_full_range_heavy_counts_match_reference_log(eng):
     for d, n in k["classes"].items():
         if int(d) <= 32:
             assert classes[int(d)] == n, d
+
+
+def test_c2_full_depth_matches_reference_digest(eng):
+    # r=3021377, every s <= 1510688, degrees 2..32: the certificate text
+    # must equal the reference's own sieve_ref output byte for byte
+    g = known()["sieve_r3021377_d32_s1510688"]
+    best, _ = eng.run_gpu(3021377, 32, 1510688)
+    lines = cert(best)
+    assert len(lines) == g["lines"]
+    per = {}
+    for ln in lines:
+        dd = ln.split()[1]
+        per[dd] = per.get(dd, 0) + 1
+    assert per == g["lines_per_degree"]
+    assert hashlib.md5(("\n".join(lines) + "\n").encode()).hexdigest() == g["md5"]
