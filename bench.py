@@ -63,7 +63,14 @@ def parse():
                     help="scan mode: comma-separated block sizes m to run (1 = plain)")
     ap.add_argument("--out", type=str, default="",
                     help="pipeline mode: write the certificate lines to this file")
-    ap.add_argument("--mode", choices=["full", "squaring", "fft", "scan", "pipeline"],
+    ap.add_argument("--serve-jobs", type=int, default=10, help="serve mode: number of jobs")
+    ap.add_argument("--serve-rate", type=float, default=0.5,
+                    help="serve mode: Poisson arrival rate, jobs per second")
+    ap.add_argument("--serve-max-len", type=int, default=5000,
+                    help="serve mode: longest s slice (the mix draws 10^3..10^6, capped here)")
+    ap.add_argument("--serve-interval", type=int, default=2048,
+                    help="serve mode: deep-scan interval length (gcds per s = 2048 / this)")
+    ap.add_argument("--mode", choices=["full", "squaring", "fft", "scan", "pipeline", "serve"],
                     default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
@@ -258,6 +265,87 @@ def mode_pipeline(args):
                       "md5": hashlib.md5(text.encode()).hexdigest()}), flush=True)
 
 
+def serve_jobs(args):
+    """The serving mix: r from {132049, 756839, 3021377, 136279841}, a seeded
+    contiguous s slice of log-uniform length 10^3..10^6 (capped at
+    --serve-max-len, and at 256 for r = 136279841), sieve bound 32, deep
+    bound 2048, Poisson arrivals at --serve-rate."""
+    from mersenne_trinomial_amd.serve import Job
+    rng = np.random.default_rng(args.seed)
+    t = 0.0
+    jobs = []
+    for i in range(args.serve_jobs):
+        r = int(rng.choice([132049, 756839, 3021377, R]))
+        hi_len = min(10 ** 6, args.serve_max_len, 256 if r == R else 10 ** 6)
+        n = int(round(np.exp(rng.uniform(np.log(min(1000, hi_len)), np.log(hi_len)))))
+        lo = int(rng.integers(1, (r - 1) // 2 - n + 2))
+        t += float(rng.exponential(1.0 / args.serve_rate))
+        jobs.append(Job(job_id=i, r=r, s_lo=lo, s_hi=lo + n - 1, depth=32, maxd=2048, arrival=t))
+    return jobs
+
+
+def mode_serve(args):
+    """Poisson stream of mixed jobs through the request loop: p50/p99 job
+    latency and s-values/sec; two jobs are re-run alone afterwards and
+    their certificates compared with their lines in the mix."""
+    import copy
+    from mersenne_trinomial_amd.serve import RequestLoop
+    jobs = serve_jobs(args)
+    pristine = copy.deepcopy(jobs)
+    iv = args.serve_interval
+    loop = RequestLoop(block_m=0, L0=iv, Lmax=iv)
+    for sv_r in sorted({j.r for j in jobs}):
+        loop.server(sv_r)                     # slabs allocated before the clock starts
+    t0 = time.perf_counter()
+    pending = list(jobs)
+    completed = []
+    last_note = time.perf_counter()
+    while pending or loop.busy():
+        if time.perf_counter() - last_note > 10:
+            last_note = time.perf_counter()
+            state = {r: (sv.last_live, sv.last_queued) for r, sv in loop.servers.items()}
+            print(f"t={last_note - t0:.0f}s: {len(completed)} jobs done, "
+                  f"server (live, queued) by r: {state}", file=sys.stderr, flush=True)
+        now = time.perf_counter() - t0
+        while pending and pending[0].arrival <= now:
+            j = pending.pop(0)
+            j.arrival = t0 + j.arrival
+            loop.submit(j)
+        done = loop.pump(1)
+        completed.extend(done)
+        for j in done:
+            print(f"job {j.job_id} r={j.r} n={j.s_hi - j.s_lo + 1} done in "
+                  f"{j.done_at - j.arrival:.2f}s", file=sys.stderr, flush=True)
+        if not loop.busy() and pending:
+            time.sleep(max(0.0, min(0.05, pending[0].arrival - (time.perf_counter() - t0))))
+    makespan = time.perf_counter() - t0
+    for j in jobs:
+        if not j.done_at:
+            j.done_at = j.arrival
+    lat = np.array([j.done_at - j.arrival for j in jobs])
+    n_s = sum(j.s_hi - j.s_lo + 1 for j in jobs)
+    loop.close()
+    # the same jobs alone, one at a time
+    checks = []
+    print(f"all {len(jobs)} jobs done in {makespan:.1f}s; solo re-runs", file=sys.stderr,
+          flush=True)
+    for j0 in pristine[:2]:
+        solo = RequestLoop(block_m=0, L0=iv, Lmax=iv)
+        j0.arrival = time.perf_counter()
+        solo.submit(j0)
+        while solo.busy():
+            solo.pump(1)
+        checks.append(j0.certificate() == jobs[j0.job_id].certificate())
+        solo.close()
+    print(json.dumps({"mode": "serve", "jobs": len(jobs), "rate_per_s": args.serve_rate,
+                      "s_values": n_s, "makespan_s": makespan, "s_per_second": n_s / makespan,
+                      "latency_p50_s": float(np.percentile(lat, 50)),
+                      "latency_p99_s": float(np.percentile(lat, 99)),
+                      "mix": [[j.r, j.s_hi - j.s_lo + 1, round(j.done_at - j.arrival, 3)]
+                              for j in jobs],
+                      "solo_equals_mix": checks}), flush=True)
+
+
 def apply_tuning(args):
     from mersenne_trinomial_amd._capi import check, lib
     if args.sq_ppt:
@@ -285,6 +373,8 @@ def main():
         return mode_scan(args)
     if args.mode == "pipeline":
         return mode_pipeline(args)
+    if args.mode == "serve":
+        return mode_serve(args)
 
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib

@@ -314,6 +314,36 @@ int gf2t_scan_line(const gf2t_scan_results *res, size_t i, char *buf,
                    size_t cap);
 void gf2t_scan_results_free(gf2t_scan_results *res);
 
+/* ------------------------------------------------------------------ *
+ * Deep-scan server with continuous batching.  No reference counterpart
+ * (the reference scans one s at a time); every s gets the line
+ * gf2t_scan_run would give it, alone or in any mix.
+ * One server per r.  Its slab holds a fixed number of slots, sized from
+ * mem_bytes (0 = half of the free device memory): admission is by device
+ * footprint.  Slot s values of any job advance together, m degrees per
+ * round (the blocked scheme), each with its own degree frontier and
+ * interval ends; queued s values enter free slots at round boundaries.
+ * One caller at a time per server (calls take its lock).
+ * ------------------------------------------------------------------ */
+typedef struct gf2t_server gf2t_server;
+int gf2t_server_create(uint64_t r, const gf2t_scan_params *params,
+                       size_t mem_bytes, gf2t_server **out);
+int gf2t_server_capacity(const gf2t_server *sv, size_t *slots,
+                         size_t *bytes_per_slot);
+/* queue the s values of job `job`: sieve survivors at depth k0, scanned
+ * to min(maxd, Swan bound) (maxd 0 = the Swan bound) */
+int gf2t_server_submit(gf2t_server *sv, uint64_t job, const uint64_t *svals,
+                       size_t n, uint64_t k0, uint64_t maxd);
+/* up to `rounds` rounds (admission, m degrees, interval checks); reports
+ * the slots in use and the s values still queued */
+int gf2t_server_step(gf2t_server *sv, int rounds, size_t *live,
+                     size_t *queued);
+/* finished lines since the last poll: up to max_rows rows, "\n"-separated
+ * into buf (cap bytes), jobs[i] the job of row i, *nrows the count */
+int gf2t_server_poll(gf2t_server *sv, size_t max_rows, uint64_t *jobs,
+                     char *buf, size_t cap, size_t *nrows);
+void gf2t_server_destroy(gf2t_server *sv);
+
 /* Host gcd of two GF(2)[x] word arrays: degree (-1 if both zero) and up
  * to gcap words of the monic gcd.  Half-gcd recursion with Karatsuba
  * products, and device FFT products above the "gcd_dev_bits" threshold

@@ -99,6 +99,12 @@ _SIGS = {
     "gf2t_scan_results_free": (None, [_vp]),
     "gf2t_gcd_host": (C.c_int64, [_p64, _sz, _p64, _sz, _p64, _sz]),
     "gf2t_mul_host": (C.c_int, [_p64, _u64, _p64, _u64, _p64]),
+    "gf2t_server_create": (C.c_int, [_u64, C.POINTER(ScanParams), _sz, C.POINTER(_vp)]),
+    "gf2t_server_capacity": (C.c_int, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
+    "gf2t_server_submit": (C.c_int, [_vp, _u64, _p64, _sz, _u64, _u64]),
+    "gf2t_server_step": (C.c_int, [_vp, C.c_int, C.POINTER(_sz), C.POINTER(_sz)]),
+    "gf2t_server_poll": (C.c_int, [_vp, _sz, _p64, C.c_char_p, _sz, C.POINTER(_sz)]),
+    "gf2t_server_destroy": (None, [_vp]),
     "gf2t_ddf_init_slab": (C.c_int, [_vp, _vp, _sz, C.c_int, _u64, _vp, _u64, C.c_int, _vp]),
     "gf2t_square_fold_slab_g": (C.c_int, [_vp, _vp, _sz, _u64, _vp, _sz, C.c_int, _vp]),
     "gf2t_horner_slab": (C.c_int, [_vp, _sz, C.c_int, _u64, _vp, _vp, _sz, C.c_int, _vp]),

@@ -20,7 +20,7 @@ BIN = os.path.join(PKG, "bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["capi.hip", "square.hip", "fft.hip", "sieve.hip", "ddf.hip"]
-HOST_SOURCES = ["host_poly.cpp", "hgcd.cpp", "scan.cpp"]
+HOST_SOURCES = ["host_poly.cpp", "hgcd.cpp", "scan.cpp", "serve.cpp"]
 # command-line drivers with the reference's argv forms and output files
 CLIS = ["coarse_sieve", "trinomial_stage2", "trinomial_scan"]
 CXX = os.environ.get("CXX", "g++")

@@ -19,6 +19,7 @@
 // compacted), so device work follows the survivors.
 #include "../../include/gf2t.h"
 #include "host_poly.hpp"
+#include "scan_common.hpp"
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -35,10 +36,8 @@
 #include <vector>
 
 using namespace gf2t_host;
+using namespace gf2t_scan;
 
-namespace gf2t {
-void set_error(const char *fmt, ...);
-}
 
 struct gf2t_scan_results {
     std::vector<gf2t_scan_result> rows;
@@ -46,50 +45,6 @@ struct gf2t_scan_results {
 };
 
 namespace {
-
-double now_s() {
-    using namespace std::chrono;
-    return duration<double>(steady_clock::now().time_since_epoch()).count();
-}
-
-// runs fn(0..n-1) on up to `threads` threads; the first exception a
-// call throws stops the remaining work and is rethrown here
-void parallel_for(size_t n, int threads, const std::function<void(size_t)> &fn) {
-    if (n == 0) return;
-    int t = std::max(1, std::min<int>(threads, (int)n));
-    std::atomic<size_t> next{0};
-    std::atomic<bool> failed{false};
-    std::exception_ptr err;
-    std::mutex err_mu;
-    std::vector<std::thread> pool;
-    for (int i = 0; i < t; i++)
-        pool.emplace_back([&] {
-            for (;;) {
-                size_t k = next.fetch_add(1);
-                if (k >= n || failed.load()) break;
-                try {
-                    fn(k);
-                } catch (...) {
-                    std::lock_guard<std::mutex> lk(err_mu);
-                    if (!err) err = std::current_exception();
-                    failed.store(true);
-                }
-            }
-        });
-    for (auto &th : pool) th.join();
-    if (err) std::rethrow_exception(err);
-}
-
-struct DevMem {
-    void *p = nullptr;
-    ~DevMem() { if (p) gf2t_free(p); }
-    int alloc(size_t bytes) {
-        if (p) gf2t_free(p);
-        p = nullptr;
-        return gf2t_malloc(&p, bytes ? bytes : 8);
-    }
-    uint64_t *w() { return (uint64_t *)p; }
-};
 
 #define SCAN_CHECK(call)                                                       \
     do {                                                                       \
@@ -101,23 +56,6 @@ struct Live {
     size_t idx;       // index into the caller's s list
     u64 s;
 };
-
-// x^(2^k) mod g on the host (g small: the interval gcd)
-Poly frob_mod(const Poly &g, u64 k) {
-    Modulus G(g);
-    Poly h = G.reduce(Poly{2});
-    for (u64 i = 0; i < k; i++) h = G.sqr(h);
-    return h;
-}
-
-// degrees per product: per degree the blocked scheme costs m slab
-// squarings plus 1/m of a slab product
-int block_for(const gf2t_scan_params &P, u64 r) {
-    int m = P.block_m > 0 ? P.block_m : (r < (1u << 14) ? 4 : 16);
-    if (m > 63) m = 63;
-    while (m > 1 && 2 * (u64)m >= r) m /= 2;
-    return m;
-}
 
 // One cohort of s values sharing a degree bound rhigh, advanced together.
 // block_m == 1: per degree one squaring of F and one product ACC *= F + x
@@ -131,7 +69,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     const size_t nw = (size_t)((r + 63) >> 6);
     const size_t pitch = (nw + 31) & ~(size_t)31;
     const int B = (int)live.size();
-    const int m = block_for(P, r);
+    const int m = block_for(P.block_m, r);
     const size_t tab = (m > 1 ? (size_t)m : 1) * pitch;   // words per residue in F / h-table
     DevMem T0, T1, ACC, C, SV, WORK;
     SCAN_CHECK(T0.alloc(tab * B * 8));
@@ -210,31 +148,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         std::vector<char> done(n, 0);
         parallel_for(hit.size(), threads, [&](size_t h) {
             int i = hit[h];
-            gf2t_scan_result &R = rows[live[i].idx];
-            const Poly &gi = g[i];
-            u64 gd = (u64)deg(gi);
-            u64 d = 0;
-            Poly mask;
-            bool ok;
-            if (gd < 2 * ka) {                       // a single irreducible factor
-                d = gd;
-                mask = gi;
-                ok = true;
-            } else {
-                ok = resolve_least(gi, ka, kb, frob_mod(gi, ka - 1), d, mask);
-            }
-            if (ok && d > rhigh) return;             // beyond the bound: not reported
-            if (ok) {
-                R.kind = GF2T_SCAN_FOUND;
-                R.d = d;
-                hexes[live[i].idx] = hex(mask);
-            } else {
-                R.kind = GF2T_SCAN_RANGE;
-                R.rlo = ka;
-                R.rhi = std::min(kb, rhigh);
-            }
-            R.gcd_degree = gd;
-            done[i] = 1;
+            if (resolve_hit(g[i], ka, kb, rhigh, rows[live[i].idx], hexes[live[i].idx]))
+                done[i] = 1;
         });
         t0 = now_s();
         t_res += t0 - t2;
@@ -319,7 +234,7 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
     }
     size_t nw = (size_t)((r + 63) >> 6);
     const size_t pitch = (nw + 31) & ~(size_t)31;
-    const size_t per = (2 * (size_t)block_for(P, r) + 2) * pitch * 8 +
+    const size_t per = (2 * (size_t)block_for(P.block_m, r) + 2) * pitch * 8 +
                        gf2t_mulmod_slab_workspace_bytes(r, 1);
     size_t freeb = 0, totb = 0;
     int rc = gf2t_mem_info(&freeb, &totb);
