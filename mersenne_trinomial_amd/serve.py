@@ -34,6 +34,7 @@ class Server:
         check(lib.gf2t_server_create(r, C.byref(P), mem_bytes, C.byref(h)), "gf2t_server_create")
         self._h = h
         self.last_live = self.last_queued = 0
+        self.fresh = False
         slots, per = C.c_size_t(), C.c_size_t()
         check(lib.gf2t_server_capacity(h, C.byref(slots), C.byref(per)), "gf2t_server_capacity")
         self.slots, self.bytes_per_slot = slots.value, per.value
@@ -42,12 +43,14 @@ class Server:
         sv = np.ascontiguousarray(np.asarray(svals, dtype=np.uint64))
         check(lib.gf2t_server_submit(self._h, job, sv.ctypes.data_as(C.POINTER(C.c_uint64)),
                                      sv.size, k0, maxd), "gf2t_server_submit")
+        self.fresh = True
 
     def step(self, rounds: int = 1) -> tuple[int, int]:
         live, queued = C.c_size_t(), C.c_size_t()
         check(lib.gf2t_server_step(self._h, rounds, C.byref(live), C.byref(queued)),
               "gf2t_server_step")
         self.last_live, self.last_queued = live.value, queued.value
+        self.fresh = False
         return live.value, queued.value
 
     def poll(self) -> list[tuple[int, str]]:
@@ -148,12 +151,21 @@ class RequestLoop:
         return bool(self.waiting) or any(j.pending for j in self.jobs.values())
 
     def pump(self, rounds: int = 1) -> list[Job]:
-        """One scheduling pass; returns the jobs that completed in it."""
+        """One scheduling pass: sieve the newly arrived jobs, then step every
+        server with work on its own host thread (servers of different r
+        share the device; each one's gcds run on its own thread pool).
+        Returns the jobs that completed in it."""
         if self.waiting:
             self._sieve_waiting()
         finished = []
+        busy = [sv for sv in self.servers.values() if sv.last_live or sv.last_queued or sv.fresh]
+        if len(busy) > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(max_workers=len(busy)) as ex:
+                list(ex.map(lambda sv: sv.step(rounds), busy))
+        elif busy:
+            busy[0].step(rounds)
         for sv in self.servers.values():
-            sv.step(rounds)
             for job_id, line in sv.poll():
                 j = self.jobs[job_id]
                 j.lines[int(line.split()[0])] = line
