@@ -16,7 +16,6 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._capi import ScanParams, check, lib
-from .certs import sieve_lines
 from .device import init
 from .engine import run_gpu
 
@@ -92,13 +91,65 @@ class Job:
         return [self.lines[s] for s in range(self.s_lo, self.s_hi + 1)]
 
 
+class _Worker:
+    """Runs one server on its own host thread: drains its submissions,
+    steps it while it has work and hands finished lines back, so servers
+    of different r progress independently (a long gcd phase of one does
+    not hold up the others)."""
+
+    def __init__(self, server: Server, rounds: int):
+        import queue
+        import threading
+        self.server = server
+        self.rounds = rounds
+        self.inbox: "queue.Queue" = queue.Queue()
+        self.outbox: "queue.Queue" = queue.Queue()
+        self.error = None
+        self._stop = threading.Event()
+        self._wake = threading.Event()
+        self._thread = threading.Thread(target=self._run, daemon=True)
+        self._thread.start()
+
+    def submit(self, job: int, svals, k0: int, maxd: int) -> None:
+        self.inbox.put((job, svals, k0, maxd))
+        self._wake.set()
+
+    def _run(self) -> None:
+        import queue
+        try:
+            while not self._stop.is_set():
+                while True:
+                    try:
+                        job, svals, k0, maxd = self.inbox.get_nowait()
+                    except queue.Empty:
+                        break
+                    self.server.submit(job, svals, k0, maxd)
+                    self.server.fresh = True
+                if self.server.last_live or self.server.last_queued or self.server.fresh:
+                    self.server.step(self.rounds)
+                    for item in self.server.poll():
+                        self.outbox.put(item)
+                else:
+                    self._wake.wait(0.01)
+                    self._wake.clear()
+        except Exception as e:  # reported to the caller by RequestLoop.pump
+            self.error = e
+
+    def close(self) -> None:
+        self._stop.set()
+        self._wake.set()
+        self._thread.join()
+        self.server.close()
+
+
 class RequestLoop:
     """Admits jobs as they arrive, sieves each new batch of one r in one
-    pass, feeds survivors to the per-r servers and steps every server with
-    work, so the work of jobs that overlap in time overlaps on the device."""
+    pass and feeds the survivors to that r's server; every server runs on
+    its own worker thread, so the work of jobs that overlap in time
+    overlaps on the device."""
 
     def __init__(self, mem_fraction: float = 0.6, block_m: int = 0, gcd_threads: int = 0,
-                 L0: int = 64, Lmax: int = 4096):
+                 L0: int = 64, Lmax: int = 4096, rounds: int = 1):
         from .device import mem_info
         init()
         self.free0 = mem_info()[0]
@@ -106,16 +157,22 @@ class RequestLoop:
         self.block_m = block_m
         self.gcd_threads = gcd_threads
         self.L0, self.Lmax = L0, Lmax
-        self.servers: dict[int, Server] = {}
+        self.rounds = rounds
+        self.workers: dict[int, _Worker] = {}
         self.jobs: dict[int, Job] = {}
         self.waiting: list[Job] = []
 
+    @property
+    def servers(self) -> dict[int, Server]:
+        return {r: w.server for r, w in self.workers.items()}
+
     def server(self, r: int) -> Server:
-        if r not in self.servers:
+        if r not in self.workers:
             budget = int(self.free0 * self.mem_fraction / 4)   # up to four r at once
-            self.servers[r] = Server(r, mem_bytes=budget, block_m=self.block_m,
-                                     gcd_threads=self.gcd_threads, L0=self.L0, Lmax=self.Lmax)
-        return self.servers[r]
+            sv = Server(r, mem_bytes=budget, block_m=self.block_m,
+                        gcd_threads=self.gcd_threads, L0=self.L0, Lmax=self.Lmax)
+            self.workers[r] = _Worker(sv, self.rounds)
+        return self.workers[r].server
 
     def submit(self, job: Job) -> None:
         self.jobs[job.job_id] = job
@@ -130,20 +187,19 @@ class RequestLoop:
             smax = max(j.s_hi for j in js)
             depth = max(j.depth for j in js)
             best, _ = run_gpu(r, depth, smax)
-            found = {}
-            for ln in sieve_lines(best[: smax + 1]):
-                s = int(ln.split()[0], 10)
-                found[s] = ln
+            self.server(r)
             for j in js:
+                seg = best[j.s_lo: j.s_hi + 1]
                 surv = []
-                for s in range(j.s_lo, j.s_hi + 1):
-                    if s in found and int(found[s].split()[1]) <= j.depth:
-                        j.lines[s] = found[s]
+                for k, s in enumerate(range(j.s_lo, j.s_hi + 1)):
+                    key = int(seg[k])
+                    if key != 0xFFFFFFFFFFFFFFFF and (key >> 48) <= j.depth:
+                        j.lines[s] = f"{s} {key >> 48} p{key & 0xFFFFFFFFFFFF:x}"
                     else:
                         surv.append(s)
                 j.pending = len(surv)
                 if surv:
-                    self.server(r).submit(j.job_id, surv, k0=j.depth, maxd=j.maxd)
+                    self.workers[r].submit(j.job_id, surv, j.depth, j.maxd)
                 else:
                     j.done_at = time.perf_counter()
 
@@ -151,31 +207,34 @@ class RequestLoop:
         return bool(self.waiting) or any(j.pending for j in self.jobs.values())
 
     def pump(self, rounds: int = 1) -> list[Job]:
-        """One scheduling pass: sieve the newly arrived jobs, then step every
-        server with work on its own host thread (servers of different r
-        share the device; each one's gcds run on its own thread pool).
-        Returns the jobs that completed in it."""
+        """Sieve the newly arrived jobs and collect finished lines; returns
+        the jobs that completed.  (`rounds` is kept for callers; each
+        worker steps on its own.)"""
+        import queue
         if self.waiting:
             self._sieve_waiting()
         finished = []
-        busy = [sv for sv in self.servers.values() if sv.last_live or sv.last_queued or sv.fresh]
-        if len(busy) > 1:
-            from concurrent.futures import ThreadPoolExecutor
-            with ThreadPoolExecutor(max_workers=len(busy)) as ex:
-                list(ex.map(lambda sv: sv.step(rounds), busy))
-        elif busy:
-            busy[0].step(rounds)
-        for sv in self.servers.values():
-            for job_id, line in sv.poll():
+        got_any = False
+        for w in self.workers.values():
+            if w.error is not None:
+                raise w.error
+            while True:
+                try:
+                    job_id, line = w.outbox.get_nowait()
+                except queue.Empty:
+                    break
+                got_any = True
                 j = self.jobs[job_id]
                 j.lines[int(line.split()[0])] = line
                 j.pending -= 1
                 if j.pending == 0:
                     j.done_at = time.perf_counter()
                     finished.append(j)
+        if not got_any and not finished:
+            time.sleep(0.002)
         return finished
 
     def close(self) -> None:
-        for sv in self.servers.values():
-            sv.close()
-        self.servers = {}
+        for w in self.workers.values():
+            w.close()
+        self.workers = {}
