@@ -27,6 +27,7 @@
 #include <vector>
 #include <map>
 #include <mutex>
+#include <condition_variable>
 #include <cstring>
 #include <algorithm>
 
@@ -596,26 +597,55 @@ __global__ void k_mat2_pointwise(u64 *__restrict__ f0, const u64 *__restrict__ f
     }
 }
 
+// A pool of contexts, each with its own non-blocking stream and device
+// buffer, so products requested by different host threads (the gcds of
+// different s values) run concurrently on the device.
+constexpr int HM_CONTEXTS = 8;
 struct HostMulCtx {
-    std::mutex mu;
     hipStream_t st = nullptr;
     u64 *dbuf = nullptr;
     size_t dcap = 0;   // words
+    bool busy = false;
+};
+struct HostMulPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    HostMulCtx ctx[HM_CONTEXTS];
 } g_hm;
 
-int hm_reserve(size_t words) {
-    if (!g_hm.st) GF2T_HIP(hipStreamCreate(&g_hm.st));
-    if (words <= g_hm.dcap) return GF2T_OK;
-    if (g_hm.dbuf) GF2T_HIP(hipFree(g_hm.dbuf));
-    g_hm.dbuf = nullptr;
-    g_hm.dcap = 0;
+// RAII: a free context for the duration of one product
+struct HmLease {
+    HostMulCtx *c = nullptr;
+    HmLease() {
+        std::unique_lock<std::mutex> lk(g_hm.mu);
+        for (;;) {
+            for (auto &x : g_hm.ctx)
+                if (!x.busy) { x.busy = true; c = &x; return; }
+            g_hm.cv.wait(lk);
+        }
+    }
+    ~HmLease() {
+        {
+            std::lock_guard<std::mutex> lk(g_hm.mu);
+            c->busy = false;
+        }
+        g_hm.cv.notify_one();
+    }
+};
+
+int hm_reserve(HostMulCtx &c, size_t words) {
+    if (!c.st) GF2T_HIP(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    if (words <= c.dcap) return GF2T_OK;
+    if (c.dbuf) GF2T_HIP(hipFree(c.dbuf));
+    c.dbuf = nullptr;
+    c.dcap = 0;
     size_t want = words + words / 4;
-    if (hipMalloc(&g_hm.dbuf, want * 8) != hipSuccess) {
+    if (hipMalloc(&c.dbuf, want * 8) != hipSuccess) {
         (void)hipGetLastError();
         gf2t::set_error("gf2t host product service: cannot allocate %zu bytes", want * 8);
         return GF2T_ENOMEM;
     }
-    g_hm.dcap = want;
+    c.dcap = want;
     return GF2T_OK;
 }
 
@@ -633,14 +663,15 @@ extern "C" int gf2t_mul_host(const uint64_t *ha, uint64_t abits, const uint64_t 
         std::memset(hout, 0, ow * 8);
         return GF2T_OK;
     }
-    std::lock_guard<std::mutex> lk(g_hm.mu);
+    HmLease lease;
+    HostMulCtx &hc = *lease.c;
     size_t aw = words_of(abits), bw = words_of(bbits);
     size_t ww = (gf2t_mul_workspace_bytes(abits, bbits) + 7) / 8;
-    int rc = hm_reserve(aw + bw + ow + ww + 64);   // + room to align the workspace
+    int rc = hm_reserve(hc, aw + bw + ow + ww + 64);   // + room to align the workspace
     if (rc) return rc;
-    u64 *da = g_hm.dbuf, *db = da + aw, *dout = db + bw, *dw = dout + ow;
+    u64 *da = hc.dbuf, *db = da + aw, *dout = db + bw, *dw = dout + ow;
     dw = (u64 *)(((uintptr_t)dw + 255) & ~(uintptr_t)255);
-    hipStream_t st = g_hm.st;
+    hipStream_t st = hc.st;
     GF2T_HIP(hipMemcpyAsync(da, ha, aw * 8, hipMemcpyHostToDevice, st));
     GF2T_HIP(hipMemcpyAsync(db, hb, bw * 8, hipMemcpyHostToDevice, st));
     rc = mul_core(da, abits, db, bbits, dout, dw, st);
@@ -674,21 +705,22 @@ extern "C" int gf2t_mat2_host(const uint64_t *const hm[4], const uint64_t mbits[
     u64 cm = (mb + 31) >> 5, cv = (vb + 31) >> 5;
     int m = fft_m_for(cm + cv - 1);
     if (m > FFT_MAX_M) { gf2t::set_error("gf2t_mat2_host: product too large"); return GF2T_ERANGE; }
-    std::lock_guard<std::mutex> lk(g_hm.mu);
     int rc = ensure_basis();
     if (rc) return rc;
+    HmLease lease;
+    HostMulCtx &hc = *lease.c;
     const gf2t_fft_plan *plan = cached_plan(m);
     size_t n = (size_t)1 << m, ow = gf2t_mul_out_words(mb, vb);
     size_t in_w[6] = {words_of(mbits[0]), words_of(mbits[1]), words_of(mbits[2]),
                       words_of(mbits[3]), words_of(abits), words_of(bbits)};
     size_t stage = 0;
     for (size_t w : in_w) stage += w + 4;
-    rc = hm_reserve(6 * n + 2 * ow + stage + 64);
+    rc = hm_reserve(hc, 6 * n + 2 * ow + stage + 64);
     if (rc) return rc;
-    hipStream_t st = g_hm.st;
+    hipStream_t st = hc.st;
     u64 *F[6];
-    for (int i = 0; i < 6; i++) F[i] = g_hm.dbuf + (size_t)i * n;
-    u64 *dout = g_hm.dbuf + 6 * n, *dst = dout + 2 * ow;
+    for (int i = 0; i < 6; i++) F[i] = hc.dbuf + (size_t)i * n;
+    u64 *dout = hc.dbuf + 6 * n, *dst = dout + 2 * ow;
     const u64 *src[6] = {hm[0], hm[1], hm[2], hm[3], ha, hb};
     const u64 bits[6] = {mbits[0], mbits[1], mbits[2], mbits[3], abits, bbits};
     for (int i = 0; i < 6; i++) {
