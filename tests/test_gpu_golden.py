@@ -87,3 +87,11 @@ def test_c2_full_depth_matches_reference_digest(eng):
         per[dd] = per.get(dd, 0) + 1
     assert per == g["lines_per_degree"]
     assert hashlib.md5(("\n".join(lines) + "\n").encode()).hexdigest() == g["md5"]
+    # every certificate passes the verifier's criterion: the factor divides
+    # x^r + x^s + 1 and is irreducible
+    import oracle as O
+    for ln in lines:
+        s, d, hx = ln.split()
+        mask, d = int(hx[1:], 16), int(d)
+        assert O.divides_trinomial_small(3021377, int(s), mask, d), ln
+        assert O.is_irreducible_small(mask, d), ln

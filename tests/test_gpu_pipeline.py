@@ -62,3 +62,34 @@ def test_least_degree_sample_vs_oracle_scan(c1):
         d = int(lines[int(s)].split()[1])
         kind, d2, hx, _ = O.scan_one(R, int(s), 20, d)
         assert (kind, d2) == (1, d) and lines[int(s)] == f"{s} {d} p{hx}"
+
+
+def test_slice_r136279841_256_certified():
+    # a 256-s slice at r=136279841: sieve lines to degree 32, deep scan of
+    # the survivors to degree 1024 ("s u" beyond); every factor line passes
+    # the verifier (divides x^r + x^s + 1, irreducible)
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import certs
+    r, lo = 136279841, 20250000
+    best, _ = M.run_gpu(r, 32, lo + 255)
+    seg = best[lo:lo + 256]
+    lines = {}
+    surv = []
+    for k, key in enumerate(seg):
+        s = lo + k
+        if key == O.NONE:
+            surv.append(s)
+        else:
+            lines[s] = certs.key_line(s, int(key))
+    assert 0 < len(surv) < 64
+    for o in M.scan(r, surv, k0=32, maxd=1024, L0=1024, Lmax=1024, gcd_threads=16):
+        lines[o.s] = o.line()
+    assert sorted(lines) == list(range(lo, lo + 256))
+    for s, ln in lines.items():
+        parts = ln.split()
+        if len(parts) == 3:
+            f = O.poly_from_hex(parts[2][1:])
+            assert O.deg(f) == int(parts[1])
+            assert O.verify_factor(r, s, f), ln
+        else:
+            assert parts[1] == "u", ln
