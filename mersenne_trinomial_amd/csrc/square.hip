@@ -160,8 +160,8 @@ __global__ __launch_bounds__(256) void k_stream_copy(const ulonglong2 *__restric
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (NT) {
-                __builtin_nontemporal_store(v[k].x, &dst[i + k * stride].x);
-                __builtin_nontemporal_store(v[k].y, &dst[i + k * stride].y);
+                typedef u64 v2u __attribute__((ext_vector_type(2)));
+                __builtin_nontemporal_store(v2u{v[k].x, v[k].y}, (v2u *)&dst[i + k * stride]);
             } else {
                 dst[i + k * stride] = v[k];
             }
