@@ -34,10 +34,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one k_square_fold_batch launch moves for 256 residues at
 # r = 136279841 (default tuning), from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # runs scaled by the factors a copy kernel of known size gives (x2.0 for
-# FETCH_SIZE, x1.0 for WRITE_SIZE): 6,736,154,912 B fetched + 4,570,116,256 B
+# FETCH_SIZE, x1.0 for WRITE_SIZE): 6,736,592,640 B fetched + 4,360,962,104 B
 # written.  The command lines and raw counts are kept in
 # profiles/k_square_fold_batch_r136279841_256res.md.
-SQ_TRAFFIC_256 = 6736154912 + 4570116256
+SQ_TRAFFIC_256 = 6736592640 + 4360962104
 
 
 def parse():

@@ -86,7 +86,8 @@ int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r,
 /* Batched Frobenius chains: residue i (r[i], s[i], nw_i words in
  * dres[i], scratch dtmp[i]) is squared steps[i] times; steps may differ
  * (uneven frontiers) and r may differ between residues.  Results end in
- * dres[i].  All arrays are host arrays of n entries.  No reference
+ * dres[i] (buffers 16-byte aligned, as gf2t_malloc returns them).  All
+ * arrays are host arrays of n entries.  No reference
  * counterpart: it batches the per-s loop of src/trinomial_scan.cu:364-
  * 366 / src/tsfactor.cu:294-300 across s values. */
 int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,

@@ -138,6 +138,59 @@ u64 irreducible_count(int d) {
     return (u64)(acc / d);
 }
 
+// Per-degree host constants, computed once per process: the smallest
+// primitive modulus q, the antilog tables g^k and g^(65536 k) (k < 2^16),
+// and the trace mask (bit i = Tr(x^i)).  The g^(65536 k) table steps by a
+// fixed multiplier, which is linear over GF(2): four 256-entry byte tables
+// make each step four lookups.
+struct DegHost {
+    bool ready = false, trace_ok = false;
+    u64 q = 0;
+    u32 tmask = 0;
+    std::vector<u32> lo, hi;
+};
+std::mutex g_deg_mu;
+DegHost g_deg[33];
+
+const DegHost &deg_host(int d) {
+    std::lock_guard<std::mutex> lk(g_deg_mu);
+    DegHost &H = g_deg[d];
+    if (H.ready) return H;
+    H.q = primitive_poly(d);
+    H.lo.assign(65536, 0);
+    H.hi.assign(65536, 0);
+    if (H.q) {
+        const u64 q = H.q;
+        u64 a = 1;
+        for (u64 k = 0; k < 65536; k++) {
+            H.lo[k] = (u32)a;
+            a <<= 1;
+            if ((a >> d) & 1) a ^= q;
+        }
+        const u64 g16 = a;   // g^65536
+        u32 T[4][256];
+        for (int j = 0; j < 4; j++)
+            for (u64 v = 0; v < 256; v++) {
+                u64 e = v << (8 * j);
+                T[j][v] = (e >> d) ? 0u : (u32)hmulmod(e, g16, q, d);
+            }
+        u32 b = 1;
+        for (u64 k = 0; k < 65536; k++) {
+            H.hi[k] = b;
+            b = T[0][b & 255] ^ T[1][(b >> 8) & 255] ^ T[2][(b >> 16) & 255] ^ T[3][b >> 24];
+        }
+        H.trace_ok = true;
+        for (int i = 0; i < d; i++) {
+            u64 y = (u64)1 << i, acc = y;
+            for (int k = 1; k < d; k++) { y = hmulmod(y, y, q, d); acc ^= y; }
+            if (acc > 1) { H.trace_ok = false; break; }
+            H.tmask |= (u32)acc << i;
+        }
+    }
+    H.ready = true;
+    return H;
+}
+
 // ---------------------------------------------------------- device side
 struct DegreeParams {
     int d;
@@ -490,47 +543,28 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
     unsigned long long *cnt64 = (unsigned long long *)bCnt.p;   // [0] nsurv [1] nqueue
     unsigned *cnt32 = (unsigned *)((char *)bCnt.p + 32);         // [0] nheavy [1] bad
     u32 *Alo = (u32 *)bTab.p, *Ahi = Alo + 65536;
-    std::vector<u32> hlo(65536), hhi(65536);
     hipEvent_t ev[4];
     for (auto &e : ev) GF2T_HIP(hipEventCreate(&e));
 
     int status = GF2T_OK;
     for (int d = dmin; d <= dmax && status == GF2T_OK; d++) {
         const u64 M = ((u64)1 << d) - 1;
-        const u64 q = primitive_poly(d);
+        const DegHost &H = deg_host(d);
+        const u64 q = H.q;
         const u64 rmod = r % M;
         if (!q || gcd_host(rmod ? rmod : M, M) != 1) {
             gf2t::set_error("gf2t_sieve: d=%d: gcd(r, 2^d-1) != 1, unsupported r", d);
             status = GF2T_EINVAL;
             break;
         }
-        // g^k and g^(k<<16) tables on the host (LFSR steps)
-        u64 a = 1;
-        for (u64 k = 0; k < 65536; k++) {
-            hlo[k] = (u32)a;
-            a <<= 1;
-            if ((a >> d) & 1) a ^= q;
+        if (!H.trace_ok) {
+            gf2t::set_error("gf2t_sieve: d=%d: trace not in GF(2)", d);
+            status = GF2T_EINTERNAL;
+            break;
         }
-        u64 g16 = a, b = 1;   // a = g^65536
-        for (u64 k = 0; k < 65536; k++) {
-            hhi[k] = (u32)b;
-            b = hmulmod(b, g16, q, d);
-        }
-        GF2T_HIP(hipMemcpy(Alo, hlo.data(), 65536 * 4, hipMemcpyHostToDevice));
-        GF2T_HIP(hipMemcpy(Ahi, hhi.data(), 65536 * 4, hipMemcpyHostToDevice));
-        // trace mask: bit i = Tr(x^i) = sum_k (x^i)^(2^k), which lies in GF(2)
-        u32 tmask = 0;
-        for (int i = 0; i < d; i++) {
-            u64 y = (u64)1 << i, acc = y;
-            for (int k = 1; k < d; k++) { y = hmulmod(y, y, q, d); acc ^= y; }
-            if (acc > 1) {
-                gf2t::set_error("gf2t_sieve: d=%d: trace not in GF(2)", d);
-                status = GF2T_EINTERNAL;
-                break;
-            }
-            tmask |= (u32)acc << i;
-        }
-        if (status) break;
+        GF2T_HIP(hipMemcpy(Alo, H.lo.data(), 65536 * 4, hipMemcpyHostToDevice));
+        GF2T_HIP(hipMemcpy(Ahi, H.hi.data(), 65536 * 4, hipMemcpyHostToDevice));
+        const u32 tmask = H.tmask;
         DegreeParams P{d, M, q, rmod, smax, tmask, Alo, Ahi};
 
         const u64 ncls = irreducible_count(d);

@@ -67,8 +67,13 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
             square_fold_pair_fast(d.in, d.c, i, o0, o1);
             if (NT) {
                 // streaming store: F^2 is not re-read before the next launch
-                __builtin_nontemporal_store(o0, d.out + 2 * i);
-                if (2 * i + 1 < d.c.nw) __builtin_nontemporal_store(o1, d.out + 2 * i + 1);
+                // (one 16-byte store per pair; residue buffers are 16-byte
+                // aligned)
+                typedef u64 v2u __attribute__((ext_vector_type(2)));
+                if (2 * i + 1 < d.c.nw)
+                    __builtin_nontemporal_store(v2u{o0, o1}, (v2u *)(d.out + 2 * i));
+                else
+                    __builtin_nontemporal_store(o0, d.out + 2 * i);
             } else {
                 d.out[2 * i] = o0;
                 if (2 * i + 1 < d.c.nw) d.out[2 * i + 1] = o1;
@@ -256,8 +261,10 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
         return GF2T_ERANGE;
     }
     for (int i = 0; i < n; i++)
-        if (!valid_rs(r[i], s[i]) || !dres[i] || !dtmp[i] || dres[i] == dtmp[i]) {
-            gf2t::set_error("gf2t_square_fold_batch: residue %d has bad r/s/buffers", i);
+        if (!valid_rs(r[i], s[i]) || !dres[i] || !dtmp[i] || dres[i] == dtmp[i] ||
+            (((uintptr_t)dres[i] | (uintptr_t)dtmp[i]) & 15)) {
+            gf2t::set_error("gf2t_square_fold_batch: residue %d has bad r/s/buffers "
+                            "(distinct, 16-byte aligned)", i);
             return GF2T_EINVAL;
         }
     hipStream_t st = (hipStream_t)stream;
