@@ -263,6 +263,10 @@ typedef struct gf2t_sieve_stats {
 int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax,
                uint64_t *dbest, gf2t_sieve_stats *stats);
 
+/* The sieve keeps its device tables (up to ~83 GB at dmax = 32) between
+ * calls; this frees them.  No reference counterpart. */
+int gf2t_sieve_release(void);
+
 /* Smallest primitive polynomial of degree d (find_primitive_poly,
  * src/gf2_field_setup.h:49-57); 0 if d is out of range. */
 uint64_t gf2t_primitive_poly(int d);

@@ -99,6 +99,7 @@ _SIGS = {
     "gf2t_scan_results_free": (None, [_vp]),
     "gf2t_gcd_host": (C.c_int64, [_p64, _sz, _p64, _sz, _p64, _sz]),
     "gf2t_mul_host": (C.c_int, [_p64, _u64, _p64, _u64, _p64]),
+    "gf2t_sieve_release": (C.c_int, []),
     "gf2t_server_create": (C.c_int, [_u64, C.POINTER(ScanParams), _sz, C.POINTER(_vp)]),
     "gf2t_server_capacity": (C.c_int, [_vp, C.POINTER(_sz), C.POINTER(_sz)]),
     "gf2t_server_submit": (C.c_int, [_vp, _u64, _p64, _sz, _u64, _u64]),

@@ -494,7 +494,11 @@ __global__ void k_zero_counters(unsigned long long *a, unsigned *b) {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
     int ensure(size_t b) {
         if (bytes >= b) return GF2T_OK;
         if (p) GF2T_HIP(hipFree(p));
@@ -506,7 +510,23 @@ struct DevBuf {
     }
 };
 
+struct SieveWork {
+    DevBuf bL, bSurv, bQueue, bHeavy, bTab, bCnt, bP1, bP2, bCur;
+};
+std::mutex g_sw_mu;
+SieveWork *g_sw = nullptr;   // never destroyed at exit: the runtime may be gone by then
+
 }  // namespace
+
+extern "C" int gf2t_sieve_release(void) {
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    if (g_sw) {
+        for (DevBuf *b : {&g_sw->bL, &g_sw->bSurv, &g_sw->bQueue, &g_sw->bHeavy, &g_sw->bTab,
+                          &g_sw->bCnt, &g_sw->bP1, &g_sw->bP2, &g_sw->bCur})
+            b->release();
+    }
+    return GF2T_OK;
+}
 
 extern "C" uint64_t gf2t_primitive_poly(int d) {
     if (d < 2 || d > 32) return 0;
@@ -520,7 +540,13 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
         gf2t::set_error("gf2t_sieve: need 2 <= dmin <= dmax <= 32, smax >= 1, dbest");
         return GF2T_EINVAL;
     }
-    DevBuf bL, bSurv, bQueue, bHeavy, bTab, bCnt, bP1, bP2, bCur;
+    // device tables stay allocated between calls (grow-only; freed by
+    // gf2t_sieve_release), so repeated sieves pay no allocation
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    if (!g_sw) g_sw = new SieveWork();
+    DevBuf &bL = g_sw->bL, &bSurv = g_sw->bSurv, &bQueue = g_sw->bQueue, &bHeavy = g_sw->bHeavy,
+           &bTab = g_sw->bTab, &bCnt = g_sw->bCnt, &bP1 = g_sw->bP1, &bP2 = g_sw->bP2,
+           &bCur = g_sw->bCur;
     int rc;
     if ((rc = bL.ensure(((size_t)1 << dmax) * sizeof(u32)))) return rc;
     if (dmax >= LP_MIN_D) {
