@@ -391,7 +391,7 @@ def main():
     P64 = C.POINTER(C.c_uint64)
     rr = np.full(n, R, dtype=np.uint64)
     kk = np.full(n, args.chain_steps, dtype=np.uint64)
-    stats = (SieveStats * 33)()
+    stats = (SieveStats * 35)()
     svals = None
 
     def one_step(timed: bool):

@@ -255,11 +255,14 @@ typedef struct gf2t_sieve_stats {
 } gf2t_sieve_stats;
 
 /* Mark every s in [1, smax] divisible by an irreducible of degree d in
- * [dmin, dmax] (2 <= dmin <= dmax <= 32) into dbest (smax+1 uint64 on
+ * [dmin, dmax] (2 <= dmin <= dmax <= 34) into dbest (smax+1 uint64 on
  * the device, atomic min of (d<<48)|mask; initialise to ~0 or a prior
  * image).  Replaces run_gpu/sieve_degree_gpu (src/coarse_sieve.cu:852-
  * 918, :1119-1225).  stats (host, dmax+1 entries, may be NULL) gets one
- * record per degree.  Synchronous. */
+ * record per degree.  Degrees 33..34 (coarse_sieve_wide, src/
+ * gf2_wide_field.h) use 64-bit field elements and dense tables of about
+ * 201 GB at d = 34; the tables of degrees <= 32 are freed first.
+ * Synchronous. */
 int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax,
                uint64_t *dbest, gf2t_sieve_stats *stats);
 

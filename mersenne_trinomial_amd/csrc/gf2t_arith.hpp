@@ -93,6 +93,91 @@ GF2T_HD u32 fmul(u32 a, u32 b, u64 q, int d) { return fred(clmul32(a, b), q, d);
 GF2T_HD u32 fsqr(u32 a, u64 q, int d) { return fred(spread32(a), q, d); }
 
 
+// ------------------------------------------------ GF(2^d), 32 < d <= 40
+// 64 x 64 -> 128-bit carry-less product (Karatsuba over clmul32)
+GF2T_HD void clmul64(u64 a, u64 b, u64 &lo, u64 &hi) {
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 l = clmul32(a0, b0), h = clmul32(a1, b1);
+    u64 mid = clmul32(a0 ^ a1, b0 ^ b1) ^ l ^ h;
+    lo = l ^ (mid << 32);
+    hi = h ^ (mid >> 32);
+}
+// remainder of the 128-bit product (degree <= 2d-2) modulo q (degree d)
+GF2T_HD u64 fred64(u64 lo, u64 hi, u64 q, int d) {
+    for (int b = 2 * d - 2; b >= d; b--) {
+        u64 bit = b >= 64 ? (hi >> (b - 64)) & 1 : (lo >> b) & 1;
+        if (!bit) continue;
+        int sh = b - d;
+        lo ^= q << sh;
+        if (sh) hi ^= q >> (64 - sh);
+    }
+    return lo;
+}
+GF2T_HD u64 fmul64(u64 a, u64 b, u64 q, int d) {
+    u64 lo, hi;
+    clmul64(a, b, lo, hi);
+    return fred64(lo, hi, q, d);
+}
+// strict minimum of its rotations within d <= 63 bits (Lyndon word)
+GF2T_HD bool is_lyndon64(u64 j, int d, u64 M) {
+    u64 jj = j;
+    for (int i = 1; i < d; i++) {
+        jj = ((jj << 1) | (jj >> (d - 1))) & M;
+        if (jj <= j) return false;
+    }
+    return true;
+}
+GF2T_HD u64 gcd64(u64 a, u64 b) {
+    while (b) { u64 t = a % b; a = b; b = t; }
+    return a;
+}
+// inverse of a modulo m (gcd(a, m) = 1, 2 <= m < 2^62)
+GF2T_HD u64 inv64(u64 a, u64 m) {
+    u64 r0 = m, r1 = a % m;
+    long long t0 = 0, t1 = 1;
+    while (r1) {
+        u64 q = r0 / r1;
+        u64 r2 = r0 - q * r1;
+        long long t2 = t0 - (long long)q * t1;
+        r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+    }
+    if (t0 < 0) t0 += (long long)m;
+    return (u64)t0;
+}
+// a * b mod m for a, b < m < 2^62 (shift-and-add, no 128-bit division)
+GF2T_HD u64 mulmod64(u64 a, u64 b, u64 m) {
+    u64 r = 0;
+    a %= m;
+    while (b) {
+        if (b & 1) { r += a; if (r >= m) r -= m; }
+        a <<= 1;
+        if (a >= m) a -= m;
+        b >>= 1;
+    }
+    return r;
+}
+// Berlekamp-Massey minimal polynomial as minpoly_bm, for 32 < d <= 40:
+// the trace sequence is generated on the fly (2d terms do not fit a word)
+GF2T_HD u64 minpoly_bm64(u64 beta, u64 q, int d, u64 tmask) {
+    u64 C = 1, B = 1, W = 0, pw = 1;
+    int L = 0, m = 1;
+    for (int n = 0; n < 2 * d; n++) {
+        u64 a = (u64)(__builtin_popcountll(pw & tmask) & 1);
+        pw = fmul64(pw, beta, q, d);
+        W = (W << 1) | a;                               // bit j = a_{n-j}
+        if (!(__builtin_popcountll(C & W) & 1)) { m++; continue; }
+        if (m > 63) return 0;   // cannot happen at linear complexity <= 40
+        u64 T = C;
+        C ^= B << m;
+        if (2 * L <= n) { L = n + 1 - L; B = T; m = 1; }
+        else m++;
+    }
+    if (L != d) return 0;
+    u64 mask = 0;
+    for (int k = 0; k <= d; k++) mask |= ((C >> (d - k)) & 1) << k;
+    return mask;
+}
+
 // ------------------------------------------- squaring + trinomial fold
 // 128-bit window of the virtual spread SP of `in` (SP[2k] = bit k of in,
 // odd positions 0) at SP position P, keeping only SP indices in [lo, hi).

@@ -510,8 +510,303 @@ struct DevBuf {
     }
 };
 
+// ===================================================== degrees 33..34
+// Elements of GF(2^d) are u64.  The log table holds d-bit exponents, so
+// it is stored as the low 32 bits (L_lo, u32) plus the top two bits
+// packed sixteen to a word (L_hi).  It is built by the same radix
+// partition as above, except that the level-2 pass runs in place, one
+// level-1 bucket at a time through a bucket-sized scratch region, so the
+// whole build needs 8 bytes per element plus the table (201 GB at d = 34,
+// inside 288 GB).  Entries are (g^i mod 2^(d-8)) << 36 | i.
+constexpr int WIDE_MAX_D = 34;
+constexpr int W_SB = 15;            // values per placed slice: 2^15
+constexpr int W_E = 8192;           // entries per block in passes 1 and 2
+constexpr int W_T = 256;
+constexpr int W_IB = 36;            // bits of i in an entry
+constexpr int W_B1 = 8;             // level-1 bucket bits
+constexpr int W_AB = 18;            // antilog table split: g^e = A1[e >> 18] * A0[e & (2^18 - 1)]
+
+struct WParams {
+    int d;
+    u64 M, q, rmod, smax, tmask;
+    const u64 *A0, *A1;
+};
+
+GF2T_DEV u64 antilog_w(const WParams &P, u64 e) {
+    return fmul64(P.A1[e >> W_AB], P.A0[e & ((1u << W_AB) - 1)], P.q, P.d);
+}
+
+// block-wide exclusive scan of cnt[0..nb) (nb <= W_T * 16) into off
+__device__ void block_scan(const unsigned *cnt, unsigned *off, unsigned nb, unsigned *part) {
+    const unsigned per = (nb + W_T - 1) / W_T;
+    unsigned t = threadIdx.x, sum = 0;
+    for (unsigned k = t * per; k < nb && k < (t + 1) * per; k++) sum += cnt[k];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        unsigned acc = 0;
+        for (unsigned k = 0; k < W_T; k++) { unsigned v = part[k]; part[k] = acc; acc += v; }
+    }
+    __syncthreads();
+    unsigned acc = part[t];
+    for (unsigned k = t * per; k < nb && k < (t + 1) * per; k++) { off[k] = acc; acc += cnt[k]; }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *__restrict__ out,
+                                                 unsigned *__restrict__ cursor,
+                                                 unsigned *__restrict__ bad) {
+    __shared__ unsigned cnt[256], off[256], gbase[256], part[W_T];
+    __shared__ unsigned long long stage[W_E];
+    __shared__ unsigned char sbkt[W_E];   // bucket of each staged entry
+    const int shift = P.d - W_B1;
+    const u64 cap = (u64)1 << shift;
+    const u64 i0 = (u64)blockIdx.x * W_E;
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    constexpr int PER = W_E / W_T;
+    const u64 my0 = i0 + (u64)threadIdx.x * PER;
+    unsigned rank[PER];
+    u64 a0 = my0 < P.M ? antilog_w(P, my0) : 0, a = a0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        rank[k] = 0;
+        if (my0 + k < P.M) rank[k] = atomicAdd(&cnt[(unsigned)(a >> shift)], 1u);
+        a <<= 1;
+        if (a >> P.d) a ^= P.q;
+    }
+    __syncthreads();
+    block_scan(cnt, off, 256, part);
+    gbase[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]) : 0;
+    a = a0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        if (my0 + k < P.M) {
+            const unsigned b = (unsigned)(a >> shift);
+            const unsigned at = off[b] + rank[k];
+            stage[at] = ((unsigned long long)(a & (cap - 1)) << W_IB) | (my0 + k);
+            sbkt[at] = (unsigned char)b;
+        }
+        a <<= 1;
+        if (a >> P.d) a ^= P.q;
+    }
+    __syncthreads();
+    const u64 nvalid = (i0 + W_E <= P.M) ? W_E : P.M - i0;
+    for (u64 k = threadIdx.x; k < nvalid; k += W_T) {
+        const unsigned b = sbkt[k];
+        u64 pos = (u64)gbase[b] + (k - off[b]);
+        if (pos >= cap) { atomicOr(bad, 4u); continue; }
+        out[(u64)b * cap + pos] = stage[k];
+    }
+}
+
+// level 2 for one level-1 bucket: entries in[0..count) (a copy of the
+// bucket), sorted by bits [W_SB, d - 8) of the stored value into
+// out[sub * 2^W_SB + pos]
+__global__ __launch_bounds__(W_T) void k_w_part2(int B2, const unsigned long long *__restrict__ in,
+                                                 u64 count, unsigned long long *__restrict__ out,
+                                                 unsigned *__restrict__ cursor,
+                                                 unsigned *__restrict__ bad) {
+    __shared__ unsigned cnt[2048], off[2048], gbase[2048], part[W_T];
+    __shared__ unsigned long long stage[W_E];
+    const unsigned nb2 = 1u << B2;
+    const u64 cap2 = (u64)1 << W_SB;
+    const u64 k0 = (u64)blockIdx.x * W_E;
+    if (k0 >= count) return;
+    const u64 nvalid = (k0 + W_E <= count) ? W_E : count - k0;
+    for (unsigned b = threadIdx.x; b < nb2; b += W_T) cnt[b] = 0;
+    __syncthreads();
+    constexpr int PER = W_E / W_T;
+    unsigned rank[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        u64 idx = (u64)k * W_T + threadIdx.x;
+        rank[k] = idx < nvalid
+                      ? atomicAdd(&cnt[(unsigned)(in[k0 + idx] >> (W_IB + W_SB)) & (nb2 - 1)], 1u)
+                      : 0;
+    }
+    __syncthreads();
+    block_scan(cnt, off, nb2, part);
+    for (unsigned b = threadIdx.x; b < nb2; b += W_T)
+        gbase[b] = cnt[b] ? atomicAdd(&cursor[b], cnt[b]) : 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        u64 idx = (u64)k * W_T + threadIdx.x;
+        if (idx < nvalid) {
+            unsigned long long e = in[k0 + idx];
+            stage[off[(unsigned)(e >> (W_IB + W_SB)) & (nb2 - 1)] + rank[k]] = e;
+        }
+    }
+    __syncthreads();
+    for (u64 k = threadIdx.x; k < nvalid; k += W_T) {
+        unsigned long long e = stage[k];
+        unsigned b = (unsigned)(e >> (W_IB + W_SB)) & (nb2 - 1);
+        u64 pos = (u64)gbase[b] + (k - off[b]);
+        if (pos >= cap2) { atomicOr(bad, 4u); continue; }
+        out[(u64)b * cap2 + pos] = e;
+    }
+}
+
+// one block per slice z of 2^W_SB values: L_lo and the packed top bits
+__global__ __launch_bounds__(1024) void k_w_place(const unsigned long long *__restrict__ in,
+                                                  u32 *__restrict__ Llo, u32 *__restrict__ Lhi) {
+    extern __shared__ u32 wtile[];
+    u32 *lo = wtile, *hi = wtile + (1u << W_SB);          // hi: 2 bits per value
+    const u64 cap = (u64)1 << W_SB;
+    const u64 z = blockIdx.x;
+    const u64 count = z ? cap : cap - 1;
+    const unsigned long long *src = in + z * cap;
+    for (u64 k = threadIdx.x; k < cap / 16; k += blockDim.x) hi[k] = 0;
+    if (z == 0 && threadIdx.x == 0) lo[0] = 0;
+    __syncthreads();
+    for (u64 k = threadIdx.x; k < count; k += blockDim.x) {
+        unsigned long long e = src[k];
+        u32 v = (u32)(e >> W_IB) & (u32)(cap - 1);
+        u64 i = e & (((u64)1 << W_IB) - 1);
+        lo[v] = (u32)i;
+        atomicOr(&hi[v >> 4], (u32)((i >> 32) & 3) << (2 * (v & 15)));
+    }
+    __syncthreads();
+    for (u64 k = threadIdx.x; k < cap; k += blockDim.x) Llo[z * cap + k] = lo[k];
+    for (u64 k = threadIdx.x; k < cap / 16; k += blockDim.x) Lhi[z * (cap / 16) + k] = hi[k];
+}
+
+__global__ void k_lyndon_w(int d, u64 M, u64 *__restrict__ surv, unsigned long long *__restrict__ nsurv,
+                           u64 cap) {
+    const int lane = threadIdx.x & 63;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 ncand = (u64)1 << (d - 2);
+    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < ncand;
+         wbase += stride) {
+        u64 c = wbase + lane;
+        u64 j = 2 * c + 1;
+        bool keep = c < ncand && is_lyndon64(j, d, M);
+        unsigned long long m = __ballot(keep);
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(nsurv, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (keep) {
+            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
+            if (pos < cap) surv[pos] = j;
+        }
+    }
+}
+
+struct MarkRecW {
+    u64 j, n, sf;
+};
+
+__global__ void k_zech_w(WParams P, const u64 *__restrict__ surv, u64 nsurv,
+                         const u32 *__restrict__ Llo, const u32 *__restrict__ Lhi,
+                         MarkRecW *__restrict__ queue, unsigned long long *__restrict__ nqueue,
+                         u64 qcap) {
+    const int lane = threadIdx.x & 63;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < nsurv;
+         wbase += stride) {
+        u64 t = wbase + lane;
+        bool keep = false;
+        MarkRecW rec{0, 0, 0};
+        if (t < nsurv) {
+            const u64 j = surv[t];
+            const u64 u = mulmod64(j, P.rmod, P.M);
+            if (u) {
+                const u64 y = antilog_w(P, u) ^ 1u;
+                const u64 v = (u64)Llo[y] | ((u64)((Lhi[y >> 4] >> (2 * (y & 15))) & 3) << 32);
+                const u64 gg = gcd64(j, P.M);
+                if (v % gg == 0) {
+                    const u64 n = P.M / gg;
+                    const u64 iv = inv64((j / gg) % n, n);
+                    const u64 s0 = mulmod64(iv, v / gg, n);
+                    const u64 sf = s0 ? s0 : n;
+                    if (sf <= P.smax) { keep = true; rec = MarkRecW{j, n, sf}; }
+                }
+            }
+        }
+        unsigned long long m = __ballot(keep);
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(nqueue, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0);
+        if (keep) {
+            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
+            if (pos < qcap) queue[pos] = rec;
+        }
+    }
+}
+
+__global__ void k_mark_w(WParams P, const MarkRecW *__restrict__ queue, u64 nq,
+                         unsigned long long *__restrict__ best, Heavy *__restrict__ heavy,
+                         unsigned *__restrict__ nheavy, unsigned *__restrict__ bad) {
+    u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq) return;
+    const MarkRecW rec = queue[t];
+    const int d = P.d;
+    u64 mask = minpoly_bm64(antilog_w(P, rec.j), P.q, d, P.tmask);
+    if (mask == 0 || !((mask >> d) & 1) || !(mask & 1)) { atomicAdd(bad, 1u); return; }
+    unsigned long long key = ((unsigned long long)d << 48) | mask;
+    u64 count = (P.smax - rec.sf) / rec.n + 1;
+    if (count <= INLINE_MARKS) {
+        for (u64 s = rec.sf; s <= P.smax; s += rec.n)
+            if (best[s] > key) atomicMin(&best[s], key);
+    } else {
+        unsigned idx = atomicAdd(nheavy, 1u);
+        if (idx < HEAVY_CAP) heavy[idx] = Heavy{rec.sf, rec.n, key};
+        else atomicAdd(bad, 2u);
+    }
+}
+
+struct WideHost {
+    bool ready = false, trace_ok = false;
+    u64 q = 0, tmask = 0;
+    std::vector<u64> a0, a1;
+};
+std::mutex g_wide_mu;
+WideHost g_wide[WIDE_MAX_D + 1];
+
+const WideHost &wide_host(int d) {
+    std::lock_guard<std::mutex> lk(g_wide_mu);
+    WideHost &H = g_wide[d];
+    if (H.ready) return H;
+    H.q = primitive_poly(d);
+    const size_t na = (size_t)1 << W_AB;
+    H.a0.assign(na, 0);
+    H.a1.assign(na, 0);
+    if (H.q) {
+        const u64 q = H.q;
+        u64 a = 1;
+        for (size_t k = 0; k < na; k++) {
+            H.a0[k] = a;
+            a <<= 1;
+            if ((a >> d) & 1) a ^= q;
+        }
+        const u64 gstep = a;   // g^(2^18); multiplying by it is linear: byte tables
+        u64 T[5][256];
+        for (int j = 0; j < 5; j++)
+            for (u64 v = 0; v < 256; v++) {
+                u64 e = v << (8 * j);
+                T[j][v] = (e >> d) ? 0 : hmulmod(e, gstep, q, d);
+            }
+        u64 b = 1;
+        for (size_t k = 0; k < na; k++) {
+            H.a1[k] = b;
+            b = T[0][b & 255] ^ T[1][(b >> 8) & 255] ^ T[2][(b >> 16) & 255] ^
+                T[3][(b >> 24) & 255] ^ T[4][(b >> 32) & 255];
+        }
+        H.trace_ok = true;
+        for (int i = 0; i < d; i++) {
+            u64 y = (u64)1 << i, acc = y;
+            for (int k = 1; k < d; k++) { y = hmulmod(y, y, q, d); acc ^= y; }
+            if (acc > 1) { H.trace_ok = false; break; }
+            H.tmask |= acc << i;
+        }
+    }
+    H.ready = true;
+    return H;
+}
+
 struct SieveWork {
     DevBuf bL, bSurv, bQueue, bHeavy, bTab, bCnt, bP1, bP2, bCur;
+    DevBuf wP1, wTmp, wLlo, wLhi, wSurv, wQueue, wTab, wCur;   // degrees 33..34
 };
 std::mutex g_sw_mu;
 SieveWork *g_sw = nullptr;   // never destroyed at exit: the runtime may be gone by then
@@ -522,28 +817,23 @@ extern "C" int gf2t_sieve_release(void) {
     std::lock_guard<std::mutex> lk(g_sw_mu);
     if (g_sw) {
         for (DevBuf *b : {&g_sw->bL, &g_sw->bSurv, &g_sw->bQueue, &g_sw->bHeavy, &g_sw->bTab,
-                          &g_sw->bCnt, &g_sw->bP1, &g_sw->bP2, &g_sw->bCur})
+                          &g_sw->bCnt, &g_sw->bP1, &g_sw->bP2, &g_sw->bCur, &g_sw->wP1,
+                          &g_sw->wTmp, &g_sw->wLlo, &g_sw->wLhi, &g_sw->wSurv, &g_sw->wQueue,
+                          &g_sw->wTab, &g_sw->wCur})
             b->release();
     }
     return GF2T_OK;
 }
 
 extern "C" uint64_t gf2t_primitive_poly(int d) {
-    if (d < 2 || d > 32) return 0;
+    if (d < 2 || d > 40) return 0;
     return primitive_poly(d);
 }
 
-extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
-                          gf2t_sieve_stats *stats) {
-    if (dmin < 2) dmin = 2;
-    if (dmax > 32 || dmin > dmax || !dbest || smax < 1) {
-        gf2t::set_error("gf2t_sieve: need 2 <= dmin <= dmax <= 32, smax >= 1, dbest");
-        return GF2T_EINVAL;
-    }
-    // device tables stay allocated between calls (grow-only; freed by
-    // gf2t_sieve_release), so repeated sieves pay no allocation
-    std::lock_guard<std::mutex> lk(g_sw_mu);
-    if (!g_sw) g_sw = new SieveWork();
+namespace {
+// degrees dmin..dmax <= 32 (u32 field elements); g_sw_mu held by the caller
+int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
+                 gf2t_sieve_stats *stats) {
     DevBuf &bL = g_sw->bL, &bSurv = g_sw->bSurv, &bQueue = g_sw->bQueue, &bHeavy = g_sw->bHeavy,
            &bTab = g_sw->bTab, &bCnt = g_sw->bCnt, &bP1 = g_sw->bP1, &bP2 = g_sw->bP2,
            &bCur = g_sw->bCur;
@@ -695,5 +985,181 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
+    return status;
+}
+
+// degrees dmin..dmax in 33..34 (u64 field elements); g_sw_mu held
+int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
+               gf2t_sieve_stats *stats) {
+    SieveWork &W = *g_sw;
+    int rc;
+    const u64 N = (u64)1 << dmax;
+    if ((rc = W.wP1.ensure(N * 8))) return rc;
+    if ((rc = W.wTmp.ensure((N >> W_B1) * 8))) return rc;
+    if ((rc = W.wLlo.ensure(N * 4))) return rc;
+    if ((rc = W.wLhi.ensure(N / 4))) return rc;
+    if ((rc = W.wTab.ensure(2 * ((size_t)1 << W_AB) * 8))) return rc;
+    if ((rc = W.wCur.ensure((256 + 2048) * 4))) return rc;
+    if ((rc = W.bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
+    if ((rc = W.bCnt.ensure(64))) return rc;
+    static std::once_flag lds_once;
+    static hipError_t lds_err = hipSuccess;
+    const int place_lds = (int)((((size_t)1 << W_SB) + ((size_t)1 << W_SB) / 16) * sizeof(u32));
+    std::call_once(lds_once, [place_lds] {
+        lds_err = hipFuncSetAttribute((const void *)k_w_place,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, place_lds);
+    });
+    GF2T_HIP(lds_err);
+    unsigned long long *cnt64 = (unsigned long long *)W.bCnt.p;
+    unsigned *cnt32 = (unsigned *)((char *)W.bCnt.p + 32);
+    unsigned *cur1 = (unsigned *)W.wCur.p, *cur2 = cur1 + 256;
+    unsigned long long *P1 = (unsigned long long *)W.wP1.p, *tmp = (unsigned long long *)W.wTmp.p;
+    u32 *Llo = (u32 *)W.wLlo.p, *Lhi = (u32 *)W.wLhi.p;
+    u64 *A0 = (u64 *)W.wTab.p, *A1 = A0 + ((size_t)1 << W_AB);
+    Heavy *heavy = (Heavy *)W.bHeavy.p;
+    hipEvent_t ev[4];
+    for (auto &e : ev) GF2T_HIP(hipEventCreate(&e));
+    int status = GF2T_OK;
+    for (int d = dmin; d <= dmax && status == GF2T_OK; d++) {
+        const u64 M = ((u64)1 << d) - 1;
+        const WideHost &H = wide_host(d);
+        const u64 q = H.q, rmod = r % M;
+        if (!q || gcd_host(rmod ? rmod : M, M) != 1) {
+            gf2t::set_error("gf2t_sieve: d=%d: gcd(r, 2^d-1) != 1, unsupported r", d);
+            status = GF2T_EINVAL;
+            break;
+        }
+        if (!H.trace_ok) {
+            gf2t::set_error("gf2t_sieve: d=%d: trace not in GF(2)", d);
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        GF2T_HIP(hipMemcpy(A0, H.a0.data(), H.a0.size() * 8, hipMemcpyHostToDevice));
+        GF2T_HIP(hipMemcpy(A1, H.a1.data(), H.a1.size() * 8, hipMemcpyHostToDevice));
+        WParams P{d, M, q, rmod, smax, H.tmask, A0, A1};
+        const u64 ncls = irreducible_count(d);
+        const u64 qcap = ncls / 16 + 65536;
+        if ((rc = W.wSurv.ensure((ncls + 64) * 8))) { status = rc; break; }
+        if ((rc = W.wQueue.ensure(qcap * sizeof(MarkRecW)))) { status = rc; break; }
+        u64 *surv = (u64 *)W.wSurv.p;
+        MarkRecW *queue = (MarkRecW *)W.wQueue.p;
+
+        GF2T_HIP(hipEventRecord(ev[0], 0));
+        // log table: partition pass 1, in-place pass 2 per bucket, placement
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur1, (u64)256);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
+        hipLaunchKernelGGL(k_w_part1, dim3((unsigned)((M + W_E - 1) / W_E)), dim3(W_T), 0, 0, P,
+                           P1, cur1, cnt32 + 1);
+        GF2T_CHECK_LAUNCH("k_w_part1");
+        const int B2 = d - W_B1 - W_SB;
+        const u64 cap1 = (u64)1 << (d - W_B1);
+        for (unsigned b1 = 0; b1 < 256; b1++) {
+            const u64 count = b1 ? cap1 : cap1 - 1;
+            GF2T_HIP(hipMemcpyAsync(tmp, P1 + (u64)b1 * cap1, count * 8,
+                                    hipMemcpyDeviceToDevice, 0));
+            hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((1u << B2) + 255) / 256)), dim3(256), 0,
+                               0, cur2, (u64)(1u << B2));
+            hipLaunchKernelGGL(k_w_part2, dim3((unsigned)((count + W_E - 1) / W_E)), dim3(W_T), 0,
+                               0, B2, tmp, count, P1 + (u64)b1 * cap1, cur2, cnt32 + 1);
+        }
+        GF2T_CHECK_LAUNCH("k_w_part2");
+        hipLaunchKernelGGL(k_w_place, dim3((unsigned)((u64)1 << (d - W_SB))), dim3(1024),
+                           (size_t)place_lds, 0, P1, Llo, Lhi);
+        GF2T_CHECK_LAUNCH("k_w_place");
+        unsigned chk[2];
+        GF2T_HIP(hipMemcpy(chk, cnt32, 8, hipMemcpyDeviceToHost));
+        if (chk[1]) {
+            gf2t::set_error("gf2t_sieve: d=%d: log table partition overflow", d);
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
+        GF2T_HIP(hipEventRecord(ev[1], 0));
+        const u64 lyn_blocks = std::min<u64>((((u64)1 << (d - 2)) + 255) / 256, 1u << 20);
+        hipLaunchKernelGGL(k_lyndon_w, dim3((unsigned)lyn_blocks), dim3(256), 0, 0, d, M, surv,
+                           cnt64, (u64)(ncls + 64));
+        GF2T_CHECK_LAUNCH("k_lyndon_w");
+        unsigned long long hs[2];
+        GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
+        const u64 nsurv = hs[0];
+        if (nsurv != ncls) {
+            gf2t::set_error("gf2t_sieve: d=%d: %llu classes, expected %llu", d,
+                            (unsigned long long)nsurv, (unsigned long long)ncls);
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        const u64 zb = std::min<u64>((nsurv + 255) / 256, 1u << 18);
+        hipLaunchKernelGGL(k_zech_w, dim3((unsigned)zb), dim3(256), 0, 0, P, surv, nsurv, Llo, Lhi,
+                           queue, cnt64 + 1, qcap);
+        GF2T_CHECK_LAUNCH("k_zech_w");
+        GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
+        if (hs[1] > qcap) {
+            gf2t::set_error("gf2t_sieve: d=%d: marking queue overflow (%llu)", d,
+                            (unsigned long long)hs[1]);
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        const u64 nq = hs[1];
+        if (nq) {
+            hipLaunchKernelGGL(k_mark_w, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, 0, P,
+                               queue, nq, (unsigned long long *)dbest, heavy, cnt32, cnt32 + 1);
+            GF2T_CHECK_LAUNCH("k_mark_w");
+        }
+        GF2T_HIP(hipEventRecord(ev[2], 0));
+        unsigned h32[2];
+        GF2T_HIP(hipMemcpy(h32, cnt32, 8, hipMemcpyDeviceToHost));
+        if (h32[1]) {
+            gf2t::set_error("gf2t_sieve: d=%d: %s", d,
+                            (h32[1] & 1) ? "minimal polynomial not over GF(2)"
+                                         : "heavy queue overflow");
+            status = GF2T_EINTERNAL;
+            break;
+        }
+        const unsigned nheavy = h32[0];
+        if (nheavy) {
+            unsigned gy = nheavy < 65535 ? nheavy : 65535;
+            hipLaunchKernelGGL(k_mark_heavy, dim3(64, gy), dim3(256), 0, 0, heavy, nheavy, smax,
+                               (unsigned long long *)dbest);
+            GF2T_CHECK_LAUNCH("k_mark_heavy");
+        }
+        GF2T_HIP(hipEventRecord(ev[3], 0));
+        GF2T_HIP(hipEventSynchronize(ev[3]));
+        if (stats) {
+            gf2t_sieve_stats &S = stats[d];
+            S.d = d;
+            S.q = q;
+            S.classes = nsurv;
+            S.marking = nq;
+            S.heavy = nheavy;
+            GF2T_HIP(hipEventElapsedTime(&S.ms_table, ev[0], ev[1]));
+            GF2T_HIP(hipEventElapsedTime(&S.ms_classes, ev[1], ev[2]));
+            GF2T_HIP(hipEventElapsedTime(&S.ms_heavy, ev[2], ev[3]));
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return status;
+}
+}  // namespace
+
+extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
+                          gf2t_sieve_stats *stats) {
+    if (dmin < 2) dmin = 2;
+    if (dmax > WIDE_MAX_D || dmin > dmax || !dbest || smax < 1) {
+        gf2t::set_error("gf2t_sieve: need 2 <= dmin <= dmax <= %d, smax >= 1, dbest",
+                        WIDE_MAX_D);
+        return GF2T_EINVAL;
+    }
+    // device tables stay allocated between calls (grow-only; freed by
+    // gf2t_sieve_release), so repeated sieves pay no allocation
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    if (!g_sw) g_sw = new SieveWork();
+    int status = GF2T_OK;
+    if (dmin <= 32) status = sieve_narrow(r, dmin, std::min(dmax, 32), smax, dbest, stats);
+    if (status == GF2T_OK && dmax >= 33) {
+        // the narrow tables make room for the wide ones
+        for (DevBuf *b : {&g_sw->bL, &g_sw->bP1, &g_sw->bP2, &g_sw->bSurv, &g_sw->bQueue})
+            b->release();
+        status = sieve_wide(r, std::max(dmin, 33), dmax, smax, dbest, stats);
+    }
     return status;
 }

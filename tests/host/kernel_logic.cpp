@@ -38,6 +38,16 @@ u32 kl_gcd32(u32 a, u32 b) { return gcd32(a, b); }
 u32 kl_inv32(u32 a, u32 m) { return inv32(a, m); }
 u64 kl_mod_mersenne(u64 x, int d) { return mod_mersenne(x, d, ((u64)1 << d) - 1); }
 u64 kl_minpoly_bm(u32 beta, u64 q, int d, u32 tmask) { return minpoly_bm(beta, q, d, tmask); }
+u64 kl_fmul64(u64 a, u64 b, u64 q, int d) { return fmul64(a, b, q, d); }
+u64 kl_minpoly_bm64(u64 beta, u64 q, int d, u64 tmask) { return minpoly_bm64(beta, q, d, tmask); }
+u64 kl_mulmod64(u64 a, u64 b, u64 m) { return mulmod64(a, b, m); }
+u64 kl_inv64(u64 a, u64 m) { return inv64(a, m); }
+u64 kl_gcd64(u64 a, u64 b) { return gcd64(a, b); }
+u64 kl_count_classes64(int d) {
+    u64 M = ((u64)1 << d) - 1, n = 0;
+    for (u64 c = 0; c < ((u64)1 << (d - 2)); c++) n += is_lyndon64(2 * c + 1, d, M);
+    return n;
+}
 // the sieve's class enumeration: candidates j = 2c + 1, c < 2^(d-2)
 u64 kl_count_classes(int d) {
     u32 M = (u32)(((u64)1 << d) - 1);

@@ -24,7 +24,10 @@ def test_host_only_entry_points():
     assert lib.gf2t_version() >= 1
     for d in range(2, 21):
         assert lib.gf2t_primitive_poly(d) == O.find_primitive_poly(d)
-    assert lib.gf2t_primitive_poly(1) == 0 and lib.gf2t_primitive_poly(33) == 0
+    assert lib.gf2t_primitive_poly(1) == 0 and lib.gf2t_primitive_poly(41) == 0
+    for d in (33, 34):
+        q = lib.gf2t_primitive_poly(d)
+        assert q >> d == 1 and O.is_irreducible_small(q, d)
     assert lib.gf2t_mul_out_words(100, 200) == O.lib().gf2o_mul_out_words(100, 200)
     b = np.zeros(64, dtype=np.uint64)
     n = C.c_int()
@@ -41,7 +44,7 @@ def test_argument_errors_return_status_not_exit():
     assert lib.gf2t_square_fold(8, 16, 31, 0, None) == -1       # s = 0
     assert lib.gf2t_fold(8, 100, 16, 31, 3, None) == -1         # src_bits > 2r-1
     assert lib.gf2t_fft_plan_create(0, C.byref(C.c_void_p())) == -1
-    assert lib.gf2t_sieve(101, 2, 33, 10, 8, None) == -1
+    assert lib.gf2t_sieve(101, 2, 35, 10, 8, None) == -1
 
 
 def test_engine_basis_verify():

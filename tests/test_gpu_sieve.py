@@ -106,3 +106,33 @@ def test_partitioned_log_table_degrees_vs_oracle(eng, r, depth, smax):
     ob, _ = O.sieve(r, depth, smax, nthreads=8)
     assert np.array_equal(best, ob)
     assert all(s.classes == O.count_irreducible(s.d) for s in st)
+
+
+def test_wide_degrees_33_34_full_range_match_reference_log(eng):
+    # degrees 33 and 34 over every s <= 68139920 at r=136279841, continuing
+    # from the d<=32 image: the reference's runs (src/coarse_sieve.log:
+    # 67-101) leave 3,620,029 survivors at d<=33 and 3,515,197 at d<=34,
+    # with 111,820 and 104,832 new certificates; classes 260,300,986 and
+    # 505,286,415; no heavy progressions.  Sampled new marks are genuine
+    # irreducible factors of their trinomials.
+    smax = R136 // 2
+    base, _ = eng.run_gpu(R136, 32, smax)
+    best, st = eng.run_gpu(R136, 34, smax, best=base, min_depth=33)
+    deg = best[1:] >> np.uint64(48)
+    marked = best[1:] != O.NONE
+    assert int(np.count_nonzero(~(marked & (deg <= 33)))) == 3620029
+    assert int(np.count_nonzero(~marked)) == 3515197
+    assert int(np.count_nonzero(marked & (deg == 33))) == 111820
+    assert int(np.count_nonzero(marked & (deg == 34))) == 104832
+    got = {s.d: (s.classes, s.heavy) for s in st}
+    assert got == {33: (260300986, 0), 34: (505286415, 0)}
+    rng = np.random.default_rng(7)
+    for d in (33, 34):
+        idx = np.nonzero(marked & (deg == d))[0] + 1
+        for s in rng.choice(idx, size=40, replace=False):
+            k = int(best[s])
+            mask = k & 0xFFFFFFFFFFFF
+            assert O.is_irreducible_small(mask, d)
+            assert O.divides_trinomial_small(R136, int(s), mask, d)
+    # the d<=32 part is untouched
+    assert np.array_equal(best[base != O.NONE], base[base != O.NONE])

@@ -46,6 +46,14 @@ def kl(tmp_path_factory):
     lib.kl_count_classes.argtypes = [C.c_int]
     lib.kl_count_classes_all.restype = C.c_uint64
     lib.kl_count_classes_all.argtypes = [C.c_int]
+    for name, args in [("kl_fmul64", [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]),
+                       ("kl_minpoly_bm64", [C.c_uint64, C.c_uint64, C.c_int, C.c_uint64]),
+                       ("kl_mulmod64", [C.c_uint64, C.c_uint64, C.c_uint64]),
+                       ("kl_inv64", [C.c_uint64, C.c_uint64]),
+                       ("kl_gcd64", [C.c_uint64, C.c_uint64]),
+                       ("kl_count_classes64", [C.c_int])]:
+        getattr(lib, name).restype = C.c_uint64
+        getattr(lib, name).argtypes = args
     return lib
 
 
@@ -229,3 +237,69 @@ def test_class_enumeration_counts(kl):
         assert kl.kl_count_classes(d) == O.count_irreducible(d), d
     for d in range(2, 19):
         assert kl.kl_count_classes_all(d) == O.count_irreducible(d), d
+
+
+def _pmul(a, b):
+    r = 0
+    while b:
+        if b & 1:
+            r ^= a
+        a <<= 1
+        b >>= 1
+    return r
+
+
+def _pmod(a, q):
+    dq = q.bit_length() - 1
+    while a and a.bit_length() - 1 >= dq:
+        a ^= q << (a.bit_length() - 1 - dq)
+    return a
+
+
+def _irreducible(d, rng):
+    while True:
+        q = (1 << d) | int(rng.integers(0, 1 << 62)) % (1 << d) | 1
+        if O.is_irreducible_small(q, d):
+            return q
+
+
+def test_wide_field_arithmetic(kl):
+    # 32 < d <= 40: products against Python carry-less products, the
+    # Berlekamp-Massey minimal polynomial is irreducible of degree d and
+    # vanishes at beta, number theory against Python integers
+    rng = np.random.default_rng(33)
+    for d in (33, 34, 35, 36, 40):
+        q = _irreducible(d, rng)
+        tmask = 0
+        for i in range(d):
+            y, acc = 1 << i, 1 << i
+            for _ in range(1, d):
+                y = _pmod(_pmul(y, y), q)
+                acc ^= y
+            assert acc in (0, 1)
+            tmask |= acc << i
+        for _ in range(20):
+            a = int(rng.integers(0, 1 << 62)) % (1 << d)
+            b = int(rng.integers(0, 1 << 62)) % (1 << d)
+            assert kl.kl_fmul64(a, b, q, d) == _pmod(_pmul(a, b), q)
+        for _ in range(6):
+            beta = int(rng.integers(2, 1 << 62)) % (1 << d)
+            mask = kl.kl_minpoly_bm64(beta, q, d, tmask)
+            if mask == 0:          # beta in a proper subfield (rare)
+                continue
+            assert mask.bit_length() - 1 == d and O.is_irreducible_small(mask, d)
+            acc, pw = 0, 1
+            for k in range(d + 1):
+                if (mask >> k) & 1:
+                    acc ^= pw
+                pw = kl.kl_fmul64(pw, beta, q, d)
+            assert acc == 0
+    M = (1 << 34) - 1
+    for _ in range(200):
+        a, b = int(rng.integers(1, M)), int(rng.integers(1, M))
+        assert kl.kl_mulmod64(a, b, M) == a * b % M
+        assert kl.kl_gcd64(a, M) == np.gcd(a, M)
+        if np.gcd(a, M) == 1:
+            assert kl.kl_inv64(a, M) * a % M == 1
+    for d in range(3, 23):
+        assert kl.kl_count_classes64(d) == O.count_irreducible(d), d
