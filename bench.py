@@ -7,12 +7,14 @@ One step = one pass of the GPU hot path over one batch of synthetic input:
   (b) Frobenius chains F <- F^2 mod x^r + x^s + 1 for 256 sieve survivors
       (seeded draw; 17 MB residues resident in HBM, 4.36 GB in flight)
       advanced 512 squarings (one checkpoint interval) in the batched
-      form, starting from F = x.
+      form, starting from F = x; two squarings per launch
+      (k_square2_fold_batch: F is read once and F^4 written once).
 value = certified s-values / step wall time (sieve certificates only; the
 chain time is included in the step but certifies nothing by itself).
-roofline = the squaring+fold kernel (k_square_fold_batch): algorithmic
-bytes 16*nw per residue-step times 256 residues per launch, over its
-average launch time measured with HIP events on its stream.
+roofline = the squaring kernel of the chain: algorithmic bytes 16*nw per
+residue per launch (read the residue, write its square or fourth power)
+times 256 residues, over its average launch time measured with HIP
+events on its stream.
 """
 from __future__ import annotations
 
@@ -31,13 +33,16 @@ sys.path.insert(0, ROOT)
 R = 136279841
 SMAX = R // 2
 HBM_PEAK_GBS = 8000.0
-# HBM bytes one k_square_fold_batch launch moves for 256 residues at
-# r = 136279841 (default tuning), from rocprofv3 FETCH_SIZE and WRITE_SIZE
-# runs scaled by the factors a copy kernel of known size gives (x2.0 for
-# FETCH_SIZE, x1.0 for WRITE_SIZE): 6,736,592,640 B fetched + 4,360,962,104 B
-# written.  The command lines and raw counts are kept in
-# profiles/k_square_fold_batch_r136279841_256res.md.
-SQ_TRAFFIC_256 = 6736592640 + 4360962104
+# HBM bytes one launch of each squaring kernel moves for 256 residues at
+# r = 136279841, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by
+# the factors a copy kernel of known size gives (x2.0 for FETCH_SIZE, x1.0
+# for WRITE_SIZE).  The command lines and raw counts are kept in
+# profiles/k_square_fold_batch_r136279841_256res.md and
+# profiles/k_square2_fold_batch_r136279841_256res.md.
+SQ_TRAFFIC_256 = {
+    "k_square_fold_batch": 6736592640 + 4360962104,    # one squaring per launch
+    "k_square2_fold_batch": 10870893363 + 4388719821,  # two squarings per launch
+}
 
 
 def parse():
@@ -54,6 +59,8 @@ def parse():
     ap.add_argument("--sq-ppt", type=int, default=0, help="override the squaring pairs/lane")
     ap.add_argument("--sq-order", type=int, default=-1, help="override the squaring tile order")
     ap.add_argument("--sq-nt", type=int, default=-1, help="override streaming stores (0/1)")
+    ap.add_argument("--sq-fuse", type=int, default=-1,
+                    help="override two squarings per launch (0/1; default on)")
     ap.add_argument("--scan-r", type=int, default=R, help="scan mode: trinomial degree r")
     ap.add_argument("--scan-s", type=str, default="474",
                     help="scan mode: comma-separated s values")
@@ -70,12 +77,15 @@ def parse():
                     help="serve mode: longest s slice (the mix draws 10^3..10^6, capped here)")
     ap.add_argument("--serve-interval", type=int, default=2048,
                     help="serve mode: deep-scan interval length (gcds per s = 2048 / this)")
-    ap.add_argument("--mode", choices=["full", "squaring", "fft", "scan", "pipeline", "serve"],
+    ap.add_argument("--mode",
+                    choices=["full", "squaring", "fft", "scan", "pipeline", "serve", "sieve"],
                     default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
                          "fft: the Cantor-product batch (64 seeded operand pairs of 2^16..2^27 "
-                         "bits, each product folded mod x^r + x^s + 1)")
+                         "bits, each product folded mod x^r + x^s + 1); sieve: --steps full-range "
+                         "coarse sieves at r = 136279841 to --depth (up to 34), per-degree "
+                         "device times")
     return ap.parse_args()
 
 
@@ -148,10 +158,14 @@ def mode_squaring(args):
                                      kk.ctypes.data_as(P64), n, None))
     e1.record()
     ms = e0.elapsed_ms(e1)
-    launch_ms = ms / args.chain_steps
+    kernel, launches, per = chain_launches(args.chain_steps, args.sq_fuse != 0)
+    launch_ms = ms / (args.chain_steps / per)
     gbs = 16 * nw * n / (launch_ms * 1e-3) / 1e9
     print(json.dumps({"mode": "squaring", "residues": n, "steps": args.chain_steps,
                       "sq_ppt": args.sq_ppt, "sq_order": args.sq_order, "sq_nt": args.sq_nt,
+                      "sq_fuse": args.sq_fuse, "kernel": kernel, "launches": launches,
+                      "squarings_per_launch": per, "chain_ms": ms,
+                      "residue_steps_per_s": n * args.chain_steps / (ms * 1e-3),
                       "avg_launch_ms": launch_ms, "GB_per_s": gbs, "frac": gbs / HBM_PEAK_GBS,
                       "copy_bytes_moved": 2 * nbytes16, "copy_ms": copy_ms,
                       "copy_GB_per_s": 2 * nbytes16 / (copy_ms * 1e-3) / 1e9,
@@ -228,6 +242,46 @@ def mode_scan(args):
                           "maxd": args.scan_maxd, "block_m": bm, "n_s": len(svals),
                           "seconds": dt, "s_per_second": len(svals) / dt,
                           "lines": [o.line() for o in out][:16]}), flush=True)
+
+
+def mode_sieve(args):
+    """The coarse sieve alone: --warmup + --steps runs over every s <= (r-1)/2
+    to --depth, each reported with its wall time and the device time of
+    the top degrees (log table / classes+marking).  Depths 33..34 build the
+    wide tables (about 201 GB of HBM at 34)."""
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import SieveStats, check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+
+    device.init(0)
+    dbest = DeviceBuffer.words(SMAX + 1)
+    stats = (SieveStats * 35)()
+    walls = []
+    for i in range(args.warmup + args.steps):
+        dbest.fill(0xFF)
+        device.synchronize()
+        t0 = time.perf_counter()
+        check(lib.gf2t_sieve(R, 2, args.depth, SMAX, dbest.ptr, stats), "gf2t_sieve")
+        device.synchronize()
+        dt = time.perf_counter() - t0
+        top = [{"d": int(stats[d].d), "ms_table": round(stats[d].ms_table, 1),
+                "ms_classes": round(stats[d].ms_classes + stats[d].ms_heavy, 1),
+                "classes": int(stats[d].classes), "marking": int(stats[d].marking)}
+               for d in range(max(2, args.depth - 3), args.depth + 1)]
+        timed = i >= args.warmup
+        if timed:
+            walls.append(dt)
+        print(f"sieve run {i} ({'timed' if timed else 'warmup'}) d=2..{args.depth} "
+              f"wall {dt:.3f}s {top}", file=sys.stderr, flush=True)
+    best = dbest.download(SMAX + 1)
+    certified = int(np.count_nonzero(best[1:] != np.uint64(0xFFFFFFFFFFFFFFFF)))
+    ms_deg = {int(stats[d].d): round(stats[d].ms_table + stats[d].ms_classes + stats[d].ms_heavy, 2)
+              for d in range(2, args.depth + 1)}
+    print(json.dumps({"mode": "sieve", "r": R, "smax": SMAX, "depth": args.depth,
+                      "certified": certified, "survivors": SMAX - certified,
+                      "wall_s": [round(w, 4) for w in walls],
+                      "s_certified_per_s": certified / float(np.mean(walls)) if walls else None,
+                      "ms_per_degree": ms_deg}), flush=True)
 
 
 def mode_pipeline(args):
@@ -354,6 +408,17 @@ def apply_tuning(args):
         check(lib.gf2t_tune(b"sq_order", args.sq_order))
     if args.sq_nt >= 0:
         check(lib.gf2t_tune(b"sq_nt", args.sq_nt))
+    if args.sq_fuse >= 0:
+        check(lib.gf2t_tune(b"sq_fuse", args.sq_fuse))
+
+
+def chain_launches(steps: int, fuse: bool):
+    """(kernel, launches) of a chain of `steps` squarings: with fusion,
+    steps // 2 two-squaring launches (plus one single launch for an odd
+    count, which the roofline line does not cover)."""
+    if fuse and steps >= 2:
+        return "k_square2_fold_batch", steps // 2, 2
+    return "k_square_fold_batch", steps, 1
 
 
 def main():
@@ -364,7 +429,8 @@ def main():
               "with one elementwise-min exchange of best[])", file=sys.stderr)
         sys.exit(2)
     apply_tuning(args)
-    default_tuning = not (args.sq_ppt or args.sq_order >= 0 or args.sq_nt >= 0)
+    default_tuning = not (args.sq_ppt or args.sq_order >= 0 or args.sq_nt >= 0
+                          or args.sq_fuse >= 0)
     if args.mode == "squaring":
         return mode_squaring(args)
     if args.mode == "fft":
@@ -375,6 +441,8 @@ def main():
         return mode_pipeline(args)
     if args.mode == "serve":
         return mode_serve(args)
+    if args.mode == "sieve":
+        return mode_sieve(args)
 
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib
@@ -437,7 +505,8 @@ def main():
     per_step_ms = 1e3 * wall / args.steps
     value = certified / (wall / args.steps)
 
-    launch_ms = float(np.mean(chain_ms)) / args.chain_steps
+    kernel, launches, per = chain_launches(args.chain_steps, args.sq_fuse != 0)
+    launch_ms = float(np.mean(chain_ms)) / (args.chain_steps / per)
     bytes_per_launch = 16 * nw * n
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     deg = [{"d": stats[d].d, "ms": round(stats[d].ms_table + stats[d].ms_classes + stats[d].ms_heavy, 2),
@@ -459,19 +528,21 @@ def main():
         "config": {
             "workload": f"r={R}: coarse sieve d=2..{args.depth} over all s in 1..{SMAX} + "
                         f"{args.chain_steps} batched squaring+fold steps on {n} sieve survivors "
-                        f"(17 MB residues)",
+                        f"(17 MB residues, {per} squarings per launch)",
             "r": R, "smax": SMAX, "depth": args.depth, "residues": n,
             "chain_steps": args.chain_steps, "seed": args.seed,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_square_fold_batch",
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": SQ_TRAFFIC_256 if (n == 256 and default_tuning) else None,
+            "traffic": SQ_TRAFFIC_256.get(kernel) if (n == 256 and default_tuning) else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "squarings_per_launch": per,
+            "launches": launches,
             "avg_launch_ms": launch_ms,
         },
         "sieve": {"ms": float(np.mean(sieve_ms)), "certified": certified,

@@ -96,15 +96,20 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
 
 /* Tuning knobs of the kernels (the autotuner's handle; the reference
  * reads GF2C_* environment variables, src/gf2_cantor_cuda.h:24-61):
- *   "sq_ppt"   output pairs per lane in the batched squaring (1, 2, 4)
- *   "sq_order" 1 = XCD-aware tile order in the batched squaring, 0 = linear
+ *   "sq_fuse"  1 = two squarings per launch in gf2t_square_fold_steps and
+ *              gf2t_square_fold_batch while every live chain has two
+ *              steps left; 0 = one squaring per launch
+ *   "sq_ppt"   output pairs per lane in the one-step batched squaring
+ *              (1, 2, 4)
+ *   "sq_order" 1 = XCD-aware tile order in the one-step batched squaring,
+ *              0 = linear
  *   "sq_nt"    1 = non-temporal (streaming) stores in the batched squaring
  *              and in gf2t_stream_copy
  *   "gcd_dev_bits" products inside the host gcds (gf2t_gcd_host, the
  *              deep scan) go to the device when both operands have at
  *              least this many bits; 0 = host Karatsuba only
- * Environment defaults: GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0), GF2T_SQ_NT (1),
- * GF2T_GCD_DEV_BITS (2^18, set by gf2t_init; 0 before it). */
+ * Environment defaults: GF2T_SQ_FUSE (1), GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0),
+ * GF2T_SQ_NT (1), GF2T_GCD_DEV_BITS (2^18, set by gf2t_init; 0 before it). */
 int gf2t_tune(const char *key, int64_t value);
 
 /* Slab form: residues i = 0..n-1 at din + i*pitch (pitch >= nw words),

@@ -8,8 +8,17 @@
 
 #if defined(__HIP__) || defined(__HIPCC__)
 #define GF2T_HD __host__ __device__ __forceinline__
+#define GF2T_HDM __host__ __device__ __forceinline__
 #else
 #define GF2T_HD static inline
+#define GF2T_HDM inline   // member functions
+#endif
+// global-memory pointers in device code (direct global loads with a
+// scalar base and a 32-bit offset instead of flat loads)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GF2T_GLOBAL __attribute__((address_space(1)))
+#else
+#define GF2T_GLOBAL
 #endif
 
 #ifndef GF2T_U_TYPES
@@ -179,13 +188,118 @@ GF2T_HD u64 minpoly_bm64(u64 beta, u64 q, int d, u64 tmask) {
 }
 
 // ------------------------------------------- squaring + trinomial fold
-// 128-bit window of the virtual spread SP of `in` (SP[2k] = bit k of in,
-// odd positions 0) at SP position P, keeping only SP indices in [lo, hi).
-GF2T_HD void vspread128(const u64 *in, size_t nw, u64 P, u64 lo, u64 hi, u64 &w0,
-                        u64 &w1) {
+// Byte permute: byte k of the result is byte (sel_k & 7) of the 8-byte
+// value hi:lo (v_perm_b32 with selectors 0..7, the only ones used here).
+GF2T_HD u32 perm8(u32 hi, u32 lo, u32 sel) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+    const u64 v = ((u64)hi << 32) | lo;
+    u32 out = 0;
+    for (int k = 0; k < 4; k++)
+        out |= (u32)((v >> (8 * ((sel >> (8 * k)) & 7))) & 0xFF) << (8 * k);
+    return out;
+#endif
+}
+// low 32 bits of (hi:lo) >> (sh & 31) (v_alignbit_b32)
+GF2T_HD u32 alignbit(u32 hi, u32 lo, u32 sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+#else
+    return (u32)((((u64)hi << 32) | lo) >> (sh & 31));
+#endif
+}
+// (a >> sh) | (b << (64 - sh)) for 0 <= sh < 64 (sh = 0 gives a)
+GF2T_HD u64 funnel64(u64 a, u64 b, u32 sh) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u32 lo, hi;
+    if (sh < 32) { lo = alignbit(a1, a0, sh); hi = alignbit(b0, a1, sh); }
+    else { lo = alignbit(b0, a1, sh); hi = alignbit(b1, b0, sh); }
+    return ((u64)hi << 32) | lo;
+}
+// Spread of 32 bits into 64 (bit k -> bit 2k) by table lookup: the low
+// three bits of each nibble select its spread byte from the eight-entry
+// table {00 01 04 05 10 11 14 15} with one byte permute (four nibbles at
+// a time), the nibble's top bit lands at bit 6 of that byte, and two
+// more permutes interleave the bytes of the even and the odd nibbles.
+GF2T_HD u64 spread32p(u32 x) {
+    const u32 T_HI = 0x15141110u, T_LO = 0x05040100u;
+    const u32 e = perm8(T_HI, T_LO, x & 0x07070707u) | ((x & 0x08080808u) << 3);
+    const u32 o = perm8(T_HI, T_LO, (x >> 4) & 0x07070707u) | ((x >> 1) & 0x40404040u);
+    const u32 lo = perm8(o, e, 0x05010400u);   // e0 o0 e1 o1
+    const u32 hi = perm8(o, e, 0x07030602u);   // e2 o2 e3 o3
+    return ((u64)hi << 32) | lo;
+}
+GF2T_HD void spread64(u64 v, u64 &lo, u64 &hi) {
+    lo = spread32p((u32)v);
+    hi = spread32p((u32)(v >> 32));
+}
+// Bit interleave: bit k of e -> bit 2k, bit k of o -> bit 2k + 1, i.e.
+// spread(e) + x * spread(o).  The same nibble lookups as spread32p, with
+// a second table (the spreads shifted left by one) for o, so each byte
+// of the result is one lookup of e or'ed with one of o.
+GF2T_HD u64 interleave32(u32 e, u32 o) {
+    const u32 TE_HI = 0x15141110u, TE_LO = 0x05040100u;
+    const u32 TO_HI = 0x2A282220u, TO_LO = 0x0A080200u;
+    const u32 lo_n = perm8(TE_HI, TE_LO, e & 0x07070707u) | ((e & 0x08080808u) << 3) |
+                     perm8(TO_HI, TO_LO, o & 0x07070707u) | ((o & 0x08080808u) << 4);
+    const u32 hi_n = perm8(TE_HI, TE_LO, (e >> 4) & 0x07070707u) | ((e >> 1) & 0x40404040u) |
+                     perm8(TO_HI, TO_LO, (o >> 4) & 0x07070707u) | (o & 0x80808080u);
+    const u32 lo = perm8(hi_n, lo_n, 0x05010400u);
+    const u32 hi = perm8(hi_n, lo_n, 0x07030602u);
+    return ((u64)hi << 32) | lo;
+}
+
+// Word sources of the squaring.  src(win, k) is word k of the input,
+// zero at and beyond nw; src.raw32(win, q) reads 32-bit word q unchecked,
+// for the region fast path whose reads all lie inside the nw words (see
+// below).
+// `win` names the fold window a read serves (0 = SP[0,r), 1..4 = the
+// four shifted windows): a plain array ignores it, the two-squaring
+// kernel's tile keeps one LDS copy per window.
+struct SqArray {
+    const u64 *p;
+    size_t nw;
+    GF2T_HDM u64 operator()(int, size_t k) const { return k < nw ? p[k] : 0; }
+    // 32-bit words q, q+1 (and q+2) of the input, unchecked, as one
+    // aligned 8-byte load plus one 4-byte load; qodd = q & 1, uniform
+    // over the caller's lanes.  32-bit byte offsets: residues are below
+    // 2^29 words.
+    GF2T_HDM u32 ld32(u32 q) const {
+        return *(const GF2T_GLOBAL u32 *)((const GF2T_GLOBAL char *)p + (u32)(q << 2));
+    }
+    GF2T_HDM u64 ld64(u32 q) const {   // q even
+        return *(const GF2T_GLOBAL u64 *)((const GF2T_GLOBAL char *)p + (u32)(q << 2));
+    }
+    GF2T_HDM void raw3(int, u32 q, u32 qodd, u32 &a, u32 &b, u32 &c) const {
+        if (qodd) {
+            a = ld32(q);
+            const u64 w = ld64(q + 1);
+            b = (u32)w; c = (u32)(w >> 32);
+        } else {
+            const u64 w = ld64(q);
+            a = (u32)w; b = (u32)(w >> 32);
+            c = ld32(q + 2);
+        }
+    }
+    GF2T_HDM void raw2(int, u32 q, u32 qodd, u32 &a, u32 &b) const {
+        if (qodd) { a = ld32(q); b = ld32(q + 1); }
+        else { const u64 w = ld64(q); a = (u32)w; b = (u32)(w >> 32); }
+    }
+};
+
+// 128-bit window of the virtual spread SP of the source at SP position
+// P, keeping only SP indices in [lo, hi).
+template <class Src>
+GF2T_HD void vspread128(const Src &src, int win, u64 P, u64 lo, u64 hi, u64 &w0, u64 &w1) {
     if (P >= hi || P + 128 <= lo) { w0 = 0; w1 = 0; return; }
-    u64 v = win64(in, nw, (P + 1) >> 1);
-    u64 a = spread32((u32)v), b = spread32((u32)(v >> 32));
+    const u64 q = (P + 1) >> 1;
+    const size_t w = (size_t)(q >> 6);
+    const u32 sh = (u32)(q & 63);
+    u64 v = src(win, w);
+    if (sh) v = funnel64(v, src(win, w + 1), sh);
+    u64 a, b;
+    spread64(v, a, b);
     if (P & 1) { a <<= 1; b <<= 1; }
     int lc = lo > P ? (int)(lo - P) : 0;
     int hc = (hi - P) >= 128 ? 128 : (int)(hi - P);
@@ -196,35 +310,23 @@ GF2T_HD void vspread128(const u64 *in, size_t nw, u64 P, u64 lo, u64 hi, u64 &w0
 // output words 2i, 2i+1 of in^2 mod x^r + x^s + 1:
 //   SP[0,r) + SP[r,2r-1)>>r + SP[r,2r-s)>>(r-s)
 //   + SP[2r-s,2r-1)>>(2r-s) + SP[2r-s,2r-1)>>(2r-2s)
-GF2T_HD void square_fold_pair(const u64 *in, size_t nw, u64 r, u64 s, size_t i, u64 &o0,
-                              u64 &o1) {
+template <class Src>
+GF2T_HD void square_fold_pair_src(const Src &src, u64 r, u64 s, size_t i, u64 &o0, u64 &o1) {
     const u64 b = (u64)i << 7;
     u64 t0, t1;
-    vspread128(in, nw, b, 0, r, o0, o1);
-    vspread128(in, nw, b + r, r, 2 * r - 1, t0, t1);
+    vspread128(src, 0, b, 0, r, o0, o1);
+    vspread128(src, 1, b + r, r, 2 * r - 1, t0, t1);
     o0 ^= t0; o1 ^= t1;
-    vspread128(in, nw, b + r - s, r, 2 * r - s, t0, t1);
+    vspread128(src, 2, b + r - s, r, 2 * r - s, t0, t1);
     o0 ^= t0; o1 ^= t1;
-    vspread128(in, nw, b + 2 * r - s, 2 * r - s, 2 * r - 1, t0, t1);
+    vspread128(src, 3, b + 2 * r - s, 2 * r - s, 2 * r - 1, t0, t1);
     o0 ^= t0; o1 ^= t1;
-    vspread128(in, nw, b + 2 * r - 2 * s, 2 * r - s, 2 * r - 1, t0, t1);
+    vspread128(src, 4, b + 2 * r - 2 * s, 2 * r - s, 2 * r - 1, t0, t1);
     o0 ^= t0; o1 ^= t1;
 }
-
-// ---- fast path: unmasked windows, 32-bit spreads
-// spread of the low 16 bits of x into 32 bits (bit k -> bit 2k)
-GF2T_HD u32 spread16(u32 x) {
-    x &= 0xFFFFu;
-    x = (x | (x << 8)) & 0x00FF00FFu;
-    x = (x | (x << 4)) & 0x0F0F0F0Fu;
-    x = (x | (x << 2)) & 0x33333333u;
-    x = (x | (x << 1)) & 0x55555555u;
-    return x;
-}
-GF2T_HD void spread64(u64 v, u64 &lo, u64 &hi) {
-    u32 a = (u32)v, b = (u32)(v >> 32);
-    lo = (u64)spread16(a) | ((u64)spread16(a >> 16) << 32);
-    hi = (u64)spread16(b) | ((u64)spread16(b >> 16) << 32);
+GF2T_HD void square_fold_pair(const u64 *in, size_t nw, u64 r, u64 s, size_t i, u64 &o0,
+                              u64 &o1) {
+    square_fold_pair_src(SqArray{in, nw}, r, s, i, o0, o1);
 }
 
 // Window k of the squaring reads the 64 input bits starting at input bit
@@ -232,8 +334,10 @@ GF2T_HD void spread64(u64 v, u64 &lo, u64 &hi) {
 // shifts the spread left by one.  Q_k, and so the word offset, the bit
 // shift and the parity, are the same for every pair of one residue.
 struct SqWin {
-    u64 off;
-    u32 sh, odd;
+    u64 off;      // 64-bit word offset of Q_k (tile planning)
+    u32 off32;    // 32-bit word offset of Q_k
+    u32 sh;       // Q_k mod 32
+    u32 odd;
 };
 struct SqConst {
     u64 r, s;
@@ -244,7 +348,8 @@ GF2T_HD SqWin sq_win(u64 P0) {   // P0 = window offset in spread positions (b = 
     u64 Q = (P0 + 1) >> 1;
     SqWin w;
     w.off = Q >> 6;
-    w.sh = (u32)(Q & 63);
+    w.off32 = (u32)(Q >> 5);
+    w.sh = (u32)(Q & 31);
     w.odd = (u32)(P0 & 1);
     return w;
 }
@@ -257,52 +362,205 @@ GF2T_HD SqConst sq_const(u64 r, u64 s) {
     c.w5 = sq_win(2 * r - 2 * s);
     return c;
 }
-GF2T_HD void win_spread(const u64 *in, size_t nw, size_t i, const SqWin &w, u64 &lo, u64 &hi) {
-    size_t k = i + (size_t)w.off;
-    u64 v = k < nw ? in[k] : 0;
-    if (w.sh) {
-        u64 h = k + 1 < nw ? in[k + 1] : 0;
-        v = (v >> w.sh) | (h << (64 - w.sh));
-    }
-    spread64(v, lo, hi);
-    if (w.odd) { lo <<= 1; hi <<= 1; }   // spread bits are even: no carry between words
-}
-
 // Output pair i of in^2 mod T.  Away from the points s, 2s and r every
 // window is either fully inside or fully outside its clip range:
 //   [0, s)  : SP[0,r) + SP[r,..)>>r + SP[2r-s,..)>>(2r-s)          (W1 W2 W4)
 //   [s, 2s) : W1 + W2 + SP[r,2r-s)>>(r-s) + SP[2r-s,..)>>(2r-2s)   (W1 W2 W3 W5)
 //   [2s, r) : W1 + W2 + W3
 // so those pairs skip all masking; pairs straddling s, 2s or r take the
-// clipped general path.
-GF2T_HD void square_fold_pair_fast(const u64 *in, const SqConst &C, size_t i, u64 &o0, u64 &o1) {
-    const u64 b = (u64)i << 7;
-    const u64 s = C.s, r = C.r;
-    int region;
-    if (b + 128 <= s) region = 0;
-    else if (b >= s && b + 128 <= 2 * s) region = 1;
-    else if (b >= 2 * s && b + 128 <= r) region = 2;
-    else region = 3;
-    if (region == 3) {
-        square_fold_pair(in, C.nw, r, s, i, o0, o1);
-        return;
+// clipped general path.  In these three regions every window's last
+// input bit is at most r - 1 (e.g. W2 in [2s, r): 64i + (r+1)/2 + 63
+// with 128i + 128 <= r), so each read, the funnel shift's second word
+// included, lies inside the nw input words.
+GF2T_HD int sq_region(u64 b, u64 b1, u64 s, u64 r) {   // bit range [b, b1)
+    if (b1 <= s) return 0;
+    if (b >= s && b1 <= 2 * s) return 1;
+    if (b >= 2 * s && b1 <= r) return 2;
+    return 3;
+}
+// The 64 input bits from 32-bit word q on, shifted right by sh (< 32):
+// three words and two funnel shifts (two words when sh = 0, so no read
+// goes past the window's last bit).  qodd = q & 1 is the same for every
+// pair of a window.
+template <class Src>
+GF2T_HD u64 sq_winval(const Src &src, int win, u32 q, u32 qodd, u32 sh) {
+    u32 a, b, c;
+    if (!sh) {
+        src.raw2(win, q, qodd, a, b);
+        return ((u64)b << 32) | a;
     }
-    u64 t0, t1;
-    u64 x = i < C.nw ? in[i] : 0;
-    spread64(x, o0, o1);
-    win_spread(in, C.nw, i, C.w2, t0, t1);
-    o0 ^= t0; o1 ^= t1;
-    if (region == 0) {
-        win_spread(in, C.nw, i, C.w4, t0, t1);
-        o0 ^= t0; o1 ^= t1;
+    src.raw3(win, q, qodd, a, b, c);
+    return ((u64)alignbit(c, b, sh) << 32) | alignbit(b, a, sh);
+}
+// Output pair i in region REG (0..2).  Spreading is linear and a window
+// at an odd spread position is the spread shifted by one, so the window
+// values are summed by parity first (E: even windows, O: odd) and the
+// pair is the interleave of E and O.  The parities follow from those of
+// r and s (W2: r, W3: r - s, W4: s, W5: even), so they are template
+// arguments and the sums cost one xor per window.
+template <int REG, int RP, int SP, class Src>
+GF2T_HD void sq_pair_t(const Src &src, const SqConst &C, u32 i, u64 &o0, u64 &o1) {
+    u64 E = sq_winval(src, 0, 2 * i, 0, 0), O = 0;
+    const u64 v2 = sq_winval(src, 1, 2 * i + C.w2.off32, C.w2.off32 & 1, C.w2.sh);
+    if (RP) O ^= v2; else E ^= v2;
+    if (REG == 0) {
+        const u64 v4 = sq_winval(src, 3, 2 * i + C.w4.off32, C.w4.off32 & 1, C.w4.sh);
+        if (SP) O ^= v4; else E ^= v4;
     } else {
-        win_spread(in, C.nw, i, C.w3, t0, t1);
-        o0 ^= t0; o1 ^= t1;
-        if (region == 1) {
-            win_spread(in, C.nw, i, C.w5, t0, t1);
-            o0 ^= t0; o1 ^= t1;
+        const u64 v3 = sq_winval(src, 2, 2 * i + C.w3.off32, C.w3.off32 & 1, C.w3.sh);
+        if (RP ^ SP) O ^= v3; else E ^= v3;
+        if (REG == 1) E ^= sq_winval(src, 4, 2 * i + C.w5.off32, C.w5.off32 & 1, C.w5.sh);
+    }
+    o0 = interleave32((u32)E, (u32)O);
+    o1 = interleave32((u32)(E >> 32), (u32)(O >> 32));
+}
+// The twelve (region 0..2) x (r, s parity) variants: sq_variant calls
+// f(SqV<REG, RP, SP>{}) for the one that applies, so a caller picks it
+// once for a whole tile or window (the arguments are uniform there).
+template <int REG, int RP, int SP>
+struct SqV {
+    template <class Src>
+    GF2T_HDM void pair(const Src &src, const SqConst &C, u32 i, u64 &o0, u64 &o1) const {
+        sq_pair_t<REG, RP, SP>(src, C, i, o0, o1);
+    }
+};
+template <class F>
+GF2T_HD void sq_variant(int region, const SqConst &C, F &&f) {
+    switch (region * 4 + (int)((C.r & 1) << 1) + (int)(C.s & 1)) {
+    case 0: f(SqV<0, 0, 0>{}); break;
+    case 1: f(SqV<0, 0, 1>{}); break;
+    case 2: f(SqV<0, 1, 0>{}); break;
+    case 3: f(SqV<0, 1, 1>{}); break;
+    case 4: f(SqV<1, 0, 0>{}); break;
+    case 5: f(SqV<1, 0, 1>{}); break;
+    case 6: f(SqV<1, 1, 0>{}); break;
+    case 7: f(SqV<1, 1, 1>{}); break;
+    case 8: f(SqV<2, 0, 0>{}); break;
+    case 9: f(SqV<2, 0, 1>{}); break;
+    case 10: f(SqV<2, 1, 0>{}); break;
+    default: f(SqV<2, 1, 1>{}); break;
+    }
+}
+// The same pair in two stages, for a caller that issues the loads of
+// several pairs before combining any: the raw window values of pair i
+// (region reg 0..2, uniform), then their sums by parity and interleave.
+struct SqRawPair {
+    u64 x, v2, vb, v5;   // W1, W2, W4 (region 0) or W3, W5 (region 1, else 0)
+};
+template <class Src>
+GF2T_HD SqRawPair sq_raw_pair(const Src &src, const SqConst &C, u32 i, int reg) {
+    SqRawPair w;
+    w.x = sq_winval(src, 0, 2 * i, 0, 0);
+    w.v2 = sq_winval(src, 1, 2 * i + C.w2.off32, C.w2.off32 & 1, C.w2.sh);
+    const u32 boff = reg == 0 ? C.w4.off32 : C.w3.off32, bsh = reg == 0 ? C.w4.sh : C.w3.sh;
+    w.vb = sq_winval(src, reg == 0 ? 3 : 2, 2 * i + boff, boff & 1, bsh);
+    w.v5 = 0;
+    if (reg == 1) w.v5 = sq_winval(src, 4, 2 * i + C.w5.off32, C.w5.off32 & 1, C.w5.sh);
+    return w;
+}
+GF2T_HD void sq_combine_pair(const SqRawPair &w, const SqConst &C, int reg, u64 &o0, u64 &o1) {
+    const u64 m2 = (C.r & 1) ? ~0ULL : 0;
+    const u64 mb = ((reg == 0 ? C.s : C.r ^ C.s) & 1) ? ~0ULL : 0;
+    const u64 O = (w.v2 & m2) ^ (w.vb & mb);
+    const u64 E = w.x ^ w.v2 ^ w.vb ^ w.v5 ^ O;
+    o0 = interleave32((u32)E, (u32)O);
+    o1 = interleave32((u32)(E >> 32), (u32)(O >> 32));
+}
+// one pair in region 0..2
+template <class Src>
+GF2T_HD void sq_pair_region(const Src &src, const SqConst &C, u32 i, int region, u64 &o0,
+                            u64 &o1) {
+    sq_variant(region, C, [&](auto v) { v.pair(src, C, i, o0, o1); });
+}
+template <class Src>
+GF2T_HD void square_fold_pair_fast_src(const Src &src, const SqConst &C, size_t i, u64 &o0,
+                                       u64 &o1) {
+    const u64 b = (u64)i << 7;
+    const int region = sq_region(b, b + 128, C.s, C.r);
+    if (region == 3) square_fold_pair_src(src, C.r, C.s, i, o0, o1);
+    else sq_pair_region(src, C, (u32)i, region, o0, o1);
+}
+GF2T_HD void square_fold_pair_fast(const u64 *in, const SqConst &C, size_t i, u64 &o0, u64 &o1) {
+    square_fold_pair_fast_src(SqArray{in, C.nw}, C, i, o0, o1);
+}
+// region shared by the pairs [i0, i1) (3: mixed or clipped)
+GF2T_HD int sq_tile_region(const SqConst &C, size_t i0, size_t i1) {
+    return sq_region((u64)i0 << 7, (u64)i1 << 7, C.s, C.r);
+}
+
+// ---- two squarings per pass: F -> G = F^2 -> H = G^2 (mod T)
+// A tile of output pairs [i0, i0 + SQ2_TP) of H reads G only through
+// the windows of its pairs: window k covers G words [i0 + off_k,
+// i0 + off_k + SQ2_TP] (the last one for the funnel shift).  With
+// SQ2_TP = 510 those words lie inside the 256 G pairs from
+// (i0 + off_k) >> 1 on, so each of the 256 threads computes one G pair
+// per window, straight from F, into that window's copy (512 words).
+// Tiles inside one region need three or four of the five windows; a
+// tile that straddles s, 2s or r takes all five.  The tile's H pairs are
+// then formed from those copies: F is read once per two squarings.
+constexpr int SQ2_WINS = 5;
+constexpr u32 SQ2_TP = 510;
+constexpr u32 SQ2_GP = 256;          // G pairs per window
+constexpr u32 SQ2_LEN = 2 * SQ2_GP;  // G words per window copy
+GF2T_HD u64 sq2_off(const SqConst &C, int k) {   // word offset of window k
+    return k == 0 ? 0 : k == 1 ? C.w2.off : k == 2 ? C.w3.off : k == 3 ? C.w4.off : C.w5.off;
+}
+GF2T_HD u32 sq2_need(int region) {   // windows read by a tile of this region
+    return region == 0 ? 0x0Bu : region == 1 ? 0x17u : region == 2 ? 0x07u : 0x1Fu;
+}
+// first G pair of window k for the tile at i0
+GF2T_HD u64 sq2_p0(const SqConst &C, size_t i0, int k) { return ((u64)i0 + sq2_off(C, k)) >> 1; }
+// G pairs p0 + j (j < SQ2_GP) of one window: words 2(p0 + j) and
+// 2(p0 + j) + 1 of G, zero at and beyond nw.  greg = the region of the
+// window's G pairs, sq_tile_region(C, p0, p0 + SQ2_GP).
+GF2T_HD void sq2_g_pair_clipped(const u64 *F, const SqConst &C, u64 p, u64 &o0, u64 &o1) {
+    o0 = o1 = 0;
+    if (2 * p < C.nw) square_fold_pair_fast(F, C, (size_t)p, o0, o1);
+    if (2 * p + 1 >= C.nw) o1 = 0;
+}
+GF2T_HD void sq2_g_pair(const u64 *F, const SqConst &C, u64 p0, u32 j, u64 &o0, u64 &o1) {
+    const int greg = sq_tile_region(C, p0, p0 + SQ2_GP);
+    if (greg < 3) sq_pair_region(SqArray{F, C.nw}, C, (u32)(p0 + j), greg, o0, o1);
+    else sq2_g_pair_clipped(F, C, p0 + j, o0, o1);
+}
+// the H phase reads G through the tile's window copies: window k's copy
+// holds G words from base_k = 2 p0_k on
+struct Sq2Tile {
+    const u64 *win;
+    u64 base[SQ2_WINS];
+    GF2T_HDM u64 operator()(int k, size_t w) const {
+        return win[(u32)k * SQ2_LEN + (u32)(w - base[k])];
+    }
+    // 32-bit words of the copies (base_k is even, so q and the copy
+    // index have the same parity)
+    GF2T_HDM u32 ld32(int k, u32 q) const {
+        return ((const u32 *)win)[(u32)k * 2 * SQ2_LEN + (u32)(q - 2 * base[k])];
+    }
+    GF2T_HDM u64 ld64(int k, u32 q) const {   // q even
+        return win[(u32)k * SQ2_LEN + ((u32)(q - 2 * base[k]) >> 1)];
+    }
+    GF2T_HDM void raw3(int k, u32 q, u32 qodd, u32 &a, u32 &b, u32 &c) const {
+        if (qodd) {
+            a = ld32(k, q);
+            const u64 w = ld64(k, q + 1);
+            b = (u32)w; c = (u32)(w >> 32);
+        } else {
+            const u64 w = ld64(k, q);
+            a = (u32)w; b = (u32)(w >> 32);
+            c = ld32(k, q + 2);
         }
     }
+    GF2T_HDM void raw2(int k, u32 q, u32 qodd, u32 &a, u32 &b) const {
+        if (qodd) { a = ld32(k, q); b = ld32(k, q + 1); }
+        else { const u64 w = ld64(k, q); a = (u32)w; b = (u32)(w >> 32); }
+    }
+};
+GF2T_HD Sq2Tile sq2_tile(const u64 *win, const SqConst &C, size_t i0) {
+    Sq2Tile t;
+    t.win = win;
+#pragma unroll
+    for (int k = 0; k < SQ2_WINS; k++) t.base[k] = 2 * sq2_p0(C, i0, k);
+    return t;
 }
 
 // ---- XCD-aware visiting order of the output tiles of one residue

@@ -831,18 +831,72 @@ extern "C" uint64_t gf2t_primitive_poly(int d) {
 }
 
 namespace {
+// per-degree phase timers, destroyed on every return path
+struct SieveEvents {
+    hipEvent_t e[4] = {};
+    ~SieveEvents() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+// Device memory of the degrees <= 32: the log table, the two partition
+// buffers and the class / marking arrays of the largest degree.
+struct NarrowMem {
+    u32 *L;
+    unsigned long long *P1, *P2;
+    u32 *surv;
+    MarkRec *queue;
+};
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The degree-<=32 arrays are carved out of the wide tables whenever those
+// are allocated (they are larger: 12 bytes per element of GF(2^33) against
+// 12 per element of GF(2^32) plus the class arrays), so a sieve to degree
+// 34 never holds both sets and never frees and reallocates between them.
+int narrow_mem(int dn, NarrowMem &m) {
+    SieveWork &W = *g_sw;
+    const size_t N = (size_t)1 << dn;
+    const u64 ncls = irreducible_count(dn);
+    const size_t lbytes = align256(N * sizeof(u32)), sbytes = align256((ncls + 64) * sizeof(u32));
+    const size_t qbytes = (ncls + 64) * sizeof(MarkRec);
+    const bool part = dn >= LP_MIN_D;
+    const size_t pbytes = part ? N * 8 : 0;
+    if (W.wP1.bytes >= 2 * pbytes && W.wLlo.bytes >= lbytes + sbytes + qbytes) {
+        char *l = (char *)W.wLlo.p;
+        m.L = (u32 *)l;
+        m.surv = (u32 *)(l + lbytes);
+        m.queue = (MarkRec *)(l + lbytes + sbytes);
+        m.P1 = (unsigned long long *)W.wP1.p;
+        m.P2 = (unsigned long long *)((char *)W.wP1.p + pbytes);
+        return GF2T_OK;
+    }
+    int rc;
+    if ((rc = W.bL.ensure(lbytes))) return rc;
+    if ((rc = W.bSurv.ensure(sbytes))) return rc;
+    if ((rc = W.bQueue.ensure(qbytes))) return rc;
+    if (part) {
+        if ((rc = W.bP1.ensure(pbytes))) return rc;
+        if (lp_shape(dn).B2 > 0 && (rc = W.bP2.ensure(pbytes))) return rc;
+    }
+    m.L = (u32 *)W.bL.p;
+    m.surv = (u32 *)W.bSurv.p;
+    m.queue = (MarkRec *)W.bQueue.p;
+    m.P1 = (unsigned long long *)W.bP1.p;
+    m.P2 = (unsigned long long *)W.bP2.p;
+    return GF2T_OK;
+}
+
 // degrees dmin..dmax <= 32 (u32 field elements); g_sw_mu held by the caller
 int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
                  gf2t_sieve_stats *stats) {
-    DevBuf &bL = g_sw->bL, &bSurv = g_sw->bSurv, &bQueue = g_sw->bQueue, &bHeavy = g_sw->bHeavy,
-           &bTab = g_sw->bTab, &bCnt = g_sw->bCnt, &bP1 = g_sw->bP1, &bP2 = g_sw->bP2,
-           &bCur = g_sw->bCur;
+    DevBuf &bHeavy = g_sw->bHeavy, &bTab = g_sw->bTab, &bCnt = g_sw->bCnt, &bCur = g_sw->bCur;
     int rc;
-    if ((rc = bL.ensure(((size_t)1 << dmax) * sizeof(u32)))) return rc;
+    NarrowMem mem;
+    if ((rc = narrow_mem(dmax, mem))) return rc;
     if (dmax >= LP_MIN_D) {
         LpShape S = lp_shape(dmax);
-        if ((rc = bP1.ensure(((size_t)1 << dmax) * 8))) return rc;
-        if (S.B2 > 0 && (rc = bP2.ensure(((size_t)1 << dmax) * 8))) return rc;
         if ((rc = bCur.ensure(((size_t)1 << (S.B1 + S.B2)) * 4 + 256 * 4))) return rc;
         static std::once_flag lds_once;
         static hipError_t lds_err = hipSuccess;
@@ -859,7 +913,8 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     unsigned long long *cnt64 = (unsigned long long *)bCnt.p;   // [0] nsurv [1] nqueue
     unsigned *cnt32 = (unsigned *)((char *)bCnt.p + 32);         // [0] nheavy [1] bad
     u32 *Alo = (u32 *)bTab.p, *Ahi = Alo + 65536;
-    hipEvent_t ev[4];
+    SieveEvents evs;
+    hipEvent_t(&ev)[4] = evs.e;
     for (auto &e : ev) GF2T_HIP(hipEventCreate(&e));
 
     int status = GF2T_OK;
@@ -884,9 +939,8 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         DegreeParams P{d, M, q, rmod, smax, tmask, Alo, Ahi};
 
         const u64 ncls = irreducible_count(d);
-        if ((rc = bSurv.ensure((ncls + 64) * sizeof(u32)))) { status = rc; break; }
-        u32 *surv = (u32 *)bSurv.p;
-        u32 *L = (u32 *)bL.p;
+        u32 *surv = mem.surv;
+        u32 *L = mem.L;
         Heavy *heavy = (Heavy *)bHeavy.p;
 
         GF2T_HIP(hipEventRecord(ev[0], 0));
@@ -898,8 +952,8 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((ncur + 255) / 256)), dim3(256), 0, 0,
                                cur1, ncur);
             hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
-            unsigned long long *p1 = (unsigned long long *)bP1.p;
-            unsigned long long *p2 = S.B2 ? (unsigned long long *)bP2.p : p1;
+            unsigned long long *p1 = mem.P1;
+            unsigned long long *p2 = S.B2 ? mem.P2 : p1;
             hipLaunchKernelGGL(k_lp_part1, dim3((unsigned)((M + LP_E - 1) / LP_E)), dim3(LP_T), 0,
                                0, P, S, p1, cur1, cnt32 + 1);
             if (S.B2)
@@ -937,8 +991,7 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             status = GF2T_EINTERNAL;
             break;
         }
-        if ((rc = bQueue.ensure((ncls + 64) * sizeof(MarkRec)))) { status = rc; break; }
-        MarkRec *queue = (MarkRec *)bQueue.p;
+        MarkRec *queue = mem.queue;
         if (nsurv) {
             u64 zb = std::min<u64>((nsurv + 255) / 256, 1u << 18);
             hipLaunchKernelGGL(k_zech, dim3((unsigned)zb), dim3(256), 0, 0, P, surv, nsurv, L,
@@ -984,13 +1037,12 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             GF2T_HIP(hipEventElapsedTime(&S.ms_heavy, ev[2], ev[3]));
         }
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     return status;
 }
 
 // degrees dmin..dmax in 33..34 (u64 field elements); g_sw_mu held
-int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
-               gf2t_sieve_stats *stats) {
+// the tables of degrees up to dmax in 33..34
+int wide_mem(int dmax) {
     SieveWork &W = *g_sw;
     int rc;
     const u64 N = (u64)1 << dmax;
@@ -1002,6 +1054,14 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     if ((rc = W.wCur.ensure((256 + 2048) * 4))) return rc;
     if ((rc = W.bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
     if ((rc = W.bCnt.ensure(64))) return rc;
+    return GF2T_OK;
+}
+
+int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
+               gf2t_sieve_stats *stats) {
+    SieveWork &W = *g_sw;
+    int rc;
+    if ((rc = wide_mem(dmax))) return rc;
     static std::once_flag lds_once;
     static hipError_t lds_err = hipSuccess;
     const int place_lds = (int)((((size_t)1 << W_SB) + ((size_t)1 << W_SB) / 16) * sizeof(u32));
@@ -1017,7 +1077,8 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     u32 *Llo = (u32 *)W.wLlo.p, *Lhi = (u32 *)W.wLhi.p;
     u64 *A0 = (u64 *)W.wTab.p, *A1 = A0 + ((size_t)1 << W_AB);
     Heavy *heavy = (Heavy *)W.bHeavy.p;
-    hipEvent_t ev[4];
+    SieveEvents evs;
+    hipEvent_t(&ev)[4] = evs.e;
     for (auto &e : ev) GF2T_HIP(hipEventCreate(&e));
     int status = GF2T_OK;
     for (int d = dmin; d <= dmax && status == GF2T_OK; d++) {
@@ -1136,7 +1197,6 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             GF2T_HIP(hipEventElapsedTime(&S.ms_heavy, ev[2], ev[3]));
         }
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     return status;
 }
 }  // namespace
@@ -1154,12 +1214,15 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
     std::lock_guard<std::mutex> lk(g_sw_mu);
     if (!g_sw) g_sw = new SieveWork();
     int status = GF2T_OK;
-    if (dmin <= 32) status = sieve_narrow(r, dmin, std::min(dmax, 32), smax, dbest, stats);
-    if (status == GF2T_OK && dmax >= 33) {
-        // the narrow tables make room for the wide ones
+    if (dmax >= 33) {
+        // the degree-<=32 arrays live inside the wide tables from here on
         for (DevBuf *b : {&g_sw->bL, &g_sw->bP1, &g_sw->bP2, &g_sw->bSurv, &g_sw->bQueue})
             b->release();
-        status = sieve_wide(r, std::max(dmin, 33), dmax, smax, dbest, stats);
+        status = wide_mem(dmax);
     }
+    if (status == GF2T_OK && dmin <= 32)
+        status = sieve_narrow(r, dmin, std::min(dmax, 32), smax, dbest, stats);
+    if (status == GF2T_OK && dmax >= 33)
+        status = sieve_wide(r, std::max(dmin, 33), dmax, smax, dbest, stats);
     return status;
 }

@@ -46,6 +46,21 @@ struct SqDesc {
     const int *order;   // block -> tile (-1 idle), or null for identity
 };
 
+// output pair i (one 16-byte store when both words exist)
+template <bool NT>
+__device__ __forceinline__ void store_pair(u64 *out_, size_t nw, size_t i, u64 o0, u64 o1) {
+    typedef u64 v2u __attribute__((ext_vector_type(2)));
+    GF2T_GLOBAL u64 *out = (GF2T_GLOBAL u64 *)out_;
+    if (2 * i + 1 < nw) {
+        GF2T_GLOBAL v2u *p = (GF2T_GLOBAL v2u *)(out + 2 * i);
+        if (NT) __builtin_nontemporal_store(v2u{o0, o1}, p);
+        else *p = v2u{o0, o1};
+    } else {
+        if (NT) __builtin_nontemporal_store(o0, out + 2 * i);
+        else out[2 * i] = o0;
+    }
+}
+
 // batched: blockIdx.y selects the residue, blockIdx.x a tile of
 // SQ_THREADS*PPT output pairs (pairs j*SQ_THREADS + lane for j < PPT)
 template <int PPT, bool NT>
@@ -54,32 +69,115 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
     const SqDesc d = desc[blockIdx.y];
     int t = (int)blockIdx.x;
     if (d.order) {
-        t = d.order[blockIdx.x];
+        t = __builtin_amdgcn_readfirstlane(d.order[blockIdx.x]);   // one value per block
         if (t < 0) return;
     }
     const size_t np = (d.c.nw + 1) / 2;
-    const size_t base = (size_t)t * SQ_THREADS * PPT + threadIdx.x;
+    const size_t t0 = (size_t)t * SQ_THREADS * PPT;
+    if (t0 >= np) return;
+    // one region for the whole tile (all but a few tiles per residue)
+    const int treg = sq_tile_region(d.c, t0, t0 + SQ_THREADS * PPT < np ? t0 + SQ_THREADS * PPT : np);
+    const SqArray src{d.in, d.c.nw};
+    // streaming stores: F^2 is not re-read before the next launch (one
+    // 16-byte store per pair; residue buffers are 16-byte aligned)
+    if (treg < 3) {
+        sq_variant(treg, d.c, [&](auto v) {
 #pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        size_t i = base + (size_t)j * SQ_THREADS;
-        if (i < np) {
-            u64 o0, o1;
-            square_fold_pair_fast(d.in, d.c, i, o0, o1);
-            if (NT) {
-                // streaming store: F^2 is not re-read before the next launch
-                // (one 16-byte store per pair; residue buffers are 16-byte
-                // aligned)
-                typedef u64 v2u __attribute__((ext_vector_type(2)));
-                if (2 * i + 1 < d.c.nw)
-                    __builtin_nontemporal_store(v2u{o0, o1}, (v2u *)(d.out + 2 * i));
-                else
-                    __builtin_nontemporal_store(o0, d.out + 2 * i);
-            } else {
-                d.out[2 * i] = o0;
-                if (2 * i + 1 < d.c.nw) d.out[2 * i + 1] = o1;
+            for (int j = 0; j < PPT; j++) {
+                const size_t i = t0 + (size_t)j * SQ_THREADS + threadIdx.x;
+                if (i < np) {
+                    u64 o0, o1;
+                    v.pair(src, d.c, (u32)i, o0, o1);
+                    store_pair<NT>(d.out, d.c.nw, i, o0, o1);
+                }
+            }
+        });
+    } else {
+        for (int j = 0; j < PPT; j++) {
+            const size_t i = t0 + (size_t)j * SQ_THREADS + threadIdx.x;
+            if (i < np) {
+                u64 o0, o1;
+                square_fold_pair_fast(d.in, d.c, i, o0, o1);
+                store_pair<NT>(d.out, d.c.nw, i, o0, o1);
             }
         }
     }
+}
+
+// ---- two squarings per launch: H = (F^2)^2 mod T (tiles of SQ2_TP pairs)
+template <bool NT>
+__device__ __forceinline__ void square2_tile(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                             const SqConst &c, size_t tile, u64 *gw) {
+    static_assert(SQ_THREADS == SQ2_GP && 2 * SQ_THREADS >= SQ2_TP, "tile shape");
+    typedef u64 v2u __attribute__((ext_vector_type(2)));
+    const size_t np = (c.nw + 1) / 2;
+    const size_t i0 = tile * SQ2_TP;
+    if (i0 >= np) return;   // whole block
+    const size_t i1 = i0 + SQ2_TP < np ? i0 + SQ2_TP : np;
+    const int treg = sq_tile_region(c, i0, i1);
+    const u32 need = sq2_need(treg);
+    // G phase: one G pair per thread and window; the loads of all the
+    // windows go out before any of them is combined
+    const SqArray fsrc{in, c.nw};
+    u64 p0[SQ2_WINS];
+    int greg[SQ2_WINS];
+    SqRawPair raw[SQ2_WINS];
+#pragma unroll
+    for (int k = 0; k < SQ2_WINS; k++) {
+        p0[k] = sq2_p0(c, i0, k);
+        greg[k] = ((need >> k) & 1) ? sq_tile_region(c, p0[k], p0[k] + SQ2_GP) : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < SQ2_WINS; k++)
+        if (greg[k] >= 0 && greg[k] < 3)
+            raw[k] = sq_raw_pair(fsrc, c, (u32)(p0[k] + threadIdx.x), greg[k]);
+#pragma unroll
+    for (int k = 0; k < SQ2_WINS; k++) {
+        if (greg[k] < 0) continue;
+        u64 o0, o1;
+        if (greg[k] < 3) sq_combine_pair(raw[k], c, greg[k], o0, o1);
+        else sq2_g_pair_clipped(in, c, p0[k] + threadIdx.x, o0, o1);
+        *(v2u *)(gw + k * SQ2_LEN + 2 * threadIdx.x) = v2u{o0, o1};
+    }
+    __syncthreads();
+    // H phase from the window copies
+    const Sq2Tile src = sq2_tile(gw, c, i0);
+    if (treg < 3) {
+        sq_variant(treg, c, [&](auto v) {
+            for (int j = 0; j < 2; j++) {
+                const size_t i = i0 + (size_t)j * SQ_THREADS + threadIdx.x;
+                if (i < i1) {
+                    u64 o0, o1;
+                    v.pair(src, c, (u32)i, o0, o1);
+                    store_pair<NT>(out, c.nw, i, o0, o1);
+                }
+            }
+        });
+    } else {
+        for (int j = 0; j < 2; j++) {
+            const size_t i = i0 + (size_t)j * SQ_THREADS + threadIdx.x;
+            if (i < i1) {
+                u64 o0, o1;
+                square_fold_pair_fast_src(src, c, i, o0, o1);
+                store_pair<NT>(out, c.nw, i, o0, o1);
+            }
+        }
+    }
+}
+
+// batched: blockIdx.y selects the residue, blockIdx.x the tile
+template <bool NT>
+__global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_batch(
+        const SqDesc *__restrict__ desc) {
+    __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
+    const SqDesc d = desc[blockIdx.y];
+    square2_tile<NT>(d.in, d.out, d.c, blockIdx.x, gw);
+}
+
+__global__ __launch_bounds__(SQ_THREADS) void k_square2_fold(
+        const u64 *__restrict__ in, u64 *__restrict__ out, const SqConst c) {
+    __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
+    square2_tile<false>(in, out, c, blockIdx.x, gw);
 }
 
 // slab: residue y at in + y*pitch, its s in svec[y]
@@ -130,10 +228,19 @@ __global__ void k_xor_word0(u64 *p, u64 v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) p[0] ^= v;
 }
 
-bool valid_rs(u64 r, u64 s) { return r >= 3 && s >= 1 && 2 * s < r; }
+// r below 2^34: the kernels address a residue with 32-bit byte offsets
+bool valid_rs(u64 r, u64 s) { return r >= 3 && r < (1ULL << 34) && s >= 1 && 2 * s < r; }
 
 // tuning knobs of the batched squaring (gf2t_tune / GF2T_SQ_PPT, GF2T_SQ_ORDER)
-int g_sq_ppt = -1, g_sq_order = -1, g_sq_nt = -1;
+int g_sq_ppt = -1, g_sq_order = -1, g_sq_nt = -1, g_sq_fuse = -1;
+// two squarings per launch wherever a chain has two steps left
+bool sq_fuse() {
+    if (g_sq_fuse < 0) {
+        const char *e = getenv("GF2T_SQ_FUSE");
+        g_sq_fuse = e ? atoi(e) : 1;
+    }
+    return g_sq_fuse != 0;
+}
 bool sq_nt() {
     if (g_sq_nt < 0) {
         const char *e = getenv("GF2T_SQ_NT");
@@ -204,13 +311,22 @@ extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r
     size_t nw = c.nw;
     hipStream_t st = (hipStream_t)stream;
     u64 *a = dres, *b = dtmp;
-    for (u64 k = 0; k < steps; k++) {
-        hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((nw + 1) / 2, SQ_THREADS)),
-                           dim3(SQ_THREADS), 0, st, a, b, c);
+    const bool fuse = sq_fuse();
+    u64 launches = 0;
+    for (u64 k = 0; k < steps; launches++) {
+        if (fuse && steps - k >= 2) {
+            hipLaunchKernelGGL(k_square2_fold, dim3(blocks_for((nw + 1) / 2, SQ2_TP)),
+                               dim3(SQ_THREADS), 0, st, a, b, c);
+            k += 2;
+        } else {
+            hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((nw + 1) / 2, SQ_THREADS)),
+                               dim3(SQ_THREADS), 0, st, a, b, c);
+            k += 1;
+        }
         std::swap(a, b);
     }
     GF2T_CHECK_LAUNCH("k_square_fold");
-    if (steps & 1) GF2T_HIP(hipMemcpyAsync(dres, dtmp, nw * 8, hipMemcpyDeviceToDevice, st));
+    if (launches & 1) GF2T_HIP(hipMemcpyAsync(dres, dtmp, nw * 8, hipMemcpyDeviceToDevice, st));
     return GF2T_OK;
 }
 
@@ -240,6 +356,7 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
     if (!strcmp(key, "sq_ppt")) { g_sq_ppt = (int)value; return GF2T_OK; }
     if (!strcmp(key, "sq_order")) { g_sq_order = (int)value; return GF2T_OK; }
     if (!strcmp(key, "sq_nt")) { g_sq_nt = (int)value; return GF2T_OK; }
+    if (!strcmp(key, "sq_fuse")) { g_sq_fuse = (int)value; return GF2T_OK; }
     if (!strcmp(key, "gcd_dev_bits")) {
         gf2t_host::set_device_product_bits(value > 0 ? (uint64_t)value : 0);
         return GF2T_OK;
@@ -341,19 +458,41 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
         h[n + k] = SqDesc{dtmp[i], dres[i], c, ot};      // odd steps
     }
     GF2T_HIP(hipMemcpyAsync(g_batch.d, h.data(), need * sizeof(SqDesc), hipMemcpyHostToDevice, st));
-    u64 kmax = steps[ord[0]];
+    // Every launch advances all live chains together, by two squarings
+    // while each of them has two left, else by one; a chain leaves the
+    // prefix when its steps are done, after launches[k] launches (its
+    // value is in dtmp when that count is odd).
+    const bool fuse = sq_fuse();
+    size_t gx2 = 0;
+    for (int i = 0; i < n; i++)
+        gx2 = std::max<size_t>(gx2, (words_of(r[i]) + 1) / 2);
+    gx2 = (gx2 + SQ2_TP - 1) / SQ2_TP;
+    std::vector<u64> launches(n, 0);
     int active = n;
-    for (u64 k = 0; k < kmax; k++) {
-        while (active > 0 && steps[ord[active - 1]] <= k) active--;
-        const SqDesc *tab = g_batch.d + ((k & 1) ? n : 0);
-        if (ppt == 4) launch_batch<4>((unsigned)gx, active, tab, st);
-        else if (ppt == 2) launch_batch<2>((unsigned)gx, active, tab, st);
-        else launch_batch<1>((unsigned)gx, active, tab, st);
+    u64 nl = 0;
+    for (u64 k = 0;; nl++) {
+        while (active > 0 && steps[ord[active - 1]] <= k) launches[--active] = nl;
+        if (!active) break;
+        const SqDesc *tab = g_batch.d + ((nl & 1) ? n : 0);
+        if (fuse && steps[ord[active - 1]] - k >= 2) {
+            if (sq_nt())
+                hipLaunchKernelGGL(k_square2_fold_batch<true>, dim3((unsigned)gx2, active),
+                                   dim3(SQ_THREADS), 0, st, tab);
+            else
+                hipLaunchKernelGGL(k_square2_fold_batch<false>, dim3((unsigned)gx2, active),
+                                   dim3(SQ_THREADS), 0, st, tab);
+            k += 2;
+        } else {
+            if (ppt == 4) launch_batch<4>((unsigned)gx, active, tab, st);
+            else if (ppt == 2) launch_batch<2>((unsigned)gx, active, tab, st);
+            else launch_batch<1>((unsigned)gx, active, tab, st);
+            k += 1;
+        }
     }
     GF2T_CHECK_LAUNCH("k_square_fold_batch");
-    for (int i = 0; i < n; i++)
-        if (steps[i] & 1)
-            GF2T_HIP(hipMemcpyAsync(dres[i], dtmp[i], words_of(r[i]) * 8,
+    for (int k = 0; k < n; k++)
+        if (launches[k] & 1)
+            GF2T_HIP(hipMemcpyAsync(dres[ord[k]], dtmp[ord[k]], words_of(r[ord[k]]) * 8,
                                     hipMemcpyDeviceToDevice, st));
     // the host tables must outlive the async copies
     GF2T_HIP(hipStreamSynchronize(st));

@@ -1,6 +1,32 @@
 // Host build of the kernels' arithmetic (mersenne_trinomial_amd/csrc/
 // gf2t_arith.hpp) for CPU checks against the oracle.  Test code only.
 #include "gf2t_arith.hpp"
+#include <stdlib.h>
+#include <vector>
+
+// the two-squaring tile's window copies, with every read checked
+// against the words the G phase wrote
+struct CheckedTile {
+    Sq2Tile t;
+    const std::vector<char> *written;
+    u64 operator()(int k, size_t w) const {
+        const u64 at = w - t.base[k];
+        if (w < t.base[k] || at >= SQ2_LEN || !(*written)[(size_t)k * SQ2_LEN + at]) abort();
+        return t(k, w);
+    }
+    u32 ld32(int k, u32 q) const {
+        const u64 w = (*this)(k, q >> 1);
+        return (q & 1) ? (u32)(w >> 32) : (u32)w;
+    }
+    void raw3(int k, u32 q, u32 qodd, u32 &a, u32 &b, u32 &c) const {
+        if (qodd != (q & 1)) abort();
+        a = ld32(k, q); b = ld32(k, q + 1); c = ld32(k, q + 2);
+    }
+    void raw2(int k, u32 q, u32 qodd, u32 &a, u32 &b) const {
+        if (qodd != (q & 1)) abort();
+        a = ld32(k, q); b = ld32(k, q + 1);
+    }
+};
 
 extern "C" {
 void kl_square_fold(const u64 *in, u64 *out, u64 r, u64 s) {
@@ -21,6 +47,68 @@ void kl_square_fold_fast(const u64 *in, u64 *out, u64 r, u64 s) {
         if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
     }
 }
+// the two-squaring kernel's tiles, run one after another: the G phase
+// (one G pair per thread and needed window), then the tile's H pairs
+// through the same calls the kernel makes
+void kl_square2_fold(const u64 *in, u64 *out, u64 r, u64 s) {
+    SqConst c = sq_const(r, s);
+    const size_t np = (c.nw + 1) / 2;
+    std::vector<u64> gw((size_t)SQ2_WINS * SQ2_LEN);
+    std::vector<char> written(gw.size());
+    for (size_t i0 = 0; i0 < np; i0 += SQ2_TP) {
+        const size_t i1 = i0 + SQ2_TP < np ? i0 + SQ2_TP : np;
+        const int treg = sq_tile_region(c, i0, i1);
+        const u32 need = sq2_need(treg);
+        std::fill(gw.begin(), gw.end(), 0x5A5A5A5A5A5A5A5AULL);
+        std::fill(written.begin(), written.end(), 0);
+        for (int k = 0; k < SQ2_WINS; k++) {
+            if (!((need >> k) & 1)) continue;
+            const u64 p0 = sq2_p0(c, i0, k);
+            const int greg = sq_tile_region(c, p0, p0 + SQ2_GP);
+            for (u32 t = 0; t < SQ2_GP; t++) {
+                u64 o0, o1;
+                if (greg < 3) {
+                    SqRawPair w = sq_raw_pair(SqArray{in, c.nw}, c, (u32)(p0 + t), greg);
+                    sq_combine_pair(w, c, greg, o0, o1);
+                } else {
+                    sq2_g_pair_clipped(in, c, p0 + t, o0, o1);
+                }
+                gw[(size_t)k * SQ2_LEN + 2 * t] = o0;
+                gw[(size_t)k * SQ2_LEN + 2 * t + 1] = o1;
+                written[(size_t)k * SQ2_LEN + 2 * t] = written[(size_t)k * SQ2_LEN + 2 * t + 1] = 1;
+            }
+        }
+        CheckedTile src{sq2_tile(gw.data(), c, i0), &written};
+        for (size_t i = i0; i < i1; i++) {
+            u64 o0, o1;
+            if (treg < 3) sq_pair_region(src, c, (u32)i, treg, o0, o1);
+            else square_fold_pair_fast_src(src, c, i, o0, o1);
+            out[2 * i] = o0;
+            if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
+        }
+    }
+}
+// the single-squaring batch kernel's per-tile region path
+void kl_square_fold_tiles(const u64 *in, u64 *out, u64 r, u64 s, u32 tile_pairs) {
+    SqConst c = sq_const(r, s);
+    const size_t np = (c.nw + 1) / 2;
+    const SqArray src{in, c.nw};
+    for (size_t t0 = 0; t0 < np; t0 += tile_pairs) {
+        const size_t t1 = t0 + tile_pairs < np ? t0 + tile_pairs : np;
+        const int treg = sq_tile_region(c, t0, t1);
+        for (size_t i = t0; i < t1; i++) {
+            u64 o0, o1;
+            if (treg < 3) sq_pair_region(src, c, (u32)i, treg, o0, o1);
+            else square_fold_pair_fast(in, c, i, o0, o1);
+            out[2 * i] = o0;
+            if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
+        }
+    }
+}
+u64 kl_spread32p(u32 x) { return spread32p(x); }
+u64 kl_interleave32(u32 e, u32 o) { return interleave32(e, o); }
+u64 kl_spread32(u32 x) { return spread32(x); }
+u64 kl_funnel64(u64 a, u64 b, u32 sh) { return funnel64(a, b, sh); }
 void kl_fold(const u64 *p, u64 L, u64 *out, u64 r, u64 s) {
     size_t nw = (size_t)((r + 63) >> 6), pw = (size_t)((L + 63) >> 6);
     for (size_t w = 0; w < nw; w++) out[w] = fold_word(p, pw, L, r, s, w);

@@ -62,24 +62,41 @@ def test_full_size_golden_residue(eng):
 
 
 def test_full_size_random_s(eng):
+    # two steps: one launch of the two-squaring kernel
     rng = np.random.default_rng(2024)
     for s in (1, 2, 68139919, 68139920, int(rng.integers(3, 68139920))):
         a = rand_res(rng, R136)
         assert np.array_equal(eng.square_fold(a, R136, s, 2), O.square_fold_steps(a, R136, s, 2)), s
 
 
-@pytest.fixture(params=[(1, 0, 0), (2, 0, 0), (4, 0, 1), (1, 1, 0), (2, 1, 1), (4, 1, 0)],
-                ids=lambda p: f"ppt{p[0]}-order{p[1]}-nt{p[2]}")
+@pytest.mark.parametrize("r", [4423, 9689, 132049])
+def test_two_squarings_every_s(eng, r):
+    """Every s of small r through one two-squaring launch, so that tiles
+    meet s, 2s and r at every offset."""
+    rng = np.random.default_rng(r)
+    step = {4423: 1, 9689: 7}.get(r, 997)
+    for s in range(1, (r - 1) // 2 + 1, step):
+        a = rand_res(rng, r)
+        assert np.array_equal(eng.square_fold(a, r, s, 2), O.square_fold_steps(a, r, s, 2)), s
+
+
+@pytest.fixture(params=[(1, 0, 0, 0), (2, 0, 0, 0), (4, 0, 1, 0), (1, 1, 0, 0), (2, 1, 1, 0),
+                        (4, 1, 0, 0), (2, 0, 1, 1), (2, 0, 0, 1), (1, 1, 1, 1)],
+                ids=lambda p: f"ppt{p[0]}-order{p[1]}-nt{p[2]}-fuse{p[3]}")
 def tuned(request, eng):
+    """Single-squaring launch shapes (fuse0) and the two-squaring kernel
+    (fuse1, with the single kernel for odd remainders)."""
     from mersenne_trinomial_amd._capi import check, lib
-    ppt, order, nt = request.param
+    ppt, order, nt, fuse = request.param
     check(lib.gf2t_tune(b"sq_ppt", ppt))
     check(lib.gf2t_tune(b"sq_order", order))
     check(lib.gf2t_tune(b"sq_nt", nt))
+    check(lib.gf2t_tune(b"sq_fuse", fuse))
     yield request.param
     check(lib.gf2t_tune(b"sq_ppt", 2))
     check(lib.gf2t_tune(b"sq_order", 0))
     check(lib.gf2t_tune(b"sq_nt", 1))
+    check(lib.gf2t_tune(b"sq_fuse", 1))
 
 
 def test_batch_uneven_frontiers_mixed_r(eng, tuned):

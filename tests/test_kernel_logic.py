@@ -24,6 +24,16 @@ def kl(tmp_path_factory):
     lib.kl_square_fold.argtypes = [P, P, C.c_uint64, C.c_uint64]
     lib.kl_square_fold_fast.argtypes = [P, P, C.c_uint64, C.c_uint64]
     lib.kl_fold.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint64]
+    lib.kl_square2_fold.argtypes = [P, P, C.c_uint64, C.c_uint64]
+    lib.kl_square_fold_tiles.argtypes = [P, P, C.c_uint64, C.c_uint64, C.c_uint32]
+    lib.kl_spread32p.restype = C.c_uint64
+    lib.kl_spread32p.argtypes = [C.c_uint32]
+    lib.kl_spread32.restype = C.c_uint64
+    lib.kl_interleave32.restype = C.c_uint64
+    lib.kl_interleave32.argtypes = [C.c_uint32, C.c_uint32]
+    lib.kl_spread32.argtypes = [C.c_uint32]
+    lib.kl_funnel64.restype = C.c_uint64
+    lib.kl_funnel64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
     lib.kl_gf64_mul.restype = C.c_uint64
     lib.kl_gf64_mul.argtypes = [C.c_uint64, C.c_uint64]
     lib.kl_clmul32.restype = C.c_uint64
@@ -113,6 +123,64 @@ def test_square_fold_fast_full_size_sample(kl):
         out = np.zeros(O.words(r), np.uint64)
         kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
         assert np.array_equal(out, O.square_fold(a, r, s)), s
+
+
+def _spread_ref(x):
+    return sum(((x >> k) & 1) << (2 * k) for k in range(32))
+
+
+def test_spread_by_byte_permute(kl):
+    rng = np.random.default_rng(11)
+    xs = [0, 1, 0xFFFFFFFF, 0x80000000, 0x0F0F0F0F, 0xF0F0F0F0, 0x88888888, 0x77777777]
+    xs += [int(v) for v in rng.integers(0, 2**32, 2000, dtype=np.uint64)]
+    for x in xs:
+        assert kl.kl_spread32p(x) == _spread_ref(x) == kl.kl_spread32(x), hex(x)
+    for e, o in zip(xs, reversed(xs)):
+        assert kl.kl_interleave32(e, o) == _spread_ref(e) | (_spread_ref(o) << 1), (hex(e), hex(o))
+
+
+def test_funnel_shift(kl):
+    rng = np.random.default_rng(12)
+    for _ in range(500):
+        a, b = (int(v) for v in rng.integers(0, 2**64, 2, dtype=np.uint64))
+        for sh in range(64):
+            want = ((a >> sh) | (b << (64 - sh))) & (2**64 - 1) if sh else a
+            assert kl.kl_funnel64(a, b, sh) == want, (a, b, sh)
+
+
+def test_two_squarings_per_tile(kl):
+    """The two-squaring kernel's tiles (window plan, G phase into per-window
+    copies, H phase from them; every read checked against what the G
+    phase wrote) give (F^2)^2 mod T.  Small r puts the points s, 2s and r
+    inside tiles at every offset; the single kernel's per-tile region path
+    is checked at several tile sizes alongside."""
+    rng = np.random.default_rng(13)
+    cases = [(r, s) for r in (7, 31, 65, 127, 129, 131, 193, 257, 383)
+             for s in range(1, (r - 1) // 2 + 1)]
+    cases += [(r, s) for r in (4423, 4097, 9689, 132049, 300007)
+              for s in sorted({1, 2, 63, 64, 65, 127, 128, 129, 255, 256, 257, r // 3,
+                               (r - 1) // 2 - 64, (r - 1) // 2 - 1, (r - 1) // 2})]
+    cases += [(300007, int(s)) for s in rng.integers(1, 150003, 40)]
+    for r, s in cases:
+        a = rand_res(rng, r)
+        one = O.square_fold(a, r, s)
+        want = O.square_fold(one, r, s)
+        out = np.zeros(O.words(r), np.uint64)
+        kl.kl_square2_fold(ptr(a), ptr(out), r, s)
+        assert np.array_equal(out, want), (r, s)
+        for tp in (1, 3, 512):
+            kl.kl_square_fold_tiles(ptr(a), ptr(out), r, s, tp)
+            assert np.array_equal(out, one), (r, s, tp)
+
+
+def test_two_squarings_full_size_sample(kl):
+    rng = np.random.default_rng(14)
+    r = 136279841
+    for s in (474, 68139920):
+        a = rand_res(rng, r)
+        out = np.zeros(O.words(r), np.uint64)
+        kl.kl_square2_fold(ptr(a), ptr(out), r, s)
+        assert np.array_equal(out, O.square_fold(O.square_fold(a, r, s), r, s)), s
 
 
 def test_fold_lengths(kl):
