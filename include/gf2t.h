@@ -207,7 +207,7 @@ int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_t *db,
  * ------------------------------------------------------------------ */
 /* Pascal build of the h-table for block start kstart (degrees kstart..
  * kstart+m-1), into dh; dh_alt is an equal-sized scratch table.
- * 1 <= m <= 63. */
+ * 1 <= m <= 64. */
 int gf2t_ddf_init_slab(uint64_t *dh, uint64_t *dh_alt, size_t slot_pitch, int m,
                        uint64_t r, const uint64_t *ds, uint64_t kstart, int n,
                        void *stream);
@@ -217,6 +217,13 @@ int gf2t_ddf_init_slab(uint64_t *dh, uint64_t *dh_alt, size_t slot_pitch, int m,
 int gf2t_square_fold_slab_g(const uint64_t *din, uint64_t *dout, size_t pitch,
                             uint64_t r, const uint64_t *ds, size_t n, int group,
                             void *stream);
+/* The same advanced by two squarings in one pass (dout = din^4; din is
+ * not changed; pitch even and both slabs 16-byte aligned).  No reference
+ * counterpart: two of square_all_step (src/tsfactor.cu:278-300) fused,
+ * for the m squarings of a block. */
+int gf2t_square2_fold_slab_g(const uint64_t *din, uint64_t *dout, size_t pitch,
+                             uint64_t r, const uint64_t *ds, size_t n, int group,
+                             void *stream);
 /* dc_i = x^m + sum_j h_(i,j) x^(m-1-j) mod T_i = prod_t (x + x^(2^(k+1+t)))
  * (horner_C: k_horner_fused + k_horner_tail); 2m < r. */
 int gf2t_horner_slab(const uint64_t *dh, size_t slot_pitch, int m, uint64_t r,
@@ -299,6 +306,11 @@ typedef struct gf2t_scan_params {
     int block_m;      /* degrees per product: 1 = one product per degree
                          (trinomial_scan), m > 1 = blocked h-table scheme
                          (tsfactor), 0 = chosen from r                  */
+    int past_bound;   /* 1 = report a factor found in the last whole block
+                         of m degrees even when its degree exceeds the
+                         bound, as tsfactor does (its intervals always end
+                         on block boundaries, src/tsfactor.cu:1266-1273);
+                         0 = report only degrees <= the bound           */
 } gf2t_scan_params;
 
 typedef struct gf2t_scan_result {
@@ -312,7 +324,7 @@ typedef struct gf2t_scan_result {
 
 typedef struct gf2t_scan_results gf2t_scan_results;
 
-/* Scan every s of svals (1 <= s, 2s < r, r <= 2^23).  Replaces
+/* Scan every s of svals (1 <= s, 2s < r, r < 2^31).  Replaces
  * trinomial_scan's per-s loop (scan_one_s, src/trinomial_scan.cu:240-381)
  * and tsfactor's (src/tsfactor.cu:1089-1348) with cohorts of s values
  * advanced together on the device; results follow tsfactor's kinds

@@ -30,7 +30,7 @@ class GF2TError(RuntimeError):
 class ScanParams(C.Structure):
     _fields_ = [("k0", _u64), ("maxd", _u64), ("L0", _u64), ("Lmax", _u64),
                 ("gcd_threads", C.c_int), ("batch", C.c_int), ("verbose", C.c_int),
-                ("block_m", C.c_int)]
+                ("block_m", C.c_int), ("past_bound", C.c_int)]
 
 
 class ScanResult(C.Structure):
@@ -108,6 +108,7 @@ _SIGS = {
     "gf2t_server_destroy": (None, [_vp]),
     "gf2t_ddf_init_slab": (C.c_int, [_vp, _vp, _sz, C.c_int, _u64, _vp, _u64, C.c_int, _vp]),
     "gf2t_square_fold_slab_g": (C.c_int, [_vp, _vp, _sz, _u64, _vp, _sz, C.c_int, _vp]),
+    "gf2t_square2_fold_slab_g": (C.c_int, [_vp, _vp, _sz, _u64, _vp, _sz, C.c_int, _vp]),
     "gf2t_horner_slab": (C.c_int, [_vp, _sz, C.c_int, _u64, _vp, _vp, _sz, C.c_int, _vp]),
     "gf2t_mulx_xor_slab": (C.c_int, [_vp, _sz, _vp, _sz, _u64, _vp, C.c_int, _vp]),
     "gf2t_xor_word0_slab": (C.c_int, [_vp, _sz, _u64, C.c_int, _vp]),

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["capi.hip", "square.hip", "fft.hip", "sieve.hip", "ddf.hip"]
 HOST_SOURCES = ["host_poly.cpp", "hgcd.cpp", "scan.cpp", "serve.cpp"]
 # command-line drivers with the reference's argv forms and output files
-CLIS = ["coarse_sieve", "trinomial_stage2", "trinomial_scan"]
+CLIS = ["coarse_sieve", "trinomial_stage2", "trinomial_scan", "tsfactor"]
 CXX = os.environ.get("CXX", "g++")
 HOSTFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-mpclmul", "-msse4.1", "-pthread",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

@@ -14,12 +14,13 @@
 // (res_pitch = m*slot_pitch), so the whole table is a slab of n*m
 // residues with pitch slot_pitch whose residue y uses s_(y / m).
 #include "gf2t_internal.hpp"
+#include "host_poly.hpp"
 #include "../../include/gf2t.h"
 
 namespace {
 
 constexpr int DDF_THREADS = 256;
-constexpr int DDF_MAX_M = 63;
+constexpr int DDF_MAX_M = 64;
 
 unsigned blocks_for(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -120,7 +121,8 @@ __global__ __launch_bounds__(DDF_THREADS) void k_horner_slab(
                 acc ^= sh ? ((a << sh) | (b >> (64 - sh))) : a;
             }
             if (w == topw) acc &= (1ULL << top) - 1;   // w == topw only when top != 0
-            if (w == 0) acc ^= E ^ (1ULL << m);        // E and x^m (m <= 63)
+            if (w == 0) acc ^= E;                      // E has m - 1 <= 63 bits
+            if (w == (size_t)(m >> 6)) acc ^= 1ULL << (m & 63);   // x^m (2m < r)
             const size_t sw = (size_t)(s >> 6);
             const unsigned so = (unsigned)(s & 63);
             if (w == sw) acc ^= E << so;
@@ -267,8 +269,15 @@ extern "C" int gf2t_ddf_init_slab(uint64_t *dh, uint64_t *dh_alt, size_t slot_pi
     }
     GF2T_CHECK_LAUNCH("gf2t_ddf_init_slab");
     u64 *cur = dh, *oth = dh_alt;
-    for (u64 t = 0; t < kstart; t++) {
-        rc = gf2t_square_fold_slab_g(cur, oth, slot_pitch, r, ds, (size_t)n * m, m, stream);
+    const bool two = r >= (1u << 18) && gf2t_host::square_fuse_enabled();
+    for (u64 t = 0; t < kstart;) {
+        if (two && kstart - t >= 2) {
+            rc = gf2t_square2_fold_slab_g(cur, oth, slot_pitch, r, ds, (size_t)n * m, m, stream);
+            t += 2;
+        } else {
+            rc = gf2t_square_fold_slab_g(cur, oth, slot_pitch, r, ds, (size_t)n * m, m, stream);
+            t += 1;
+        }
         if (rc) return rc;
         std::swap(cur, oth);
     }

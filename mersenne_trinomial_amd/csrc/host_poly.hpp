@@ -67,6 +67,8 @@ void xor_shift_words(u64 *a, const u64 *b, size_t bw, u64 sh);
 // device product service; 0 keeps every product on the host.
 void set_device_product_bits(u64 bits);
 u64 device_product_bits();
+// whether chains take two squarings per launch ("sq_fuse"; square.hip)
+bool square_fuse_enabled();
 Poly mul_fast(const Poly &a, const Poly &b);       // Karatsuba / device FFT
 // The 2x2 step matrix of a stretch of the remainder sequence: it maps
 // (a, b) to (m00 a + m01 b, m10 a + m11 b).

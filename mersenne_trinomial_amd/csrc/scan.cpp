@@ -66,6 +66,11 @@ struct Live {
 int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                std::vector<Live> live, std::vector<gf2t_scan_result> &rows,
                std::vector<std::string> &hexes) {
+    if (P.k0 >= rhigh) {   // nothing left to scan: the bound is at or below the sieve depth
+        for (const Live &L : live)
+            rows[L.idx].kind = limited ? GF2T_SCAN_GAVE_UP : GF2T_SCAN_PRIMITIVE;
+        return GF2T_OK;
+    }
     const size_t nw = (size_t)((r + 63) >> 6);
     const size_t pitch = (nw + 31) & ~(size_t)31;
     const int B = (int)live.size();
@@ -90,10 +95,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         std::vector<u64> host((size_t)B * pitch, 0);
         for (int i = 0; i < B; i++) host[(size_t)i * pitch] = 2;   // F = x
         SCAN_CHECK(gf2t_memcpy_h2d(cur, host.data(), host.size() * 8, nullptr));
-        for (u64 k = 0; k < P.k0; k++) {
-            SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), B, nullptr));
-            std::swap(cur, oth);
-        }
+        SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)B, 1, P.k0, nullptr));
     }
     int n = B;
     u64 k = P.k0, L = P.L0 ? P.L0 : 64;
@@ -115,11 +117,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                 SCAN_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, n, nullptr));
                 SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(),
                                             pitch, n, WORK.p, nullptr));
-                for (int t = 0; t < m; t++) {
-                    SCAN_CHECK(gf2t_square_fold_slab_g(cur, oth, pitch, r, SV.w(),
-                                                       (size_t)n * m, m, nullptr));
-                    std::swap(cur, oth);
-                }
+                SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)n * m, m,
+                                             (u64)m, nullptr));
             }
         } else {
             for (u64 deg_k = ka; deg_k <= kb; deg_k++) {
@@ -148,7 +147,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         std::vector<char> done(n, 0);
         parallel_for(hit.size(), threads, [&](size_t h) {
             int i = hit[h];
-            if (resolve_hit(g[i], ka, kb, rhigh, rows[live[i].idx], hexes[live[i].idx]))
+            if (resolve_hit(g[i], ka, kb, rhigh, rows[live[i].idx], hexes[live[i].idx],
+                            P.past_bound != 0))
                 done[i] = 1;
         });
         t0 = now_s();

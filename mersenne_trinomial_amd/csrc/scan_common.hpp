@@ -82,8 +82,11 @@ inline Poly frob_mod(const Poly &g, u64 k) {
 // interval was clear, so each irreducible factor of g has its degree in
 // [ka, kb].  Returns false when the least degree lies beyond rhigh (a
 // factor found past the bound is not reported).
+// A factor of degree beyond rhigh (found in the last whole block) is
+// reported only when past_bound is set; the range of an unresolved hit
+// ends at rhigh, or at kb with past_bound.
 inline bool resolve_hit(const Poly &g, u64 ka, u64 kb, u64 rhigh, gf2t_scan_result &R,
-                        std::string &hex) {
+                        std::string &hex, bool past_bound = false) {
     const u64 gd = (u64)gf2t_host::deg(g);
     u64 d = 0;
     Poly mask;
@@ -95,7 +98,7 @@ inline bool resolve_hit(const Poly &g, u64 ka, u64 kb, u64 rhigh, gf2t_scan_resu
     } else {
         ok = gf2t_host::resolve_least(g, ka, kb, frob_mod(g, ka - 1), d, mask);
     }
-    if (ok && d > rhigh) return false;
+    if (ok && d > rhigh && !past_bound) return false;
     if (ok) {
         R.kind = GF2T_SCAN_FOUND;
         R.d = d;
@@ -103,10 +106,32 @@ inline bool resolve_hit(const Poly &g, u64 ka, u64 kb, u64 rhigh, gf2t_scan_resu
     } else {
         R.kind = GF2T_SCAN_RANGE;
         R.rlo = ka;
-        R.rhi = std::min(kb, rhigh);
+        R.rhi = past_bound ? kb : std::min(kb, rhigh);
     }
     R.gcd_degree = gd;
     return true;
+}
+
+// Advances a slab of n residues (residue y uses ds[y / group]) by `steps`
+// squarings, ping-ponging between cur and oth (cur ends holding the
+// result).  Two squarings per launch when the residues are large enough
+// for the two-squaring tiles to be full (r >= 2^18).
+inline int slab_square_steps(u64 *&cur, u64 *&oth, size_t pitch, u64 r, const u64 *ds, size_t n,
+                             int group, u64 steps, void *stream) {
+    const bool two = r >= (1u << 18) && gf2t_host::square_fuse_enabled();
+    for (u64 t = 0; t < steps;) {
+        int rc;
+        if (two && steps - t >= 2) {
+            rc = gf2t_square2_fold_slab_g(cur, oth, pitch, r, ds, n, group, stream);
+            t += 2;
+        } else {
+            rc = gf2t_square_fold_slab_g(cur, oth, pitch, r, ds, n, group, stream);
+            t += 1;
+        }
+        if (rc) return rc;
+        std::swap(cur, oth);
+    }
+    return GF2T_OK;
 }
 
 // degrees per product in the blocked scheme.  Per degree the scheme costs
@@ -116,7 +141,7 @@ inline bool resolve_hit(const Poly &g, u64 ka, u64 kb, u64 rhigh, gf2t_scan_resu
 // 4.7 TB/s and slab product times of ~5 us and ~8 ms per residue).
 inline int block_for(int requested, u64 r) {
     int m = requested > 0 ? requested : (r < (1u << 14) ? 4 : r < (1u << 19) ? 16 : 32);
-    if (m > 63) m = 63;
+    if (m > 64) m = 64;
     while (m > 1 && 2 * (u64)m >= r) m /= 2;
     return m;
 }

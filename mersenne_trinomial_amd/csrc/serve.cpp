@@ -224,11 +224,8 @@ struct gf2t_server {
         SV_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, n, nullptr));
         SV_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(), pitch, n,
                                   WORK.p, nullptr));
-        for (int t = 0; t < m; t++) {
-            SV_CHECK(gf2t_square_fold_slab_g(cur, oth, pitch, r, SV.w(), (size_t)n * m, m,
-                                             nullptr));
-            std::swap(cur, oth);
-        }
+        SV_CHECK(gf2t_scan::slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)n * m, m,
+                                              (u64)m, nullptr));
         for (auto &q : live) q.kdone += (u64)m;
         return GF2T_OK;
     }

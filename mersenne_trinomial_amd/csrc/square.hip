@@ -180,6 +180,26 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold(
     square2_tile<false>(in, out, c, blockIdx.x, gw);
 }
 
+// slab of n residues (pitch words apart), residue y with s = svec[y / group],
+// advanced by two squarings: the h-table step of the blocked DDF taken
+// twice in one pass.  Tiles beyond the residue exit before any barrier.
+__device__ __forceinline__ u64 uniform_u64(u64 v) {
+    return ((u64)__builtin_amdgcn_readfirstlane((u32)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((u32)v);
+}
+__global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_slab_g(
+        const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
+        const u64 *__restrict__ svec, unsigned n, unsigned group) {
+    __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
+    const size_t np = (size_t)(((r + 63) >> 6) + 1) / 2;
+    if ((size_t)blockIdx.x * SQ2_TP >= np) return;
+    for (unsigned y = blockIdx.y; y < n; y += gridDim.y) {
+        const SqConst c = sq_const(r, uniform_u64(svec[y / group]));
+        square2_tile<false>(in + (u64)y * pitch, out + (u64)y * pitch, c, blockIdx.x, gw);
+        __syncthreads();   // the next residue's first phase rewrites gw
+    }
+}
+
 // slab: residue y at in + y*pitch, its s in svec[y]
 __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_slab(
         const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
@@ -350,6 +370,10 @@ void launch_batch(unsigned gx, unsigned active, const SqDesc *tab, hipStream_t s
                            0, st, tab);
 }
 }  // namespace
+
+namespace gf2t_host {
+bool square_fuse_enabled() { return sq_fuse(); }
+}
 
 extern "C" int gf2t_tune(const char *key, int64_t value) {
     if (!key) return GF2T_EINVAL;
@@ -579,5 +603,26 @@ extern "C" int gf2t_stream_copy(void *dst, const void *src, size_t bytes, void *
         hipLaunchKernelGGL(k_stream_copy<false>, dim3(blocks ? blocks : 1), dim3(256), 0,
                            (hipStream_t)stream, (const ulonglong2 *)src, (ulonglong2 *)dst, n);
     GF2T_CHECK_LAUNCH("k_stream_copy");
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_square2_fold_slab_g(const uint64_t *din, uint64_t *dout, size_t pitch,
+                                        uint64_t r, const uint64_t *ds, size_t n, int group,
+                                        void *stream) {
+    size_t nw = words_of(r);
+    if (!din || !dout || !ds || din == dout || r < 5 || r >= (1ULL << 34) || pitch < nw ||
+        (pitch & 1) || (((uintptr_t)din | (uintptr_t)dout) & 15) || group < 1 ||
+        n > 0xFFFFFFFFu) {
+        gf2t::set_error("gf2t_square2_fold_slab_g: bad arguments (even pitch, 16-byte "
+                        "aligned slabs)");
+        return GF2T_EINVAL;
+    }
+    if (n == 0) return GF2T_OK;
+    const unsigned gx = (unsigned)(((nw + 1) / 2 + SQ2_TP - 1) / SQ2_TP);
+    const unsigned gy = n < 65535 ? (unsigned)n : 65535u;
+    hipLaunchKernelGGL(k_square2_fold_slab_g, dim3(gx, gy), dim3(SQ_THREADS), 0,
+                       (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
+                       (unsigned)group);
+    GF2T_CHECK_LAUNCH("k_square2_fold_slab_g");
     return GF2T_OK;
 }

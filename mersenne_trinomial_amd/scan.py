@@ -46,12 +46,14 @@ def swan_bound(r: int, s: int):
 
 def scan(r: int, svals, k0: int, maxd: int = 0, L0: int = 64, Lmax: int = 4096,
          gcd_threads: int = 0, batch: int = 0, verbose: int = 0,
-         block_m: int = 0) -> list[ScanOut]:
-    """Scan every s in svals (sieve survivors at depth k0)."""
+         block_m: int = 0, past_bound: bool = False) -> list[ScanOut]:
+    """Scan every s in svals (sieve survivors at depth k0).  past_bound:
+    report a factor found in the last whole block beyond the degree bound
+    (tsfactor's semantics) instead of 'u' / 'primitive'."""
     init()
     sv = np.ascontiguousarray(np.asarray(svals, dtype=np.uint64))
     P = ScanParams(k0=k0, maxd=maxd, L0=L0, Lmax=Lmax, gcd_threads=gcd_threads, batch=batch,
-                   verbose=verbose, block_m=block_m)
+                   verbose=verbose, block_m=block_m, past_bound=int(bool(past_bound)))
     res = C.c_void_p()
     check(lib.gf2t_scan_run(r, sv.ctypes.data_as(C.POINTER(C.c_uint64)), sv.size, C.byref(P),
                             C.byref(res)), "gf2t_scan_run")

@@ -43,7 +43,8 @@ def upload(device, words_list, pitch):
 
 
 @pytest.mark.parametrize("r,m,kstart", [(4423, 1, 0), (4423, 4, 5), (4423, 7, 33),
-                                        (132049, 16, 21), (9689, 63, 2)])
+                                        (132049, 16, 21), (9689, 63, 2), (9689, 64, 3),
+                                        (300007, 4, 9)])
 def test_init_and_horner_vs_oracle(dev, r, m, kstart):
     device, lib = dev
     svals = [1, 2, 63, 64, 65, (r - 1) // 2, r // 3][: 5 if r > 100000 else 7]
@@ -76,6 +77,31 @@ def test_init_and_horner_vs_oracle(dev, r, m, kstart):
     for i, s in enumerate(svals):
         for j in range(m):
             assert np.array_equal(h2[i, j, :nw], O.square_fold(h[i, j, :nw], r, s))
+
+
+@pytest.mark.parametrize("r,group", [(4423, 1), (300007, 2), (756839, 3)])
+def test_two_squarings_on_a_slab(dev, r, group):
+    """gf2t_square2_fold_slab_g = two squarings, residue y with s[y / group]"""
+    device, lib = dev
+    svals = [1, 3, min(4000, (r - 1) // 4), (r - 1) // 2 - 5, (r - 1) // 2]   # 2s < r
+    n = len(svals) * group
+    nw = O.words(r)
+    pitch = (nw + 31) & ~31
+    rng = np.random.default_rng(r)
+    a = [rng.integers(0, 2**64, nw, dtype=np.uint64) for _ in range(n)]
+    for v in a:
+        if r & 63:
+            v[-1] &= np.uint64((1 << (r & 63)) - 1)
+    A = upload(device, a, pitch)
+    B = device.DeviceBuffer.words(n * pitch)
+    S = device.DeviceBuffer.from_array(np.array(svals, np.uint64))
+    assert lib.gf2t_square2_fold_slab_g(A.ptr, B.ptr, pitch, r, S.ptr, n, group, None) == 0
+    got = B.download().reshape(n, pitch)
+    for y in range(n):
+        s = svals[y // group]
+        assert np.array_equal(got[y, :nw], O.square_fold_steps(a[y], r, s, 2)), (y, s)
+    # odd pitch: rejected (the kernel stores 16-byte pairs)
+    assert lib.gf2t_square2_fold_slab_g(A.ptr, B.ptr, pitch + 1, r, S.ptr, 1, 1, None) != 0
 
 
 def test_slab_helpers(dev):
