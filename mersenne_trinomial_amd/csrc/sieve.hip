@@ -16,8 +16,9 @@
 // Per degree, on the device:
 //   k_log_build   L[g^i] = i for all i < M (LFSR chain per lane, random
 //                 4-byte scatter: the HBM-bound part, 4*2^d bytes)
-//   k_lyndon      one lane per candidate j, 64-lane ballot compaction of
-//                 the class representatives into a dense list
+//   k_lyndon_t    a run of 16 candidates j per thread, all rotations
+//                 compared, one output reservation per block: the class
+//                 representatives as a dense list
 //   k_zech        one lane per class: Zech log (one random gather) and the
 //                 progression; classes whose progression reaches s <= smax
 //                 are ballot-compacted into a queue (about 1.6% of them at
@@ -220,6 +221,49 @@ __global__ void k_log_build(DegreeParams P, u32 *__restrict__ L) {
     }
 }
 
+// exclusive prefix sum of v over the block (blockDim.x = T), and the total;
+// wtot: T/64 words of LDS, free again after the caller's next barrier
+template <int T>
+__device__ unsigned block_excl_scan(unsigned v, unsigned *total, unsigned *wtot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wtot[w] = x;
+    __syncthreads();
+    unsigned before = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < T / 64; i++) {
+        const unsigned t = wtot[i];
+        if (i < w) before += t;
+        tot += t;
+    }
+    *total = tot;
+    return before + x - v;
+}
+
+// off[b] = sum of cnt[0..b) for the nb <= 8*T buckets of a block-level
+// counting sort, all threads taking part (each sums a run of buckets,
+// then one block scan); the caller waits at a barrier before reading off
+template <int T>
+__device__ void bucket_offsets(const unsigned *cnt, unsigned *off, unsigned nb, unsigned *wtot) {
+    const unsigned per = (nb + T - 1) / T;
+    const unsigned b0 = threadIdx.x * per;
+    unsigned v = 0;
+    for (unsigned i = 0; i < per; i++)
+        if (b0 + i < nb) v += cnt[b0 + i];
+    unsigned total;
+    unsigned acc = block_excl_scan<T>(v, &total, wtot);
+    for (unsigned i = 0; i < per; i++)
+        if (b0 + i < nb) {
+            off[b0 + i] = acc;
+            acc += cnt[b0 + i];
+        }
+}
+
 // ---- log table by radix partition (d >= LP_MIN_D) --------------------
 // The direct build above scatters 4-byte stores over 4*2^d bytes.  For
 // large d the table is built in three coalesced passes instead:
@@ -235,7 +279,7 @@ __global__ void k_log_build(DegreeParams P, u32 *__restrict__ L) {
 constexpr int LP_MIN_D = 26;
 constexpr int LP_SB = 15;            // 2^15 values per placed slice (128 KiB)
 constexpr int LP_E = 8192;           // entries per block in passes 1 and 2
-constexpr int LP_T = 256;            // threads per block in passes 1 and 2
+constexpr int LP_T = 512;            // threads per block in passes 1 and 2
 
 struct LpShape {
     int B1, B2;                      // partition bits, level 1 and 2
@@ -245,7 +289,7 @@ __global__ __launch_bounds__(LP_T) void k_lp_part1(DegreeParams P, LpShape S,
                                                    unsigned long long *__restrict__ out,
                                                    unsigned *__restrict__ cursor,
                                                    unsigned *__restrict__ bad) {
-    __shared__ unsigned cnt[256], off[256], gbase[256];
+    __shared__ unsigned cnt[256], off[256], gbase[256], wtot[LP_T / 64];
     __shared__ unsigned long long stage[LP_E];
     const unsigned nb = 1u << S.B1;
     const int shift = P.d - S.B1;
@@ -253,7 +297,7 @@ __global__ __launch_bounds__(LP_T) void k_lp_part1(DegreeParams P, LpShape S,
     const u64 i0 = (u64)blockIdx.x * LP_E;
     for (unsigned b = threadIdx.x; b < nb; b += LP_T) cnt[b] = 0;
     __syncthreads();
-    // each thread: 32 consecutive exponents by the LFSR step a <- a*g
+    // each thread: LP_E / LP_T consecutive exponents by the LFSR step a <- a*g
     constexpr int PER = LP_E / LP_T;
     const u64 my0 = i0 + (u64)threadIdx.x * PER;
     unsigned rank[PER];
@@ -269,10 +313,7 @@ __global__ __launch_bounds__(LP_T) void k_lp_part1(DegreeParams P, LpShape S,
         a = (a << 1) ^ ((a & top) ? qlow : 0u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned acc = 0;
-        for (unsigned b = 0; b < nb; b++) { off[b] = acc; acc += cnt[b]; }
-    }
+    bucket_offsets<LP_T>(cnt, off, nb, wtot);
     __syncthreads();
     for (unsigned b = threadIdx.x; b < nb; b += LP_T)
         gbase[b] = cnt[b] ? atomicAdd(&cursor[b], cnt[b]) : 0;
@@ -299,7 +340,7 @@ __global__ __launch_bounds__(LP_T) void k_lp_part2(DegreeParams P, LpShape S,
                                                    unsigned long long *__restrict__ out,
                                                    unsigned *__restrict__ cursor,
                                                    unsigned *__restrict__ bad) {
-    __shared__ unsigned cnt[512], off[512], gbase[512];
+    __shared__ unsigned cnt[512], off[512], gbase[512], wtot[LP_T / 64];
     __shared__ unsigned long long stage[LP_E];
     const unsigned nb2 = 1u << S.B2;
     const u64 cap1 = (u64)1 << (P.d - S.B1);
@@ -322,10 +363,7 @@ __global__ __launch_bounds__(LP_T) void k_lp_part2(DegreeParams P, LpShape S,
         rank[k] = idx < nvalid ? atomicAdd(&cnt[(unsigned)(ev[k] >> (32 + LP_SB)) & (nb2 - 1)], 1u) : 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned acc = 0;
-        for (unsigned b = 0; b < nb2; b++) { off[b] = acc; acc += cnt[b]; }
-    }
+    bucket_offsets<LP_T>(cnt, off, nb2, wtot);
     __syncthreads();
     for (unsigned b = threadIdx.x; b < nb2; b += LP_T)
         gbase[b] = cnt[b] ? atomicAdd(&cursor[(u64)b1 * nb2 + b], cnt[b]) : 0;
@@ -376,29 +414,57 @@ LpShape lp_shape(int d) {
 
 // A Lyndon word of length >= 2 starts with 0 and ends with 1, so only the
 // odd j below 2^(d-1) are candidates: candidate c tests j = 2c + 1,
-// c < 2^(d-2).
-__global__ void k_lyndon(int d, u64 M, u32 *__restrict__ surv, unsigned long long *__restrict__ nsurv,
-                         u64 cap) {
-    const int lane = threadIdx.x & 63;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
+// c < 2^(d-2).  Each thread tests a run of LY_PER consecutive candidates
+// against all d-1 rotations (no early exit: about one candidate in d/4 is
+// a Lyndon word, so nearly every wave runs all rotations anyway; the
+// LY_PER independent chains keep the SIMD busy), the block sums its
+// counts, and one atomic per block reserves the block's output run.
+// Survivors stay ascending within each block's run.
+constexpr int LY_T = 256, LY_PER = 16;
+
+template <class W>
+__global__ __launch_bounds__(LY_T) void k_lyndon_t(int d, W M, W *__restrict__ surv,
+                                                   unsigned long long *__restrict__ nsurv,
+                                                   u64 cap) {
+    __shared__ unsigned wtot[LY_T / 64];
+    __shared__ unsigned long long bbase;
     const u64 ncand = (u64)1 << (d - 2);
-    const u32 M32 = (u32)M;
-    // wave-uniform loop: every lane of a wave takes part in each ballot
-    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < ncand;
-         wbase += stride) {
-        u64 c = wbase + lane;
-        u32 j = (u32)(2 * c + 1);
-        bool keep = c < ncand && is_lyndon32(j, d, M32);
-        // 64-lane ballot compaction: one atomic per wave
-        unsigned long long m = __ballot(keep);
-        unsigned long long base = 0;
-        if (lane == 0 && m) base = atomicAdd(nsurv, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (keep) {
-            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
-            if (pos < cap) surv[pos] = j;
+    const u64 chunk = (u64)LY_T * LY_PER;
+    for (u64 c0 = (u64)blockIdx.x * chunk; c0 < ncand; c0 += (u64)gridDim.x * chunk) {
+        const u64 cb = c0 + (u64)threadIdx.x * LY_PER;
+        W jv[LY_PER];
+        bool ok[LY_PER];
+#pragma unroll
+        for (int i = 0; i < LY_PER; i++) {
+            jv[i] = (W)(2 * (cb + i) + 1);
+            ok[i] = cb + i < ncand;
         }
+        for (int k = 1; k < d; k++) {
+#pragma unroll
+            for (int i = 0; i < LY_PER; i++)
+                ok[i] = ok[i] && ((((jv[i] << k) | (jv[i] >> (d - k))) & M) > jv[i]);
+        }
+        unsigned keep = 0;
+#pragma unroll
+        for (int i = 0; i < LY_PER; i++) keep |= (unsigned)ok[i] << i;
+        unsigned total;
+        const unsigned pre = block_excl_scan<LY_T>((unsigned)__popc(keep), &total, wtot);
+        if (threadIdx.x == 0) bbase = total ? atomicAdd(nsurv, (unsigned long long)total) : 0;
+        __syncthreads();
+        u64 pos = bbase + pre;
+#pragma unroll
+        for (int i = 0; i < LY_PER; i++)
+            if ((keep >> i) & 1) {
+                if (pos < cap) surv[pos] = jv[i];
+                pos++;
+            }
+        __syncthreads();   // bbase and wtot are rewritten by the next chunk
     }
+}
+
+unsigned lyndon_blocks(int d) {
+    const u64 chunks = ((((u64)1 << (d - 2)) + (u64)LY_T * LY_PER - 1) / ((u64)LY_T * LY_PER));
+    return (unsigned)std::min<u64>(chunks, 8192);
 }
 
 struct MarkRec {
@@ -536,20 +602,10 @@ GF2T_DEV u64 antilog_w(const WParams &P, u64 e) {
     return fmul64(P.A1[e >> W_AB], P.A0[e & ((1u << W_AB) - 1)], P.q, P.d);
 }
 
-// block-wide exclusive scan of cnt[0..nb) (nb <= W_T * 16) into off
+// block-wide exclusive scan of cnt[0..nb) (nb <= W_T * 16) into off;
+// part: W_T / 64 words of LDS
 __device__ void block_scan(const unsigned *cnt, unsigned *off, unsigned nb, unsigned *part) {
-    const unsigned per = (nb + W_T - 1) / W_T;
-    unsigned t = threadIdx.x, sum = 0;
-    for (unsigned k = t * per; k < nb && k < (t + 1) * per; k++) sum += cnt[k];
-    part[t] = sum;
-    __syncthreads();
-    if (t == 0) {
-        unsigned acc = 0;
-        for (unsigned k = 0; k < W_T; k++) { unsigned v = part[k]; part[k] = acc; acc += v; }
-    }
-    __syncthreads();
-    unsigned acc = part[t];
-    for (unsigned k = t * per; k < nb && k < (t + 1) * per; k++) { off[k] = acc; acc += cnt[k]; }
+    bucket_offsets<W_T>(cnt, off, nb, part);
     __syncthreads();
 }
 
@@ -669,27 +725,6 @@ __global__ __launch_bounds__(1024) void k_w_place(const unsigned long long *__re
     __syncthreads();
     for (u64 k = threadIdx.x; k < cap; k += blockDim.x) Llo[z * cap + k] = lo[k];
     for (u64 k = threadIdx.x; k < cap / 16; k += blockDim.x) Lhi[z * (cap / 16) + k] = hi[k];
-}
-
-__global__ void k_lyndon_w(int d, u64 M, u64 *__restrict__ surv, unsigned long long *__restrict__ nsurv,
-                           u64 cap) {
-    const int lane = threadIdx.x & 63;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    const u64 ncand = (u64)1 << (d - 2);
-    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < ncand;
-         wbase += stride) {
-        u64 c = wbase + lane;
-        u64 j = 2 * c + 1;
-        bool keep = c < ncand && is_lyndon64(j, d, M);
-        unsigned long long m = __ballot(keep);
-        unsigned long long base = 0;
-        if (lane == 0 && m) base = atomicAdd(nsurv, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (keep) {
-            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
-            if (pos < cap) surv[pos] = j;
-        }
-    }
 }
 
 struct MarkRecW {
@@ -978,9 +1013,8 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         }
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
         GF2T_HIP(hipEventRecord(ev[1], 0));
-        u64 lyn_blocks = std::min<u64>((((u64)1 << (d - 2)) + 255) / 256, 1u << 20);
-        hipLaunchKernelGGL(k_lyndon, dim3((unsigned)lyn_blocks), dim3(256), 0, 0, d, M, surv,
-                           cnt64, (u64)(ncls + 64));
+        hipLaunchKernelGGL(k_lyndon_t<u32>, dim3(lyndon_blocks(d)), dim3(LY_T), 0, 0, d, (u32)M,
+                           surv, cnt64, (u64)(ncls + 64));
         GF2T_CHECK_LAUNCH("k_lyndon");
         unsigned long long hs[2];
         GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
@@ -1136,8 +1170,7 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         }
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
         GF2T_HIP(hipEventRecord(ev[1], 0));
-        const u64 lyn_blocks = std::min<u64>((((u64)1 << (d - 2)) + 255) / 256, 1u << 20);
-        hipLaunchKernelGGL(k_lyndon_w, dim3((unsigned)lyn_blocks), dim3(256), 0, 0, d, M, surv,
+        hipLaunchKernelGGL(k_lyndon_t<u64>, dim3(lyndon_blocks(d)), dim3(LY_T), 0, 0, d, M, surv,
                            cnt64, (u64)(ncls + 64));
         GF2T_CHECK_LAUNCH("k_lyndon_w");
         unsigned long long hs[2];
