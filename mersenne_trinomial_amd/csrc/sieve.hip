@@ -475,41 +475,65 @@ struct MarkRec {
 // progression s == s0 (mod n), and -- only for the few classes whose
 // progression reaches s <= smax -- a ballot-compacted queue entry, so the
 // expensive minimal polynomial below runs on dense waves.
-__global__ void k_zech(DegreeParams P, const u32 *__restrict__ surv, u64 nsurv,
-                       const u32 *__restrict__ L, MarkRec *__restrict__ queue,
-                       unsigned long long *__restrict__ nqueue, u64 qcap) {
-    const int lane = threadIdx.x & 63;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
+constexpr int ZC_T = 256, ZC_PER = 4;   // classes per thread per round
+
+__global__ __launch_bounds__(ZC_T) void k_zech(DegreeParams P, const u32 *__restrict__ surv,
+                                               u64 nsurv, const u32 *__restrict__ L,
+                                               MarkRec *__restrict__ queue,
+                                               unsigned long long *__restrict__ nqueue, u64 qcap) {
+    __shared__ unsigned wtot[ZC_T / 64];
+    __shared__ unsigned long long bbase;
     const u32 M = (u32)P.M;
-    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < nsurv;
-         wbase += stride) {
-        u64 t = wbase + lane;
-        bool keep = false;
-        MarkRec rec{0, 0, 0, 0};
-        if (t < nsurv) {
-            u32 j = surv[t];
-            u64 u = mod_mersenne((u64)j * P.rmod, P.d, P.M);
-            if (u) {                                   // u = 0: beta^r = 1, never a root
-                u32 v = L[antilog(P, u) ^ 1u];         // Zech logarithm of u
-                u32 gg = gcd32(j, M);
-                if (v % gg == 0) {
-                    u32 n = M / gg;
-                    u32 iv = inv32(j / gg, n);
-                    u64 prod = (u64)iv * (v / gg);
-                    u32 s0 = (u32)(gg == 1 ? mod_mersenne(prod, P.d, P.M) : prod % n);
-                    u32 sf = s0 ? s0 : n;
-                    if (sf <= P.smax) { keep = true; rec = MarkRec{j, n, sf, 0}; }
-                }
+    const u64 chunk = (u64)ZC_T * ZC_PER;
+    // a round: ZC_PER classes per thread, their table lookups all issued
+    // before any of the arithmetic, then one queue reservation per block
+    for (u64 c0 = (u64)blockIdx.x * chunk; c0 < nsurv; c0 += (u64)gridDim.x * chunk) {
+        u32 j[ZC_PER], a[ZC_PER], v[ZC_PER];
+        bool live[ZC_PER];
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) {
+            const u64 t = c0 + (u64)q * ZC_T + threadIdx.x;
+            live[q] = t < nsurv;
+            j[q] = live[q] ? surv[t] : 1u;
+        }
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) {
+            const u64 u = mod_mersenne((u64)j[q] * P.rmod, P.d, P.M);
+            live[q] = live[q] && u;                    // u = 0: beta^r = 1, never a root
+            a[q] = antilog(P, u ? u : 1) ^ 1u;
+        }
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) v[q] = live[q] ? L[a[q]] : 0u;   // Zech logarithms of u
+        unsigned keep = 0;
+        MarkRec rec[ZC_PER];
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) {
+            rec[q] = MarkRec{0, 0, 0, 0};
+            if (!live[q]) continue;
+            const u32 gg = gcd32(j[q], M);
+            if (v[q] % gg) continue;
+            const u32 n = M / gg;
+            const u32 iv = inv32(j[q] / gg, n);
+            const u64 prod = (u64)iv * (v[q] / gg);
+            const u32 s0 = (u32)(gg == 1 ? mod_mersenne(prod, P.d, P.M) : prod % n);
+            const u32 sf = s0 ? s0 : n;
+            if (sf <= P.smax) {
+                keep |= 1u << q;
+                rec[q] = MarkRec{j[q], n, sf, 0};
             }
         }
-        unsigned long long m = __ballot(keep);
-        unsigned long long base = 0;
-        if (lane == 0 && m) base = atomicAdd(nqueue, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (keep) {
-            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
-            if (pos < qcap) queue[pos] = rec;
-        }
+        unsigned total;
+        const unsigned pre = block_excl_scan<ZC_T>((unsigned)__popc(keep), &total, wtot);
+        if (threadIdx.x == 0) bbase = total ? atomicAdd(nqueue, (unsigned long long)total) : 0;
+        __syncthreads();
+        u64 pos = bbase + pre;
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++)
+            if ((keep >> q) & 1) {
+                if (pos < qcap) queue[pos] = rec[q];
+                pos++;
+            }
+        __syncthreads();   // bbase and wtot are rewritten by the next round
     }
 }
 
@@ -1027,8 +1051,9 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         }
         MarkRec *queue = mem.queue;
         if (nsurv) {
-            u64 zb = std::min<u64>((nsurv + 255) / 256, 1u << 18);
-            hipLaunchKernelGGL(k_zech, dim3((unsigned)zb), dim3(256), 0, 0, P, surv, nsurv, L,
+            const u64 zb = std::min<u64>((nsurv + (u64)ZC_T * ZC_PER - 1) / ((u64)ZC_T * ZC_PER),
+                                         1u << 16);
+            hipLaunchKernelGGL(k_zech, dim3((unsigned)zb), dim3(ZC_T), 0, 0, P, surv, nsurv, L,
                                queue, cnt64 + 1, (u64)(ncls + 64));
             GF2T_CHECK_LAUNCH("k_zech");
             GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
