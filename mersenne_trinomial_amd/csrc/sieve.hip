@@ -278,8 +278,8 @@ __device__ void bucket_offsets(const unsigned *cnt, unsigned *off, unsigned nb, 
 // (g^i mod 2^(d-B1)) << 32 | i.
 constexpr int LP_MIN_D = 26;
 constexpr int LP_SB = 15;            // 2^15 values per placed slice (128 KiB)
-constexpr int LP_E = 8192;           // entries per block in passes 1 and 2
-constexpr int LP_T = 512;            // threads per block in passes 1 and 2
+constexpr int LP_E = 16384;          // entries per block in passes 1 and 2
+constexpr int LP_T = 1024;           // threads per block in passes 1 and 2
 
 struct LpShape {
     int B1, B2;                      // partition bits, level 1 and 2
@@ -755,41 +755,65 @@ struct MarkRecW {
     u64 j, n, sf;
 };
 
-__global__ void k_zech_w(WParams P, const u64 *__restrict__ surv, u64 nsurv,
-                         const u32 *__restrict__ Llo, const u32 *__restrict__ Lhi,
-                         MarkRecW *__restrict__ queue, unsigned long long *__restrict__ nqueue,
-                         u64 qcap) {
-    const int lane = threadIdx.x & 63;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 wbase = (u64)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < nsurv;
-         wbase += stride) {
-        u64 t = wbase + lane;
-        bool keep = false;
-        MarkRecW rec{0, 0, 0};
-        if (t < nsurv) {
-            const u64 j = surv[t];
-            const u64 u = mulmod64(j, P.rmod, P.M);
-            if (u) {
-                const u64 y = antilog_w(P, u) ^ 1u;
-                const u64 v = (u64)Llo[y] | ((u64)((Lhi[y >> 4] >> (2 * (y & 15))) & 3) << 32);
-                const u64 gg = gcd64(j, P.M);
-                if (v % gg == 0) {
-                    const u64 n = P.M / gg;
-                    const u64 iv = inv64((j / gg) % n, n);
-                    const u64 s0 = mulmod64(iv, v / gg, n);
-                    const u64 sf = s0 ? s0 : n;
-                    if (sf <= P.smax) { keep = true; rec = MarkRecW{j, n, sf}; }
-                }
+// as k_zech: ZC_PER classes per thread, lookups first, one reservation per block
+__global__ __launch_bounds__(ZC_T) void k_zech_w(WParams P, const u64 *__restrict__ surv,
+                                                 u64 nsurv, const u32 *__restrict__ Llo,
+                                                 const u32 *__restrict__ Lhi,
+                                                 MarkRecW *__restrict__ queue,
+                                                 unsigned long long *__restrict__ nqueue,
+                                                 u64 qcap) {
+    __shared__ unsigned wtot[ZC_T / 64];
+    __shared__ unsigned long long bbase;
+    const u64 chunk = (u64)ZC_T * ZC_PER;
+    for (u64 c0 = (u64)blockIdx.x * chunk; c0 < nsurv; c0 += (u64)gridDim.x * chunk) {
+        u64 j[ZC_PER], y[ZC_PER], v[ZC_PER];
+        bool live[ZC_PER];
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) {
+            const u64 t = c0 + (u64)q * ZC_T + threadIdx.x;
+            live[q] = t < nsurv;
+            j[q] = live[q] ? surv[t] : 1;
+        }
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) {
+            const u64 u = mulmod64(j[q], P.rmod, P.M);
+            live[q] = live[q] && u;                    // u = 0: beta^r = 1, never a root
+            y[q] = antilog_w(P, u ? u : 1) ^ 1u;
+        }
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++)
+            v[q] = live[q] ? ((u64)Llo[y[q]] |
+                              ((u64)((Lhi[y[q] >> 4] >> (2 * (y[q] & 15))) & 3) << 32))
+                           : 0;
+        unsigned keep = 0;
+        MarkRecW rec[ZC_PER];
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++) {
+            rec[q] = MarkRecW{0, 0, 0};
+            if (!live[q]) continue;
+            const u64 gg = gcd64(j[q], P.M);
+            if (v[q] % gg) continue;
+            const u64 n = P.M / gg;
+            const u64 iv = inv64((j[q] / gg) % n, n);
+            const u64 s0 = mulmod64(iv, v[q] / gg, n);
+            const u64 sf = s0 ? s0 : n;
+            if (sf <= P.smax) {
+                keep |= 1u << q;
+                rec[q] = MarkRecW{j[q], n, sf};
             }
         }
-        unsigned long long m = __ballot(keep);
-        unsigned long long base = 0;
-        if (lane == 0 && m) base = atomicAdd(nqueue, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0);
-        if (keep) {
-            u64 pos = base + __popcll(m & ((1ULL << lane) - 1));
-            if (pos < qcap) queue[pos] = rec;
-        }
+        unsigned total;
+        const unsigned pre = block_excl_scan<ZC_T>((unsigned)__popc(keep), &total, wtot);
+        if (threadIdx.x == 0) bbase = total ? atomicAdd(nqueue, (unsigned long long)total) : 0;
+        __syncthreads();
+        u64 pos = bbase + pre;
+#pragma unroll
+        for (int q = 0; q < ZC_PER; q++)
+            if ((keep >> q) & 1) {
+                if (pos < qcap) queue[pos] = rec[q];
+                pos++;
+            }
+        __syncthreads();
     }
 }
 
@@ -1207,8 +1231,9 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             status = GF2T_EINTERNAL;
             break;
         }
-        const u64 zb = std::min<u64>((nsurv + 255) / 256, 1u << 18);
-        hipLaunchKernelGGL(k_zech_w, dim3((unsigned)zb), dim3(256), 0, 0, P, surv, nsurv, Llo, Lhi,
+        const u64 zb = std::min<u64>((nsurv + (u64)ZC_T * ZC_PER - 1) / ((u64)ZC_T * ZC_PER),
+                                     1u << 16);
+        hipLaunchKernelGGL(k_zech_w, dim3((unsigned)zb), dim3(ZC_T), 0, 0, P, surv, nsurv, Llo, Lhi,
                            queue, cnt64 + 1, qcap);
         GF2T_CHECK_LAUNCH("k_zech_w");
         GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
