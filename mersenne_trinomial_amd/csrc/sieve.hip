@@ -604,10 +604,11 @@ struct DevBuf {
 // Elements of GF(2^d) are u64.  The log table holds d-bit exponents, so
 // it is stored as the low 32 bits (L_lo, u32) plus the top two bits
 // packed sixteen to a word (L_hi).  It is built by the same radix
-// partition as above, except that the level-2 pass runs in place, one
-// level-1 bucket at a time through a bucket-sized scratch region, so the
-// whole build needs 8 bytes per element plus the table (201 GB at d = 34,
-// inside 288 GB).  Entries are (g^i mod 2^(d-8)) << 36 | i.
+// partition as above, except that the level-2 split runs one level-1
+// bucket at a time, to a bucket-sized scratch region and back (two
+// counting-sort passes), so the whole build needs 8 bytes per element
+// plus the table (201 GB at d = 34, inside 288 GB).  Entries are
+// (g^i mod 2^(d-8)) << 36 | i.
 constexpr int WIDE_MAX_D = 34;
 constexpr int W_SB = 15;            // values per placed slice: 2^15
 constexpr int W_E = 8192;           // entries per block in passes 1 and 2
@@ -680,50 +681,58 @@ __global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *
     }
 }
 
-// level 2 for one level-1 bucket: entries in[0..count) (a copy of the
-// bucket), sorted by bits [W_SB, d - 8) of the stored value into
-// out[sub * 2^W_SB + pos]
-__global__ __launch_bounds__(W_T) void k_w_part2(int B2, const unsigned long long *__restrict__ in,
-                                                 u64 count, unsigned long long *__restrict__ out,
-                                                 unsigned *__restrict__ cursor,
+// One counting-sort pass of the level-2 split, which runs as two such
+// passes per level-1 bucket (about half the d - 23 level-2 bits each, so
+// every block writes runs of 128-256 entries): source bucket blockIdx.y
+// (src_cap entries apart; the first holds first_count) is partitioned by
+// the nbits-bit digit at bit `shift` of the stored value into
+// destination buckets src * 2^nbits + digit of dst_cap entries each.
+constexpr int WS_MAXB = 64;   // digits per pass
+constexpr int WS_T = 512;     // threads per block
+__global__ __launch_bounds__(WS_T) void k_w_split(const unsigned long long *__restrict__ in,
+                                                 u64 src_cap, u64 first_count, int shift,
+                                                 int nbits, unsigned long long *__restrict__ out,
+                                                 u64 dst_cap, unsigned *__restrict__ cursor,
                                                  unsigned *__restrict__ bad) {
-    __shared__ unsigned cnt[2048], off[2048], gbase[2048], part[W_T];
+    __shared__ unsigned cnt[WS_MAXB], off[WS_MAXB], gbase[WS_MAXB], part[WS_T / 64];
     __shared__ unsigned long long stage[W_E];
-    const unsigned nb2 = 1u << B2;
-    const u64 cap2 = (u64)1 << W_SB;
+    const unsigned nd = 1u << nbits;
+    const unsigned src = blockIdx.y;
+    const u64 count = src ? src_cap : first_count;
     const u64 k0 = (u64)blockIdx.x * W_E;
     if (k0 >= count) return;
     const u64 nvalid = (k0 + W_E <= count) ? W_E : count - k0;
-    for (unsigned b = threadIdx.x; b < nb2; b += W_T) cnt[b] = 0;
+    const unsigned long long *sp = in + (u64)src * src_cap + k0;
+    const int dsh = W_IB + shift;
+    for (unsigned b = threadIdx.x; b < nd; b += WS_T) cnt[b] = 0;
     __syncthreads();
-    constexpr int PER = W_E / W_T;
+    constexpr int PER = W_E / WS_T;
     unsigned rank[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        u64 idx = (u64)k * W_T + threadIdx.x;
-        rank[k] = idx < nvalid
-                      ? atomicAdd(&cnt[(unsigned)(in[k0 + idx] >> (W_IB + W_SB)) & (nb2 - 1)], 1u)
-                      : 0;
+        const u64 idx = (u64)k * WS_T + threadIdx.x;
+        rank[k] = idx < nvalid ? atomicAdd(&cnt[(unsigned)(sp[idx] >> dsh) & (nd - 1)], 1u) : 0;
     }
     __syncthreads();
-    block_scan(cnt, off, nb2, part);
-    for (unsigned b = threadIdx.x; b < nb2; b += W_T)
-        gbase[b] = cnt[b] ? atomicAdd(&cursor[b], cnt[b]) : 0;
+    bucket_offsets<WS_T>(cnt, off, nd, part);
+    __syncthreads();
+    for (unsigned b = threadIdx.x; b < nd; b += WS_T)
+        gbase[b] = cnt[b] ? atomicAdd(&cursor[src * nd + b], cnt[b]) : 0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        u64 idx = (u64)k * W_T + threadIdx.x;
+        const u64 idx = (u64)k * WS_T + threadIdx.x;
         if (idx < nvalid) {
-            unsigned long long e = in[k0 + idx];
-            stage[off[(unsigned)(e >> (W_IB + W_SB)) & (nb2 - 1)] + rank[k]] = e;
+            const unsigned long long e = sp[idx];
+            stage[off[(unsigned)(e >> dsh) & (nd - 1)] + rank[k]] = e;
         }
     }
     __syncthreads();
-    for (u64 k = threadIdx.x; k < nvalid; k += W_T) {
-        unsigned long long e = stage[k];
-        unsigned b = (unsigned)(e >> (W_IB + W_SB)) & (nb2 - 1);
-        u64 pos = (u64)gbase[b] + (k - off[b]);
-        if (pos >= cap2) { atomicOr(bad, 4u); continue; }
-        out[(u64)b * cap2 + pos] = e;
+    for (u64 k = threadIdx.x; k < nvalid; k += WS_T) {
+        const unsigned long long e = stage[k];
+        const unsigned b = (unsigned)(e >> dsh) & (nd - 1);
+        const u64 pos = (u64)gbase[b] + (k - off[b]);
+        if (pos >= dst_cap) { atomicOr(bad, 4u); continue; }
+        out[((u64)src * nd + b) * dst_cap + pos] = e;
     }
 }
 
@@ -1189,24 +1198,29 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         MarkRecW *queue = (MarkRecW *)W.wQueue.p;
 
         GF2T_HIP(hipEventRecord(ev[0], 0));
-        // log table: partition pass 1, in-place pass 2 per bucket, placement
+        // log table: partition pass 1, level-2 split per bucket, placement
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur1, (u64)256);
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
         hipLaunchKernelGGL(k_w_part1, dim3((unsigned)((M + W_E - 1) / W_E)), dim3(W_T), 0, 0, P,
                            P1, cur1, cnt32 + 1);
         GF2T_CHECK_LAUNCH("k_w_part1");
-        const int B2 = d - W_B1 - W_SB;
-        const u64 cap1 = (u64)1 << (d - W_B1);
+        // level 2 per level-1 bucket: bucket -> scratch by the high half of
+        // the level-2 bits, scratch -> bucket by the low half
+        const int B2 = d - W_B1 - W_SB, B2a = (B2 + 1) / 2, B2b = B2 - B2a;
+        const u64 cap1 = (u64)1 << (d - W_B1), capA = cap1 >> B2a;
         for (unsigned b1 = 0; b1 < 256; b1++) {
-            const u64 count = b1 ? cap1 : cap1 - 1;
-            GF2T_HIP(hipMemcpyAsync(tmp, P1 + (u64)b1 * cap1, count * 8,
-                                    hipMemcpyDeviceToDevice, 0));
+            const u64 count = b1 ? cap1 : cap1 - 1;   // value 0 never occurs
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur2, (u64)(1u << B2a));
+            hipLaunchKernelGGL(k_w_split, dim3((unsigned)((count + W_E - 1) / W_E), 1), dim3(WS_T),
+                               0, 0, P1 + (u64)b1 * cap1, cap1, count, W_SB + B2b, B2a, tmp, capA,
+                               cur2, cnt32 + 1);
             hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((1u << B2) + 255) / 256)), dim3(256), 0,
                                0, cur2, (u64)(1u << B2));
-            hipLaunchKernelGGL(k_w_part2, dim3((unsigned)((count + W_E - 1) / W_E)), dim3(W_T), 0,
-                               0, B2, tmp, count, P1 + (u64)b1 * cap1, cur2, cnt32 + 1);
+            hipLaunchKernelGGL(k_w_split, dim3((unsigned)((capA + W_E - 1) / W_E), 1u << B2a),
+                               dim3(WS_T), 0, 0, tmp, capA, b1 ? capA : capA - 1, W_SB, B2b,
+                               P1 + (u64)b1 * cap1, (u64)1 << W_SB, cur2, cnt32 + 1);
         }
-        GF2T_CHECK_LAUNCH("k_w_part2");
+        GF2T_CHECK_LAUNCH("k_w_split");
         hipLaunchKernelGGL(k_w_place, dim3((unsigned)((u64)1 << (d - W_SB))), dim3(1024),
                            (size_t)place_lds, 0, P1, Llo, Lhi);
         GF2T_CHECK_LAUNCH("k_w_place");
