@@ -424,7 +424,7 @@ def chain_launches(steps: int, fuse: bool):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(the deep scan shards by independent s values, the sieve by class ranges "
               "with one elementwise-min exchange of best[])", file=sys.stderr)
         sys.exit(2)

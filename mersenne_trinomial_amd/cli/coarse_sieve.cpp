@@ -98,7 +98,7 @@ int main(int argc, char **argv) {
     }
     if (min_depth < 2) min_depth = 2;
     if (depth > 34) {
-        fprintf(stderr, "depth %d: this build sieves degrees <= 34\n", depth);
+        fprintf(stderr, "depth %d: degrees above 34 are not supported\n", depth);
         return 1;
     }
 
