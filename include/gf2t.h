@@ -294,6 +294,8 @@ uint64_t gf2t_primitive_poly(int d);
 #define GF2T_SCAN_GAVE_UP 2    /* "s u": maxd reached below the Swan bound  */
 #define GF2T_SCAN_PRIMITIVE 3  /* "s primitive": no factor up to the bound  */
 #define GF2T_SCAN_RANGE 4      /* "s rA-B": factor in [A,B], unresolved     */
+#define GF2T_SCAN_STOPPED 5    /* the progress callback stopped the scan
+                                  before this s finished (no line)          */
 
 typedef struct gf2t_scan_params {
     uint64_t k0;      /* depth the coarse sieve reached (no factor <= k0) */
@@ -311,6 +313,16 @@ typedef struct gf2t_scan_params {
                          bound, as tsfactor does (its intervals always end
                          on block boundaries, src/tsfactor.cu:1266-1273);
                          0 = report only degrees <= the bound           */
+    /* Called after every interval of a cohort with the degree k_done
+     * that each of the n listed s values is now known to be clear
+     * through (no factor of degree <= k_done), the s values still being
+     * scanned.  A scan restarted for them with k0 = k_done continues
+     * exactly where this one is (the checkpoint of the reference's
+     * tsfactor, src/tsfactor.cu:717-828, reduced to what resuming
+     * needs).  Returning nonzero stops the scan: every s not finished
+     * then gets GF2T_SCAN_STOPPED.  May be null. */
+    int (*progress)(void *user, uint64_t k_done, const uint64_t *svals, size_t n);
+    void *progress_user;
 } gf2t_scan_params;
 
 typedef struct gf2t_scan_result {

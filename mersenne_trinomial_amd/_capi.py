@@ -30,7 +30,8 @@ class GF2TError(RuntimeError):
 class ScanParams(C.Structure):
     _fields_ = [("k0", _u64), ("maxd", _u64), ("L0", _u64), ("Lmax", _u64),
                 ("gcd_threads", C.c_int), ("batch", C.c_int), ("verbose", C.c_int),
-                ("block_m", C.c_int), ("past_bound", C.c_int)]
+                ("block_m", C.c_int), ("past_bound", C.c_int),
+                ("progress", C.c_void_p), ("progress_user", C.c_void_p)]
 
 
 class ScanResult(C.Structure):

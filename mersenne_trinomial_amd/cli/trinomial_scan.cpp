@@ -11,12 +11,17 @@
 //     s rA-B        a factor with degree in [A, B], unresolved
 //     s u           no factor of degree <= maxd
 //     s primitive   maxd = 0 and no factor up to the Swan bound
-// Every survivor of a cohort advances together on the device, so the
-// reference's --state per-s checkpoint has no equivalent: an interrupted
-// run resumes at the first cohort with unwritten results.
+// Every survivor of a cohort advances together on the device.  Instead of
+// the reference's per-s --state files, <out_prefix>.gf2t.ckpt records the
+// degree through which the s values of the running cohort are clear (every
+// --ckpt-mins minutes, and on SIGINT / SIGTERM, after which the run writes
+// the results it has and exits with status 3); rerunning the same command
+// resumes them from there.
 #include "gf2t.h"
 #include "gf2t_engine.hpp"
+#include "scan_ckpt.hpp"
 #include <chrono>
+#include <map>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -32,27 +37,25 @@ static double now_s() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// lines of the finished s values (an s the progress callback stopped
+// gets none), in the order of sv
 static int scan_lines(u64 r, const std::vector<u64> &sv, const gf2t_scan_params &P,
                       std::vector<std::string> &lines) {
     gf2t_scan_results *res = nullptr;
     int rc = gf2t_scan_run(r, sv.data(), sv.size(), &P, &res);
     if (rc != GF2T_OK) return rc;
-    char buf[4096];
     lines.clear();
     for (size_t i = 0; i < gf2t_scan_results_count(res); i++) {
-        std::vector<char> big;
-        if (gf2t_scan_line(res, i, buf, sizeof buf) == GF2T_OK) {
-            lines.push_back(buf);
-        } else {
-            gf2t_scan_result row;
-            gf2t_scan_results_get(res, i, &row, nullptr, 0);
-            big.resize(row.hex_len + 64);
-            gf2t_scan_line(res, i, big.data(), big.size());
-            lines.push_back(big.data());
-        }
+        gf2t_scan_result row;
+        gf2t_scan_results_get(res, i, &row, nullptr, 0);
+        if (row.kind == GF2T_SCAN_STOPPED) continue;
+        std::vector<char> buf(row.hex_len + 96);
+        rc = gf2t_scan_line(res, i, buf.data(), buf.size());
+        if (rc != GF2T_OK) break;
+        lines.push_back(buf.data());
     }
     gf2t_scan_results_free(res);
-    return GF2T_OK;
+    return rc;
 }
 
 static int selftest() {
@@ -109,7 +112,7 @@ int main(int argc, char **argv) {
         fprintf(stderr,
                 "usage: %s <r> <survivors.txt> <k0> <maxd> <out_prefix>\n"
                 "          [--interval Lmax] [--l0 L0] [--gcd-threads N] [--batch B]\n"
-                "          [--verbose 0..3]\n"
+                "          [--verbose 0..3] [--ckpt-mins N] [--no-ckpt]\n"
                 "       %s --selftest\n"
                 "k0 must be the depth the coarse sieve actually reached; maxd 0 scans\n"
                 "to the Swan bound.\n", argv[0], argv[0]);
@@ -124,8 +127,15 @@ int main(int argc, char **argv) {
     P.Lmax = 4096;
     P.gcd_threads = 8;
     std::string prefix = argv[5];
+    double ckpt_mins = 15;
+    bool no_ckpt = false;
+    long stop_after = -1;
     for (int i = 6; i < argc; i++) {
-        if (!strcmp(argv[i], "--interval") && i + 1 < argc) P.Lmax = strtoull(argv[++i], 0, 10);
+        if (!strcmp(argv[i], "--ckpt-mins") && i + 1 < argc) ckpt_mins = atof(argv[++i]);
+        else if (!strcmp(argv[i], "--no-ckpt")) no_ckpt = true;
+        else if (!strcmp(argv[i], "--stop-after-intervals") && i + 1 < argc)   // test hook
+            stop_after = atol(argv[++i]);
+        else if (!strcmp(argv[i], "--interval") && i + 1 < argc) P.Lmax = strtoull(argv[++i], 0, 10);
         else if (!strcmp(argv[i], "--l0") && i + 1 < argc) P.L0 = strtoull(argv[++i], 0, 10);
         else if (!strcmp(argv[i], "--gcd-threads") && i + 1 < argc) P.gcd_threads = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--batch") && i + 1 < argc) P.batch = atoi(argv[++i]);
@@ -172,14 +182,56 @@ int main(int argc, char **argv) {
             ": %zu survivors (%zu already done), %d gcd threads\n",
             r, P.k0, P.maxd, surv.size(), surv.size() - todo.size(), P.gcd_threads);
     gf2t::check(gf2t_init(0), "gf2t_init");
+    // checkpoint: s values of an interrupted cohort resume from the degree
+    // they were clear through
+    scan_ckpt::Writer W;
+    W.p.r = r;
+    W.p.skip = P.k0;
+    W.p.m = 1;
+    W.p.maxd = P.maxd;
+    W.every_s = ckpt_mins * 60.0;
+    W.stop_after = stop_after;
+    W.verbose = P.verbose;
+    W.last = scan_ckpt::Writer::now();
+    if (!no_ckpt) W.path = prefix + ".gf2t.ckpt";
+    std::map<u64, u64> resume;   // s -> degree it is clear through
+    {
+        scan_ckpt::Ckpt c;
+        if (!W.path.empty() && scan_ckpt::load(W.path, c)) {
+            if (c.p == W.p) {
+                for (u64 s : c.s)
+                    if (!done.count(s)) resume[s] = c.k_done;
+                fprintf(stderr, "resuming %zu s from %s (clear through degree %" PRIu64 ")\n",
+                        resume.size(), W.path.c_str(), c.k_done);
+            } else {
+                fprintf(stderr, "checkpoint %s is for other parameters; ignored\n", W.path.c_str());
+            }
+        }
+    }
+    P.progress = &scan_ckpt::Writer::cb;
+    P.progress_user = &W;
+    scan_ckpt::install_signals();
     // results are flushed a chunk at a time so an interrupted run resumes
     const size_t chunk = P.batch > 0 ? (size_t)P.batch : 4096;
     double t0 = now_s();
     u64 nfound = 0, nu = 0, nrange = 0, nprim = 0;
-    for (size_t at = 0; at < todo.size(); at += chunk) {
+    for (size_t at = 0; at < todo.size() && !W.stopped; at += chunk) {
         std::vector<u64> sv(todo.begin() + at, todo.begin() + std::min(todo.size(), at + chunk));
+        // resumed s values first (by their degree), then the rest from k0
+        std::map<u64, std::vector<u64>> by_k0;
+        for (u64 s : sv) {
+            auto it = resume.find(s);
+            by_k0[it == resume.end() ? P.k0 : std::max(P.k0, it->second)].push_back(s);
+        }
         std::vector<std::string> lines;
-        gf2t::check(scan_lines(r, sv, P, lines), "gf2t_scan_run");
+        for (auto &g : by_k0) {
+            if (W.stopped) break;
+            gf2t_scan_params Q = P;
+            Q.k0 = g.first;
+            std::vector<std::string> part;
+            gf2t::check(scan_lines(r, g.second, Q, part), "gf2t_scan_run");
+            lines.insert(lines.end(), part.begin(), part.end());
+        }
         for (const std::string &ln : lines) {
             fprintf(fres, "%s\n", ln.c_str());
             if (ln.find(" primitive") != std::string::npos) nprim++;
@@ -191,6 +243,13 @@ int main(int argc, char **argv) {
         fprintf(stderr, "%zu/%zu scanned (%.1fs)\n", std::min(todo.size(), at + chunk), todo.size(),
                 now_s() - t0);
     }
+    if (W.stopped) {
+        fclose(fres);
+        fprintf(stderr, "interrupted: results so far in %s; run the same command to resume\n",
+                rpath.c_str());
+        return 3;
+    }
+    if (!W.path.empty()) remove(W.path.c_str());
     fclose(fres);
     fprintf(stderr, "done: %" PRIu64 " found, %" PRIu64 " ranges, %" PRIu64 " u, %" PRIu64
             " primitive, %.1fs total\n", nfound, nrange, nu, nprim, now_s() - t0);

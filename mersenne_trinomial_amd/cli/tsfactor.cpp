@@ -17,11 +17,14 @@
 // reference reports it.  -z gives up where the reference's interval
 // schedule (linear, or --sched opt) first exceeds Q blocks; -Z, --q0 and
 // the schedule otherwise only shape the intervals, which do not change
-// any line.  The per-s TSFC checkpoint is replaced by cohort-sized
-// result flushes: an interrupted run resumes at the first cohort with
-// unwritten results.
+// any line.  The per-s TSFC checkpoint (<out>.ckpt, not read here) is
+// replaced by <out>.gf2t.ckpt: the degree through which the s values of
+// the running cohort are clear, written every --ckpt-mins minutes and on
+// SIGINT / SIGTERM (the run then writes the results it has and exits with
+// status 3); rerunning the same command resumes them from there.
 #include "gf2t.h"
 #include "gf2t_engine.hpp"
+#include "scan_ckpt.hpp"
 #include <chrono>
 #include <cinttypes>
 #include <cmath>
@@ -29,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <set>
 #include <string>
 #include <vector>
@@ -60,10 +64,10 @@ static void usage() {
             "                  run on the GPU (default 2^18)\n"
             "  --legacy-sq     one squaring per launch\n"
             "  --batch B       s values per device cohort (default: by memory)\n"
+            "  --ckpt-mins N   checkpoint cadence (default 15); --no-ckpt: none\n"
             "  -v, -vv         verbose\n"
-            "Accepted for compatibility, no effect here: --no-ckpt, --ckpt-mins N,\n"
-            "  --canzass-max D, --pend-max N, --gpu-gcd, --allow-no-ntl,\n"
-            "  --selftest-reps N.\n");
+            "Accepted for compatibility, no effect here: --canzass-max D, --pend-max N,\n"
+            "  --gpu-gcd, --allow-no-ntl, --selftest-reps N.\n");
     exit(1);
 }
 
@@ -168,6 +172,10 @@ static int scan_lines(u64 r, const std::vector<u64> &sv, const gf2t_scan_params 
     for (size_t i = 0; i < gf2t_scan_results_count(res); i++) {
         gf2t_scan_result row;
         gf2t_scan_results_get(res, i, &row, nullptr, 0);
+        if (row.kind == GF2T_SCAN_STOPPED) {   // the checkpoint holds it
+            lines.push_back(std::string());
+            continue;
+        }
         std::vector<char> buf(row.hex_len + 96);
         rc = gf2t_scan_line(res, i, buf.data(), buf.size());
         if (rc != GF2T_OK) break;
@@ -179,14 +187,17 @@ static int scan_lines(u64 r, const std::vector<u64> &sv, const gf2t_scan_params 
 
 // One line per s of sv, in order: s values are grouped by the bound the
 // reference would scan them to (Swan bound or --maxd, cut short by -z).
+// s values in `resume` start from the degree recorded there; an empty
+// line is an s the progress callback stopped.
 static int tsfactor_lines(u64 r, const std::vector<u64> &sv, const Opts &O,
+                          const std::map<u64, u64> &resume, scan_ckpt::Writer *W,
                           std::vector<std::string> &out) {
     struct Group {
-        u64 maxd;
+        u64 k0, maxd;
         bool past;
         std::vector<size_t> idx;
     };
-    std::map<std::pair<u64, int>, Group> groups;
+    std::map<std::tuple<u64, u64, int>, Group> groups;
     out.assign(sv.size(), std::string());
     for (size_t i = 0; i < sv.size(); i++) {
         int swan = 0;
@@ -202,15 +213,23 @@ static int tsfactor_lines(u64 r, const std::vector<u64> &sv, const Opts &O,
         // cut: degrees up to last (a block boundary), then "u";
         // otherwise the bound with the last block's overshoot
         const u64 maxd = cut ? last : (limited ? O.maxd : 0);
-        Group &G = groups[{maxd, cut ? 0 : 1}];
+        auto rs = resume.find(sv[i]);
+        const u64 k0 = rs == resume.end() ? O.skip : std::max(O.skip, rs->second);
+        Group &G = groups[std::make_tuple(k0, maxd, cut ? 0 : 1)];
+        G.k0 = k0;
         G.maxd = maxd;
         G.past = !cut;
         G.idx.push_back(i);
     }
     for (auto &kv : groups) {
         Group &G = kv.second;
+        if (W && W->stopped) break;   // the rest restart from the checkpoint or --skip
         gf2t_scan_params P = {};
-        P.k0 = O.skip;
+        P.k0 = G.k0;
+        if (W) {
+            P.progress = &scan_ckpt::Writer::cb;
+            P.progress_user = W;
+        }
         P.maxd = G.maxd;
         P.L0 = 64;
         P.Lmax = 4096;
@@ -376,6 +395,9 @@ int main(int argc, char **argv) {
     bool do_selftest = false, do_bench = false, bench_gcd = false, legacy = false;
     int device = 0;
     u64 gcd_bits = 0;
+    double ckpt_mins = 15;
+    bool no_ckpt = false;
+    long stop_after = -1;
     std::string out_prefix = "ts";
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
@@ -416,10 +438,12 @@ int main(int argc, char **argv) {
         } else if (a == "--sched-rho") {
             double rho = atof(need("--sched-rho"));
             if (rho > 1.0) O.sched.rho_min = rho;
-        } else if (a == "--ckpt-mins" || a == "--canzass-max" || a == "--pend-max" ||
-                   a == "--selftest-reps")
+        } else if (a == "--ckpt-mins") ckpt_mins = atof(need("--ckpt-mins"));
+        else if (a == "--no-ckpt") no_ckpt = true;
+        else if (a == "--stop-after-intervals") stop_after = atol(need(a.c_str()));   // test hook
+        else if (a == "--canzass-max" || a == "--pend-max" || a == "--selftest-reps")
             need(a.c_str());
-        else if (a == "--no-ckpt" || a == "--gpu-gcd" || a == "--allow-no-ntl") {
+        else if (a == "--gpu-gcd" || a == "--allow-no-ntl") {
         } else if (a == "-v") O.verbose++;
         else if (a == "-vv") O.verbose += 2;
         else if (a.size() > 1 && a[0] == '-' && !isdigit((unsigned char)a[1])) {
@@ -504,8 +528,8 @@ int main(int argc, char **argv) {
     }
     if (FILE *f = fopen((out_prefix + ".ckpt").c_str(), "rb")) {
         fclose(f);
-        fprintf(stderr, "note: %s.ckpt is not read here; an unfinished s restarts from "
-                        "--skip\n", out_prefix.c_str());
+        fprintf(stderr, "note: %s.ckpt (the reference's own checkpoint) is not read here\n",
+                out_prefix.c_str());
     }
     std::vector<u64> todo;
     for (u64 s : svs)
@@ -516,6 +540,32 @@ int main(int argc, char **argv) {
     if (legacy) gf2t::check(gf2t_tune("sq_fuse", 0), "gf2t_tune");
     fprintf(stderr, "r=%" PRIu64 " skip=%" PRIu64 " m=%d q0=%ld%s%s%s\n", r, O.skip, O.m, O.q0,
             O.zq ? " -z" : "", O.Zq ? " -Z" : "", O.sched.opt ? " sched=opt" : "");
+    scan_ckpt::Writer W;
+    W.p.r = r;
+    W.p.skip = O.skip;
+    W.p.m = (u64)O.m;
+    W.p.maxd = O.maxd;
+    W.p.zq = (u64)O.zq;
+    W.every_s = ckpt_mins * 60.0;
+    W.stop_after = stop_after;
+    W.verbose = O.verbose;
+    W.last = scan_ckpt::Writer::now();
+    if (!no_ckpt) W.path = out_prefix + ".gf2t.ckpt";
+    std::map<u64, u64> resume;
+    {
+        scan_ckpt::Ckpt c;
+        if (!W.path.empty() && scan_ckpt::load(W.path, c)) {
+            if (c.p == W.p) {
+                for (u64 s : c.s)
+                    if (!done.count(s)) resume[s] = c.k_done;
+                fprintf(stderr, "resuming %zu s from %s (clear through degree %" PRIu64 ")\n",
+                        resume.size(), W.path.c_str(), c.k_done);
+            } else {
+                fprintf(stderr, "checkpoint %s is for other parameters; ignored\n", W.path.c_str());
+            }
+        }
+    }
+    scan_ckpt::install_signals();
     FILE *rf = fopen(res_path.c_str(), "a");
     if (!rf) {
         fprintf(stderr, "cannot append %s\n", res_path.c_str());
@@ -524,12 +574,13 @@ int main(int argc, char **argv) {
     const size_t chunk = O.batch > 0 ? (size_t)O.batch : 4096;
     const double t0 = now_s();
     u64 nfound = 0, nu = 0, nrange = 0, nprim = 0;
-    for (size_t at = 0; at < todo.size(); at += chunk) {
+    for (size_t at = 0; at < todo.size() && !W.stopped; at += chunk) {
         std::vector<u64> sv(todo.begin() + at, todo.begin() + std::min(todo.size(), at + chunk));
         std::vector<std::string> lines;
-        gf2t::check(tsfactor_lines(r, sv, O, lines), "tsfactor scan");
+        gf2t::check(tsfactor_lines(r, sv, O, resume, &W, lines), "tsfactor scan");
         for (size_t i = 0; i < lines.size(); i++) {
             const std::string &ln = lines[i];
+            if (ln.empty()) continue;   // stopped before it finished
             fprintf(rf, "%s\n", ln.c_str());
             printf("%s\n", ln.c_str());
             if (ln.find(" primitive") != std::string::npos) {
@@ -547,6 +598,12 @@ int main(int argc, char **argv) {
                     todo.size(), now_s() - t0);
     }
     fclose(rf);
+    if (W.stopped) {
+        fprintf(stderr, "interrupted: %" PRIu64 " found, %" PRIu64 " u so far in %s; run the "
+                        "same command to resume\n", nfound, nu, res_path.c_str());
+        return 3;
+    }
+    if (!W.path.empty()) remove(W.path.c_str());
     fprintf(stderr, "done: %" PRIu64 " found, %" PRIu64 " ranges, %" PRIu64 " u, %" PRIu64
             " primitive, %.1fs\n", nfound, nrange, nu, nprim, now_s() - t0);
     return 0;

@@ -65,7 +65,7 @@ struct Live {
 // with a host gcd(ACC, T_s) per residue.
 int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                std::vector<Live> live, std::vector<gf2t_scan_result> &rows,
-               std::vector<std::string> &hexes) {
+               std::vector<std::string> &hexes, bool &stopped) {
     if (P.k0 >= rhigh) {   // nothing left to scan: the bound is at or below the sieve depth
         for (const Live &L : live)
             rows[L.idx].kind = limited ? GF2T_SCAN_GAVE_UP : GF2T_SCAN_PRIMITIVE;
@@ -182,6 +182,15 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         n = nl;
         k = kb;
         L = std::min(2 * L, Lmax);
+        if (n > 0 && P.progress) {
+            // every s still live is clear through degree kb
+            const u64 kdone = std::min(kb, rhigh);
+            if (P.progress(P.progress_user, kdone, svals.data(), (size_t)n)) {
+                for (int i = 0; i < n; i++) rows[live[i].idx].kind = GF2T_SCAN_STOPPED;
+                stopped = true;
+                return GF2T_OK;
+            }
+        }
     }
     return GF2T_OK;
 }
@@ -241,12 +250,17 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
     if (rc) { delete res; return rc; }
     size_t cap = std::max<size_t>(1, std::min<size_t>(65535, (freeb / 2) / per));
     if (P.batch > 0) cap = std::min<size_t>(cap, (size_t)P.batch);
+    bool stopped = false;
     for (auto &G : groups)
         for (size_t off = 0; off < G.live.size(); off += cap) {
             std::vector<Live> part(G.live.begin() + off,
                                    G.live.begin() + std::min(G.live.size(), off + cap));
+            if (stopped) {
+                for (const Live &L : part) res->rows[L.idx].kind = GF2T_SCAN_STOPPED;
+                continue;
+            }
             try {
-                rc = run_cohort(r, P, G.rhigh, G.limited, part, res->rows, res->hex);
+                rc = run_cohort(r, P, G.rhigh, G.limited, part, res->rows, res->hex, stopped);
             } catch (const std::exception &e) {
                 gf2t::set_error("gf2t_scan_run: %s", e.what());
                 rc = GF2T_EHIP;

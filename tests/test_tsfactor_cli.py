@@ -121,3 +121,55 @@ def test_swan_bound_primitive(tmp_path):
     lines = (tmp_path / "w.results.txt").read_text().splitlines()
     assert {int(ln.split()[0]): ln for ln in lines} == want
     assert "PRIMITIVE TRINOMIAL CANDIDATE: s=271" in p.stderr
+
+
+def test_checkpoint_file_round_trip(tmp_path):
+    """The checkpoint file: save/load round trip, the standard CRC-32 check
+    value, and rejection of a flipped bit and of a truncated file."""
+    exe = tmp_path / "ckpt_check"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                           "-I" + os.path.join(ROOT, "mersenne_trinomial_amd", "cli"), "-o", str(exe),
+                           os.path.join(ROOT, "tests", "host", "ckpt_check.cpp")])
+    p = subprocess.run([str(exe), str(tmp_path / "x.ckpt")], capture_output=True, text=True)
+    assert p.returncode == 0 and "ckpt ok" in p.stdout, p.returncode
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog", ["tsfactor", "trinomial_scan"])
+def test_interrupted_run_resumes_from_checkpoint(tmp_path, prog):
+    """A run stopped after its first interval writes the lines it has and a
+    checkpoint (exit status 3); the same command then resumes the
+    unfinished s from the checkpoint and ends with the straight run's lines."""
+    skip, m, maxd = 8, 8, 203
+    sv = [s for s in survivors(2, 700, skip)]
+    if prog == "tsfactor":
+        last = skip + m * -(-(maxd - skip) // m)
+        args = [R, tmp_path / "surv.txt", "--skip", skip, "--maxd", maxd, "--m", m, "--out",
+                tmp_path / "c"]
+        binp = os.path.join(BIN, "tsfactor")
+    else:
+        last = maxd
+        args = [R, tmp_path / "surv.txt", skip, maxd, tmp_path / "c"]
+        binp = os.path.join(BIN, "trinomial_scan")
+    want = oracle_lines(sv, skip, last)
+    (tmp_path / "surv.txt").write_text("".join(f"{s}\n" for s in sv))
+
+    def go(*extra):
+        return subprocess.run([binp, *map(str, args), *map(str, extra)], capture_output=True,
+                              text=True, timeout=300)
+
+    p = go("--stop-after-intervals", 1)
+    assert p.returncode == 3, p.stderr
+    ck = tmp_path / "c.gf2t.ckpt"
+    assert ck.exists()
+    first = (tmp_path / "c.results.txt").read_text().splitlines()
+    assert 0 < len(first) < len(sv)
+    for ln in first:
+        assert want[int(ln.split()[0])] == ln
+    p = go()
+    assert p.returncode == 0, p.stderr
+    assert "resuming" in p.stderr and "clear through degree" in p.stderr
+    assert not ck.exists()
+    lines = (tmp_path / "c.results.txt").read_text().splitlines()
+    assert len(lines) == len(sv)
+    assert {int(ln.split()[0]): ln for ln in lines} == want
