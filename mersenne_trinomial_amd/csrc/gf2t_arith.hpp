@@ -261,30 +261,20 @@ struct SqArray {
     const u64 *p;
     size_t nw;
     GF2T_HDM u64 operator()(int, size_t k) const { return k < nw ? p[k] : 0; }
-    // 32-bit words q, q+1 (and q+2) of the input, unchecked, as one
-    // aligned 8-byte load plus one 4-byte load; qodd = q & 1, uniform
-    // over the caller's lanes.  32-bit byte offsets: residues are below
-    // 2^29 words.
-    GF2T_HDM u32 ld32(u32 q) const {
-        return *(const GF2T_GLOBAL u32 *)((const GF2T_GLOBAL char *)p + (u32)(q << 2));
+    // 32-bit words q and q + 1 of the input, unchecked, and word q + 2
+    // when sh != 0 (word q + 1 again when sh = 0, where the funnel shift
+    // does not use it, so no read passes the window's last word).  No
+    // branch on q's alignment or on sh: the loads of all the windows of
+    // a pair stay in flight together.  32-bit byte offsets: residues are
+    // below 2^29 words.
+    GF2T_HDM const GF2T_GLOBAL u32 *at(u32 q) const {
+        return (const GF2T_GLOBAL u32 *)((const GF2T_GLOBAL char *)p + (u32)(q << 2));
     }
-    GF2T_HDM u64 ld64(u32 q) const {   // q even
-        return *(const GF2T_GLOBAL u64 *)((const GF2T_GLOBAL char *)p + (u32)(q << 2));
-    }
-    GF2T_HDM void raw3(int, u32 q, u32 qodd, u32 &a, u32 &b, u32 &c) const {
-        if (qodd) {
-            a = ld32(q);
-            const u64 w = ld64(q + 1);
-            b = (u32)w; c = (u32)(w >> 32);
-        } else {
-            const u64 w = ld64(q);
-            a = (u32)w; b = (u32)(w >> 32);
-            c = ld32(q + 2);
-        }
-    }
-    GF2T_HDM void raw2(int, u32 q, u32 qodd, u32 &a, u32 &b) const {
-        if (qodd) { a = ld32(q); b = ld32(q + 1); }
-        else { const u64 w = ld64(q); a = (u32)w; b = (u32)(w >> 32); }
+    GF2T_HDM void raw3(int, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
+        const GF2T_GLOBAL u32 *w = at(q);
+        a = w[0];
+        b = w[1];
+        c = w[1 + (sh != 0)];
     }
 };
 
@@ -379,17 +369,14 @@ GF2T_HD int sq_region(u64 b, u64 b1, u64 s, u64 r) {   // bit range [b, b1)
     return 3;
 }
 // The 64 input bits from 32-bit word q on, shifted right by sh (< 32):
-// three words and two funnel shifts (two words when sh = 0, so no read
-// goes past the window's last bit).  qodd = q & 1 is the same for every
-// pair of a window.
+// three words and two funnel shifts (the third word is not read when
+// sh = 0, where alignbit returns its low operand).  qodd = q & 1 is the
+// same for every pair of a window.
 template <class Src>
 GF2T_HD u64 sq_winval(const Src &src, int win, u32 q, u32 qodd, u32 sh) {
+    (void)qodd;
     u32 a, b, c;
-    if (!sh) {
-        src.raw2(win, q, qodd, a, b);
-        return ((u64)b << 32) | a;
-    }
-    src.raw3(win, q, qodd, a, b, c);
+    src.raw3(win, q, sh, a, b, c);
     return ((u64)alignbit(c, b, sh) << 32) | alignbit(b, a, sh);
 }
 // Output pair i in region REG (0..2).  Spreading is linear and a window
@@ -531,28 +518,13 @@ struct Sq2Tile {
     GF2T_HDM u64 operator()(int k, size_t w) const {
         return win[(u32)k * SQ2_LEN + (u32)(w - base[k])];
     }
-    // 32-bit words of the copies (base_k is even, so q and the copy
-    // index have the same parity)
-    GF2T_HDM u32 ld32(int k, u32 q) const {
-        return ((const u32 *)win)[(u32)k * 2 * SQ2_LEN + (u32)(q - 2 * base[k])];
-    }
-    GF2T_HDM u64 ld64(int k, u32 q) const {   // q even
-        return win[(u32)k * SQ2_LEN + ((u32)(q - 2 * base[k]) >> 1)];
-    }
-    GF2T_HDM void raw3(int k, u32 q, u32 qodd, u32 &a, u32 &b, u32 &c) const {
-        if (qodd) {
-            a = ld32(k, q);
-            const u64 w = ld64(k, q + 1);
-            b = (u32)w; c = (u32)(w >> 32);
-        } else {
-            const u64 w = ld64(k, q);
-            a = (u32)w; b = (u32)(w >> 32);
-            c = ld32(k, q + 2);
-        }
-    }
-    GF2T_HDM void raw2(int k, u32 q, u32 qodd, u32 &a, u32 &b) const {
-        if (qodd) { a = ld32(k, q); b = ld32(k, q + 1); }
-        else { const u64 w = ld64(k, q); a = (u32)w; b = (u32)(w >> 32); }
+    // as SqArray::raw3, from the copy's 32-bit words: q, q + 1 and
+    // q + 1 + (sh != 0)
+    GF2T_HDM void raw3(int k, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
+        const u32 *w = (const u32 *)win + ((u32)k * 2 * SQ2_LEN + (u32)(q - 2 * base[k]));
+        a = w[0];
+        b = w[1];
+        c = w[1 + (sh != 0)];
     }
 };
 GF2T_HD Sq2Tile sq2_tile(const u64 *win, const SqConst &C, size_t i0) {

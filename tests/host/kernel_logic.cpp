@@ -18,13 +18,9 @@ struct CheckedTile {
         const u64 w = (*this)(k, q >> 1);
         return (q & 1) ? (u32)(w >> 32) : (u32)w;
     }
-    void raw3(int k, u32 q, u32 qodd, u32 &a, u32 &b, u32 &c) const {
-        if (qodd != (q & 1)) abort();
-        a = ld32(k, q); b = ld32(k, q + 1); c = ld32(k, q + 2);
-    }
-    void raw2(int k, u32 q, u32 qodd, u32 &a, u32 &b) const {
-        if (qodd != (q & 1)) abort();
-        a = ld32(k, q); b = ld32(k, q + 1);
+    void raw3(int k, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
+        if (sh > 31) abort();
+        a = ld32(k, q); b = ld32(k, q + 1); c = ld32(k, q + 1 + (sh != 0));
     }
 };
 

@@ -80,6 +80,20 @@ def test_two_squarings_every_s(eng, r):
         assert np.array_equal(eng.square_fold(a, r, s, 2), O.square_fold_steps(a, r, s, 2)), s
 
 
+@pytest.mark.parametrize("r", [1048573, 2000003])
+def test_two_squarings_large_s(eng, r):
+    """One two-squaring launch for s from about 128 tiles' worth of pairs
+    (s ~ 65000, where the window offsets off2 - off3 pass one tile width)
+    up to (r - 1)/2, at both parities, against the oracle."""
+    rng = np.random.default_rng(r + 1)
+    h = (r - 1) // 2
+    svals = {64900, 65023, 65024, 65025, 65151, 65152, 65153, 65279, 65280, 130000, h - 1, h}
+    svals |= {int(x) for x in rng.integers(65000, h, 40)}
+    for s in sorted(svals):
+        a = rand_res(rng, r)
+        assert np.array_equal(eng.square_fold(a, r, s, 2), O.square_fold_steps(a, r, s, 2)), s
+
+
 @pytest.fixture(params=[(1, 0, 0, 0), (2, 0, 0, 0), (4, 0, 1, 0), (1, 1, 0, 0), (2, 1, 1, 0),
                         (4, 1, 0, 0), (2, 0, 1, 1), (2, 0, 0, 1), (1, 1, 1, 1)],
                 ids=lambda p: f"ppt{p[0]}-order{p[1]}-nt{p[2]}-fuse{p[3]}")
@@ -144,6 +158,32 @@ def test_batch_full_size(eng, tuned):
                                      kk.ctypes.data_as(P), n, None))
     for s, k, a, b in zip(svals, steps, ins, bufs):
         assert np.array_equal(b.download(nw), O.square_fold_steps(a, R136, s, k)), s
+
+
+def test_batch_full_size_two_squarings(eng):
+    """One batched two-squaring launch at full size over residues whose s
+    put the points s and 2s at very different tiles (s from 474 to
+    (r - 1)/2), every residue against the oracle."""
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    svals = [474, 65100, 1234567, 30000001, 45678901, 68139919, 68139920]
+    nw = O.words(R136)
+    ins = [rand_res(rng, R136) for _ in svals]
+    bufs = [DeviceBuffer.from_array(a) for a in ins]
+    tmps = [DeviceBuffer.words(nw) for _ in svals]
+    n = len(svals)
+    P = C.POINTER(C.c_uint64)
+    rr = np.full(n, R136, dtype=np.uint64)
+    ss = np.array(svals, dtype=np.uint64)
+    kk = np.full(n, 2, dtype=np.uint64)
+    check(lib.gf2t_square_fold_batch((C.c_void_p * n)(*[b.ptr for b in bufs]),
+                                     (C.c_void_p * n)(*[t.ptr for t in tmps]),
+                                     rr.ctypes.data_as(P), ss.ctypes.data_as(P),
+                                     kk.ctypes.data_as(P), n, None))
+    for s, a, b in zip(svals, ins, bufs):
+        assert np.array_equal(b.download(nw), O.square_fold_steps(a, R136, s, 2)), s
 
 
 @pytest.mark.parametrize("r,s", [(4423, 1200), (4097, 1), (4097, 2048), (131, 65), (132049, 777)])
