@@ -324,10 +324,9 @@ GF2T_HD void square_fold_pair(const u64 *in, size_t nw, u64 r, u64 s, size_t i, 
 // shifts the spread left by one.  Q_k, and so the word offset, the bit
 // shift and the parity, are the same for every pair of one residue.
 struct SqWin {
-    u64 off;      // 64-bit word offset of Q_k (tile planning)
-    u32 off32;    // 32-bit word offset of Q_k
+    u32 off32;    // 32-bit word offset of Q_k (r < 2^34, so it fits)
     u32 sh;       // Q_k mod 32
-    u32 odd;
+    GF2T_HDM u32 off() const { return off32 >> 1; }   // 64-bit word offset
 };
 struct SqConst {
     u64 r, s;
@@ -337,10 +336,8 @@ struct SqConst {
 GF2T_HD SqWin sq_win(u64 P0) {   // P0 = window offset in spread positions (b = 0)
     u64 Q = (P0 + 1) >> 1;
     SqWin w;
-    w.off = Q >> 6;
     w.off32 = (u32)(Q >> 5);
     w.sh = (u32)(Q & 31);
-    w.odd = (u32)(P0 & 1);
     return w;
 }
 GF2T_HD SqConst sq_const(u64 r, u64 s) {
@@ -489,8 +486,8 @@ constexpr int SQ2_WINS = 5;
 constexpr u32 SQ2_TP = 510;
 constexpr u32 SQ2_GP = 256;          // G pairs per window
 constexpr u32 SQ2_LEN = 2 * SQ2_GP;  // G words per window copy
-GF2T_HD u64 sq2_off(const SqConst &C, int k) {   // word offset of window k
-    return k == 0 ? 0 : k == 1 ? C.w2.off : k == 2 ? C.w3.off : k == 3 ? C.w4.off : C.w5.off;
+GF2T_HD u32 sq2_off(const SqConst &C, int k) {   // word offset of window k
+    return k == 0 ? 0 : k == 1 ? C.w2.off() : k == 2 ? C.w3.off() : k == 3 ? C.w4.off() : C.w5.off();
 }
 GF2T_HD u32 sq2_need(int region) {   // windows read by a tile of this region
     return region == 0 ? 0x0Bu : region == 1 ? 0x17u : region == 2 ? 0x07u : 0x1Fu;
@@ -511,28 +508,166 @@ GF2T_HD void sq2_g_pair(const u64 *F, const SqConst &C, u64 p0, u32 j, u64 &o0, 
     else sq2_g_pair_clipped(F, C, p0 + j, o0, o1);
 }
 // the H phase reads G through the tile's window copies: window k's copy
-// holds G words from base_k = 2 p0_k on
+// holds G words from base_k = 2 p0_k on, at word at_k of the LDS area
 struct Sq2Tile {
     const u64 *win;
-    u64 base[SQ2_WINS];
+    u32 base[SQ2_WINS];
+    u32 at[SQ2_WINS];
     GF2T_HDM u64 operator()(int k, size_t w) const {
-        return win[(u32)k * SQ2_LEN + (u32)(w - base[k])];
+        return win[at[k] + (u32)(w - base[k])];
     }
     // as SqArray::raw3, from the copy's 32-bit words: q, q + 1 and
     // q + 1 + (sh != 0)
     GF2T_HDM void raw3(int k, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
-        const u32 *w = (const u32 *)win + ((u32)k * 2 * SQ2_LEN + (u32)(q - 2 * base[k]));
+        const u32 *w = (const u32 *)win + (2 * at[k] + (u32)(q - 2 * base[k]));
         a = w[0];
         b = w[1];
         c = w[1 + (sh != 0)];
     }
 };
-GF2T_HD Sq2Tile sq2_tile(const u64 *win, const SqConst &C, size_t i0) {
+// what the G phase of one tile computes, and where each window's copy
+// lives: compute = windows formed from F; every needed window not in
+// compute is a copy the block already holds
+struct Sq2Plan {
+    u32 compute;
+    u32 base[SQ2_WINS];
+    u32 at[SQ2_WINS];
+};
+// a stand-alone tile: every needed window computed, window k in slot k
+GF2T_HD Sq2Plan sq2_plan(const SqConst &C, size_t i0, u32 need) {
+    Sq2Plan p;
+    p.compute = need;
+#pragma unroll
+    for (int k = 0; k < SQ2_WINS; k++) {
+        p.base[k] = (u32)(2 * sq2_p0(C, i0, k));
+        p.at[k] = (u32)k * SQ2_LEN;
+    }
+    return p;
+}
+GF2T_HD Sq2Tile sq2_tile(const u64 *win, const Sq2Plan &p) {
     Sq2Tile t;
     t.win = win;
 #pragma unroll
-    for (int k = 0; k < SQ2_WINS; k++) t.base[k] = 2 * sq2_p0(C, i0, k);
+    for (int k = 0; k < SQ2_WINS; k++) {
+        t.base[k] = p.base[k];
+        t.at[k] = p.at[k];
+    }
     return t;
+}
+GF2T_HD Sq2Tile sq2_tile(const u64 *win, const SqConst &C, size_t i0) {
+    return sq2_tile(win, sq2_plan(C, i0, 0x1Fu));
+}
+
+// ---- chained tiles of the two-squaring pass
+// A tile of output pairs [i0, i0 + width) reads window k's G words
+// [i0 + off_k, i0 + off_k + width].  With delta = off2 - off3 (words),
+// window 3 of the tile at i0 + delta reads exactly the words window 2 of
+// the tile at i0 read (same base), and window 5 of the tile at i0 +
+// delta reads window 4's words of the tile at i0 shifted by
+// off5 - off4 + delta, a word or two at most.  So the output pairs are
+// cut into work units: a unit is up to SQ2_CHAIN tiles i0, i0 + delta,
+// i0 + 2 delta, ... of one column, and its block keeps the previous
+// tile's window-2 and window-4 copies in LDS as the current tile's
+// windows 3 and 5.  Chained tiles are at most SQ2_CW = SQ2_TP - 2 pairs
+// wide, so one 512-word copy of window 4 can cover both its own words
+// and the next tile's window 5 (the copy's base is placed for both;
+// where it cannot cover, window 5 is formed).  Each period of delta
+// pairs is split into ncols columns of equal width.  In [2s, r) a tile
+// then forms windows 1 and 2 only, in [s, 2s) likewise, where plain
+// tiles form three and four.  For delta < SQ2_CW (s below about
+// 128 SQ2_CW) units are the plain consecutive tiles of SQ2_TP pairs.
+// LDS slots: 0 window 1; 1, 2 windows 2 (current) and 3 (the previous
+// window 2), alternating from tile to tile; 3, 4 likewise windows 4, 5.
+constexpr u32 SQ2_CHAIN = 8;
+constexpr u32 SQ2_CW = SQ2_TP - 2;
+// pair and word indices are below 2^28 (r < 2^34), so all of it is u32.
+// The layout is computed once per residue on the host (sq2_chain, with
+// its divisions); a block derives its tiles from it with one 32-bit
+// division.
+struct Sq2Chain {
+    u32 np;       // output pairs
+    u32 period;   // distance between consecutive tiles of a unit
+    u32 ncols;    // columns per period
+    u32 len;      // tiles per unit
+    u32 chained;
+    u32 wlo, wrem;   // column widths: wlo + 1 for the first wrem columns, wlo after
+    u32 q, rm;       // np = q period + rm
+};
+GF2T_HD u32 sq2_col_start(const Sq2Chain &h, u32 c) {
+    return c * h.wlo + (c < h.wrem ? c : h.wrem);
+}
+// tiles of column c: start + j period < np for j < q, and j = q too when
+// the column starts below rm
+GF2T_HD u32 sq2_col_tiles(const Sq2Chain &h, u32 c) {
+    return h.q + (sq2_col_start(h, c) < h.rm ? 1u : 0u);
+}
+inline Sq2Chain sq2_chain(const SqConst &C) {   // host
+    Sq2Chain h;
+    h.np = (u32)((C.nw + 1) / 2);
+    const u32 d = C.w2.off() - C.w3.off();
+    h.chained = d >= SQ2_CW;
+    h.period = h.chained ? d : SQ2_TP;
+    h.ncols = h.chained ? (d + SQ2_CW - 1) / SQ2_CW : 1u;
+    h.len = h.chained ? SQ2_CHAIN : 1u;
+    h.wlo = h.period / h.ncols;
+    h.wrem = h.period % h.ncols;
+    h.q = h.np / h.period;
+    h.rm = h.np % h.period;
+    return h;
+}
+// units of one residue (column 0 has the most tiles)
+inline u64 sq2_units(const Sq2Chain &h) {
+    return (u64)h.ncols * ((sq2_col_tiles(h, 0) + h.len - 1) / h.len);
+}
+// the tiles [j0, j1) of unit u, in its column: tile j covers the pairs
+// [i0, i1) with i0 = start + j period, i1 = min(i0 + width, np);
+// false for a unit past the residue's last
+struct Sq2Unit {
+    u32 start, width, j0, j1;
+};
+GF2T_HD bool sq2_unit(const Sq2Chain &h, u32 u, Sq2Unit &w) {
+    const u32 piece = h.ncols == 1 ? u : u / h.ncols;
+    const u32 c = u - piece * h.ncols;
+    const u32 nt = sq2_col_tiles(h, c);
+    w.start = sq2_col_start(h, c);
+    w.width = h.wlo + (c < h.wrem ? 1u : 0u);
+    w.j0 = piece * h.len;
+    w.j1 = w.j0 + h.len < nt ? w.j0 + h.len : nt;
+    return w.j0 < nt;
+}
+// plan of the unit's tile number t (t = j - j0) at i0, `width` pairs of
+// its column wide, needing the windows `need`; the previous tile of the
+// unit needed prev_need and held its window 4 from word prev_base4 on
+GF2T_HD Sq2Plan sq2_chain_plan(const SqConst &C, const Sq2Chain &h, u32 i0, u32 width,
+                               u32 need, u32 t, u32 prev_need, u32 prev_base4) {
+    Sq2Plan p;
+    const u32 cur = t & 1, prv = cur ^ 1u;
+#pragma unroll
+    for (int k = 0; k < SQ2_WINS; k++) p.base[k] = (i0 + sq2_off(C, k)) & ~1u;
+    p.at[0] = 0;
+    p.at[1] = (1 + cur) * SQ2_LEN;
+    p.at[2] = (1 + prv) * SQ2_LEN;
+    p.at[3] = (3 + cur) * SQ2_LEN;
+    p.at[4] = (3 + prv) * SQ2_LEN;
+    p.compute = need;
+    if (!h.chained) return p;
+    const u32 off4 = sq2_off(C, 3), off5 = sq2_off(C, 4);
+    if (need & 8u) {   // window 4: one copy for its words and the next tile's window 5
+        const u32 own = i0 + off4, nxt = i0 + h.period + off5;
+        const u32 lo = own < nxt ? own : nxt, hi = own < nxt ? nxt : own;
+        if (hi + width < (lo & ~1u) + SQ2_LEN) p.base[3] = lo & ~1u;
+    }
+    if (t > 0) {
+        // every tile forms window 2; window 3 has the same base
+        if (need & 4u) p.compute &= ~4u;
+        const u32 lo = i0 + off5;
+        if ((need & 16u) && (prev_need & 8u) && lo >= prev_base4 &&
+            lo + width < prev_base4 + SQ2_LEN) {
+            p.compute &= ~16u;
+            p.base[4] = prev_base4;
+        }
+    }
+    return p;
 }
 
 // ---- XCD-aware visiting order of the output tiles of one residue
@@ -578,7 +713,7 @@ GF2T_HD int64_t tile_of(const TileOrder &o, u64 x) {
     return tile < o.T ? (int64_t)tile : -1;
 }
 GF2T_HD u64 sq_delta_pairs(const SqConst &c) {
-    return c.w2.off >= c.w3.off ? c.w2.off - c.w3.off : 0;
+    return c.w2.off() >= c.w3.off() ? c.w2.off() - c.w3.off() : 0;
 }
 
 // 64-bit window of p (pw words) at bit pos, restricted to bits [lo, hi)

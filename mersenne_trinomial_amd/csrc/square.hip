@@ -44,6 +44,7 @@ struct SqDesc {
     u64 *out;
     SqConst c;
     const int *order;   // block -> tile (-1 idle), or null for identity
+    Sq2Chain h;         // two-squaring tile layout
 };
 
 // output pair i (one 16-byte store when both words exist)
@@ -104,44 +105,50 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
     }
 }
 
-// ---- two squarings per launch: H = (F^2)^2 mod T (tiles of SQ2_TP pairs)
-template <bool NT>
-__device__ __forceinline__ void square2_tile(const u64 *__restrict__ in, u64 *__restrict__ out,
-                                             const SqConst &c, size_t tile, u64 *gw) {
+// ---- two squarings per launch: H = (F^2)^2 mod T
+// One tile, the output pairs [i0, i1) (at most SQ2_TP): the G phase forms
+// the windows of pl.compute, one G pair per thread and window, into the
+// plan's LDS slots; the H phase then reads G through the window copies of
+// the plan.  LOADFIRST issues the loads of every window before any is
+// combined (a stand-alone tile); without it only windows 1 and 2, which
+// every tile forms, go first and the others are loaded as they are
+// formed, which keeps the work-unit loop at 56 VGPRs (7 waves per SIMD)
+// where the full load-first form needed 84.
+template <bool NT, bool LOADFIRST = true>
+__device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                            const SqConst &c, size_t i0, size_t i1, int treg,
+                                            const Sq2Plan &pl, u64 *gw) {
     static_assert(SQ_THREADS == SQ2_GP && 2 * SQ_THREADS >= SQ2_TP, "tile shape");
     typedef u64 v2u __attribute__((ext_vector_type(2)));
-    const size_t np = (c.nw + 1) / 2;
-    const size_t i0 = tile * SQ2_TP;
-    if (i0 >= np) return;   // whole block
-    const size_t i1 = i0 + SQ2_TP < np ? i0 + SQ2_TP : np;
-    const int treg = sq_tile_region(c, i0, i1);
-    const u32 need = sq2_need(treg);
-    // G phase: one G pair per thread and window; the loads of all the
-    // windows go out before any of them is combined
     const SqArray fsrc{in, c.nw};
-    u64 p0[SQ2_WINS];
     int greg[SQ2_WINS];
     SqRawPair raw[SQ2_WINS];
 #pragma unroll
     for (int k = 0; k < SQ2_WINS; k++) {
-        p0[k] = sq2_p0(c, i0, k);
-        greg[k] = ((need >> k) & 1) ? sq_tile_region(c, p0[k], p0[k] + SQ2_GP) : -1;
+        const u64 p0 = pl.base[k] >> 1;
+        greg[k] = ((pl.compute >> k) & 1) ? sq_tile_region(c, p0, p0 + SQ2_GP) : -1;
     }
+    // loads issued ahead of any combining: every window's (LOADFIRST), or
+    // those of windows 1 and 2, which every tile forms
 #pragma unroll
-    for (int k = 0; k < SQ2_WINS; k++)
+    for (int k = 0; k < (LOADFIRST ? SQ2_WINS : 2); k++)
         if (greg[k] >= 0 && greg[k] < 3)
-            raw[k] = sq_raw_pair(fsrc, c, (u32)(p0[k] + threadIdx.x), greg[k]);
+            raw[k] = sq_raw_pair(fsrc, c, (u32)((pl.base[k] >> 1) + threadIdx.x), greg[k]);
 #pragma unroll
     for (int k = 0; k < SQ2_WINS; k++) {
         if (greg[k] < 0) continue;
         u64 o0, o1;
-        if (greg[k] < 3) sq_combine_pair(raw[k], c, greg[k], o0, o1);
-        else sq2_g_pair_clipped(in, c, p0[k] + threadIdx.x, o0, o1);
-        *(v2u *)(gw + k * SQ2_LEN + 2 * threadIdx.x) = v2u{o0, o1};
+        if (greg[k] < 3) {
+            if (!LOADFIRST && k >= 2)
+                raw[k] = sq_raw_pair(fsrc, c, (u32)((pl.base[k] >> 1) + threadIdx.x), greg[k]);
+            sq_combine_pair(raw[k], c, greg[k], o0, o1);
+        } else {
+            sq2_g_pair_clipped(in, c, (pl.base[k] >> 1) + threadIdx.x, o0, o1);
+        }
+        *(v2u *)(gw + pl.at[k] + 2 * threadIdx.x) = v2u{o0, o1};
     }
     __syncthreads();
-    // H phase from the window copies
-    const Sq2Tile src = sq2_tile(gw, c, i0);
+    const Sq2Tile src = sq2_tile(gw, pl);
     if (treg < 3) {
         sq_variant(treg, c, [&](auto v) {
             for (int j = 0; j < 2; j++) {
@@ -164,20 +171,51 @@ __device__ __forceinline__ void square2_tile(const u64 *__restrict__ in, u64 *__
         }
     }
 }
+// the plain tile number `tile` (pairs from tile * SQ2_TP on)
+template <bool NT>
+__device__ __forceinline__ void square2_tile(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                             const SqConst &c, size_t tile, u64 *gw) {
+    const size_t np = (c.nw + 1) / 2;
+    const size_t i0 = tile * SQ2_TP;
+    if (i0 >= np) return;   // whole block
+    const size_t i1 = i0 + SQ2_TP < np ? i0 + SQ2_TP : np;
+    const int treg = sq_tile_region(c, i0, i1);
+    square2_run<NT, false>(in, out, c, i0, i1, treg, sq2_plan(c, i0, sq2_need(treg)), gw);
+}
+// work unit u: its tiles one after another, each taking windows 3 and 5
+// from the copies of the tile before it where the plan says so
+template <bool NT>
+__device__ __forceinline__ void square2_unit(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                             const SqConst &c, const Sq2Chain &h, u32 u,
+                                             u64 *gw) {
+    Sq2Unit w;
+    if (!sq2_unit(h, u, w)) return;   // whole block
+    u32 prev_need = 0, prev_base4 = 0;
+    for (u32 j = w.j0; j < w.j1; j++) {
+        if (j > w.j0) __syncthreads();   // the previous H phase is done with the slots
+        const u32 i0 = w.start + j * h.period;
+        const u32 i1 = i0 + w.width < h.np ? i0 + w.width : h.np;
+        const int treg = sq_tile_region(c, i0, i1);
+        const u32 need = sq2_need(treg);
+        const Sq2Plan pl = sq2_chain_plan(c, h, i0, w.width, need, j - w.j0, prev_need, prev_base4);
+        square2_run<NT, false>(in, out, c, i0, i1, treg, pl, gw);
+        prev_need = need;
+        prev_base4 = pl.base[3];
+    }
+}
 
-// batched: blockIdx.y selects the residue, blockIdx.x the tile
+// batched: blockIdx.y selects the residue, blockIdx.x the work unit
 template <bool NT>
 __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_batch(
         const SqDesc *__restrict__ desc) {
     __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
     const SqDesc d = desc[blockIdx.y];
-    square2_tile<NT>(d.in, d.out, d.c, blockIdx.x, gw);
+    square2_unit<NT>(d.in, d.out, d.c, d.h, blockIdx.x, gw);
 }
-
 __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold(
-        const u64 *__restrict__ in, u64 *__restrict__ out, const SqConst c) {
+        const u64 *__restrict__ in, u64 *__restrict__ out, const SqConst c, const Sq2Chain h) {
     __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
-    square2_tile<false>(in, out, c, blockIdx.x, gw);
+    square2_unit<false>(in, out, c, h, blockIdx.x, gw);
 }
 
 // slab of n residues (pitch words apart), residue y with s = svec[y / group],
@@ -187,17 +225,18 @@ __device__ __forceinline__ u64 uniform_u64(u64 v) {
     return ((u64)__builtin_amdgcn_readfirstlane((u32)(v >> 32)) << 32) |
            __builtin_amdgcn_readfirstlane((u32)v);
 }
+// One tile of one residue per block: y = blockIdx.z * gridDim.y +
+// blockIdx.y (a loop over residues here held its whole state across
+// iterations, at 160-210 VGPRs and 2-3 waves per SIMD, against 47 and 8
+// without it).
 __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_slab_g(
         const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
         const u64 *__restrict__ svec, unsigned n, unsigned group) {
     __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
-    const size_t np = (size_t)(((r + 63) >> 6) + 1) / 2;
-    if ((size_t)blockIdx.x * SQ2_TP >= np) return;
-    for (unsigned y = blockIdx.y; y < n; y += gridDim.y) {
-        const SqConst c = sq_const(r, uniform_u64(svec[y / group]));
-        square2_tile<false>(in + (u64)y * pitch, out + (u64)y * pitch, c, blockIdx.x, gw);
-        __syncthreads();   // the next residue's first phase rewrites gw
-    }
+    const unsigned y = blockIdx.z * gridDim.y + blockIdx.y;
+    if (y >= n) return;   // whole block
+    const SqConst c = sq_const(r, uniform_u64(svec[y / group]));
+    square2_tile<false>(in + (u64)y * pitch, out + (u64)y * pitch, c, blockIdx.x, gw);
 }
 
 // slab: residue y at in + y*pitch, its s in svec[y]
@@ -332,11 +371,12 @@ extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r
     hipStream_t st = (hipStream_t)stream;
     u64 *a = dres, *b = dtmp;
     const bool fuse = sq_fuse();
+    const Sq2Chain h2 = sq2_chain(c);
     u64 launches = 0;
     for (u64 k = 0; k < steps; launches++) {
         if (fuse && steps - k >= 2) {
-            hipLaunchKernelGGL(k_square2_fold, dim3(blocks_for((nw + 1) / 2, SQ2_TP)),
-                               dim3(SQ_THREADS), 0, st, a, b, c);
+            hipLaunchKernelGGL(k_square2_fold, dim3((unsigned)sq2_units(h2)), dim3(SQ_THREADS), 0,
+                               st, a, b, c, h2);
             k += 2;
         } else {
             hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((nw + 1) / 2, SQ_THREADS)),
@@ -478,8 +518,9 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
         int i = ord[k];
         SqConst c = sq_const(r[i], s[i]);
         const int *ot = order ? g_batch.ord + (size_t)k * gx : nullptr;
-        h[k] = SqDesc{dres[i], dtmp[i], c, ot};          // even steps
-        h[n + k] = SqDesc{dtmp[i], dres[i], c, ot};      // odd steps
+        const Sq2Chain h2 = sq2_chain(c);
+        h[k] = SqDesc{dres[i], dtmp[i], c, ot, h2};          // even steps
+        h[n + k] = SqDesc{dtmp[i], dres[i], c, ot, h2};      // odd steps
     }
     GF2T_HIP(hipMemcpyAsync(g_batch.d, h.data(), need * sizeof(SqDesc), hipMemcpyHostToDevice, st));
     // Every launch advances all live chains together, by two squarings
@@ -487,10 +528,8 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
     // prefix when its steps are done, after launches[k] launches (its
     // value is in dtmp when that count is odd).
     const bool fuse = sq_fuse();
-    size_t gx2 = 0;
-    for (int i = 0; i < n; i++)
-        gx2 = std::max<size_t>(gx2, (words_of(r[i]) + 1) / 2);
-    gx2 = (gx2 + SQ2_TP - 1) / SQ2_TP;
+    u64 gx2 = 0;   // work units of the residue with the most
+    for (int i = 0; i < n; i++) gx2 = std::max(gx2, sq2_units(sq2_chain(sq_const(r[i], s[i]))));
     std::vector<u64> launches(n, 0);
     int active = n;
     u64 nl = 0;
@@ -620,7 +659,8 @@ extern "C" int gf2t_square2_fold_slab_g(const uint64_t *din, uint64_t *dout, siz
     if (n == 0) return GF2T_OK;
     const unsigned gx = (unsigned)(((nw + 1) / 2 + SQ2_TP - 1) / SQ2_TP);
     const unsigned gy = n < 65535 ? (unsigned)n : 65535u;
-    hipLaunchKernelGGL(k_square2_fold_slab_g, dim3(gx, gy), dim3(SQ_THREADS), 0,
+    const unsigned gz = (unsigned)((n + gy - 1) / gy);
+    hipLaunchKernelGGL(k_square2_fold_slab_g, dim3(gx, gy, gz), dim3(SQ_THREADS), 0,
                        (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
                        (unsigned)group);
     GF2T_CHECK_LAUNCH("k_square2_fold_slab_g");

@@ -5,13 +5,14 @@
 #include <vector>
 
 // the two-squaring tile's window copies, with every read checked
-// against the words the G phase wrote
+// against the words the G phase wrote: holds[x] is the G word that LDS
+// word x was last given (~0: none)
 struct CheckedTile {
     Sq2Tile t;
-    const std::vector<char> *written;
+    const std::vector<u64> *holds;
     u64 operator()(int k, size_t w) const {
         const u64 at = w - t.base[k];
-        if (w < t.base[k] || at >= SQ2_LEN || !(*written)[(size_t)k * SQ2_LEN + at]) abort();
+        if (w < t.base[k] || at >= SQ2_LEN || (*holds)[t.at[k] + at] != w) abort();
         return t(k, w);
     }
     u32 ld32(int k, u32 q) const {
@@ -43,46 +44,98 @@ void kl_square_fold_fast(const u64 *in, u64 *out, u64 r, u64 s) {
         if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
     }
 }
-// the two-squaring kernel's tiles, run one after another: the G phase
-// (one G pair per thread and needed window), then the tile's H pairs
-// through the same calls the kernel makes
+// the G phase of one tile: the windows of pl.compute, one G pair per
+// thread, into the plan's slots
+static void g_phase(const u64 *in, const SqConst &c, const Sq2Plan &pl, std::vector<u64> &gw,
+                    std::vector<u64> &holds, u64 &formed) {
+    for (int k = 0; k < SQ2_WINS; k++) {
+        if (!((pl.compute >> k) & 1)) continue;
+        formed++;
+        const u64 p0 = pl.base[k] >> 1;
+        const int greg = sq_tile_region(c, p0, p0 + SQ2_GP);
+        for (u32 t = 0; t < SQ2_GP; t++) {
+            u64 o0, o1;
+            if (greg < 3) {
+                SqRawPair w = sq_raw_pair(SqArray{in, c.nw}, c, (u32)(p0 + t), greg);
+                sq_combine_pair(w, c, greg, o0, o1);
+            } else {
+                sq2_g_pair_clipped(in, c, p0 + t, o0, o1);
+            }
+            const size_t at = pl.at[k] + 2 * t;
+            gw[at] = o0;
+            gw[at + 1] = o1;
+            holds[at] = 2 * (p0 + t);
+            holds[at + 1] = 2 * (p0 + t) + 1;
+        }
+    }
+}
+static void h_phase(const SqConst &c, size_t i0, size_t i1, int treg, const Sq2Plan &pl,
+                    const std::vector<u64> &gw, const std::vector<u64> &holds, u64 *out,
+                    std::vector<unsigned char> &hits) {
+    CheckedTile src{sq2_tile(gw.data(), pl), &holds};
+    for (size_t i = i0; i < i1; i++) {
+        u64 o0, o1;
+        if (treg < 3) sq_pair_region(src, c, (u32)i, treg, o0, o1);
+        else square_fold_pair_fast_src(src, c, i, o0, o1);
+        out[2 * i] = o0;
+        if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
+        hits[i]++;
+    }
+}
+// the two-squaring kernel's plain tiles, run one after another
 void kl_square2_fold(const u64 *in, u64 *out, u64 r, u64 s) {
     SqConst c = sq_const(r, s);
     const size_t np = (c.nw + 1) / 2;
-    std::vector<u64> gw((size_t)SQ2_WINS * SQ2_LEN);
-    std::vector<char> written(gw.size());
+    std::vector<u64> gw((size_t)SQ2_WINS * SQ2_LEN), holds(gw.size());
+    std::vector<unsigned char> hits(np);
+    u64 formed = 0;
     for (size_t i0 = 0; i0 < np; i0 += SQ2_TP) {
         const size_t i1 = i0 + SQ2_TP < np ? i0 + SQ2_TP : np;
         const int treg = sq_tile_region(c, i0, i1);
-        const u32 need = sq2_need(treg);
+        const Sq2Plan pl = sq2_plan(c, i0, sq2_need(treg));
+        std::fill(holds.begin(), holds.end(), ~0ULL);
+        g_phase(in, c, pl, gw, holds, formed);
+        h_phase(c, i0, i1, treg, pl, gw, holds, out, hits);
+    }
+}
+// the chained work units of k_square2_fold_batch, each a fresh block
+// (LDS of unknown content) walking its tiles; every output pair must be
+// formed exactly once.  stats: units with tiles, tiles, windows formed.
+void kl_square2_fold_units(const u64 *in, u64 *out, u64 r, u64 s, u64 *stats) {
+    SqConst c = sq_const(r, s);
+    const Sq2Chain h = sq2_chain(c);
+    std::vector<u64> gw((size_t)SQ2_WINS * SQ2_LEN), holds(gw.size());
+    std::vector<unsigned char> hits(h.np);
+    u64 units = 0, tiles = 0, formed = 0;
+    const u64 nu = sq2_units(h);
+    for (u64 u = 0; u < nu; u++) {
+        Sq2Unit w;
+        if (!sq2_unit(h, (u32)u, w)) continue;
+        if (w.width == 0 || w.width > SQ2_TP) abort();
+        units++;
         std::fill(gw.begin(), gw.end(), 0x5A5A5A5A5A5A5A5AULL);
-        std::fill(written.begin(), written.end(), 0);
-        for (int k = 0; k < SQ2_WINS; k++) {
-            if (!((need >> k) & 1)) continue;
-            const u64 p0 = sq2_p0(c, i0, k);
-            const int greg = sq_tile_region(c, p0, p0 + SQ2_GP);
-            for (u32 t = 0; t < SQ2_GP; t++) {
-                u64 o0, o1;
-                if (greg < 3) {
-                    SqRawPair w = sq_raw_pair(SqArray{in, c.nw}, c, (u32)(p0 + t), greg);
-                    sq_combine_pair(w, c, greg, o0, o1);
-                } else {
-                    sq2_g_pair_clipped(in, c, p0 + t, o0, o1);
-                }
-                gw[(size_t)k * SQ2_LEN + 2 * t] = o0;
-                gw[(size_t)k * SQ2_LEN + 2 * t + 1] = o1;
-                written[(size_t)k * SQ2_LEN + 2 * t] = written[(size_t)k * SQ2_LEN + 2 * t + 1] = 1;
-            }
-        }
-        CheckedTile src{sq2_tile(gw.data(), c, i0), &written};
-        for (size_t i = i0; i < i1; i++) {
-            u64 o0, o1;
-            if (treg < 3) sq_pair_region(src, c, (u32)i, treg, o0, o1);
-            else square_fold_pair_fast_src(src, c, i, o0, o1);
-            out[2 * i] = o0;
-            if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
+        std::fill(holds.begin(), holds.end(), ~0ULL);
+        u32 prev_need = 0, prev_base4 = 0;
+        for (u32 j = w.j0; j < w.j1; j++) {
+            const u32 i0 = w.start + j * h.period;
+            if (i0 >= h.np) abort();
+            const u32 i1 = i0 + w.width < h.np ? i0 + w.width : h.np;
+            const int treg = sq_tile_region(c, i0, i1);
+            const u32 need = sq2_need(treg);
+            const Sq2Plan pl = sq2_chain_plan(c, h, i0, w.width, need, j - w.j0, prev_need, prev_base4);
+            if ((pl.compute & ~need) || !(pl.compute & 2u)) abort();
+            g_phase(in, c, pl, gw, holds, formed);
+            h_phase(c, i0, i1, treg, pl, gw, holds, out, hits);
+            tiles++;
+            prev_need = need;
+            prev_base4 = pl.base[3];
         }
     }
+    for (size_t i = 0; i < h.np; i++)
+        if (hits[i] != 1) abort();
+    stats[0] = units;
+    stats[1] = tiles;
+    stats[2] = formed;
 }
 // the single-squaring batch kernel's per-tile region path
 void kl_square_fold_tiles(const u64 *in, u64 *out, u64 r, u64 s, u32 tile_pairs) {

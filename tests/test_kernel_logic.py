@@ -25,6 +25,7 @@ def kl(tmp_path_factory):
     lib.kl_square_fold_fast.argtypes = [P, P, C.c_uint64, C.c_uint64]
     lib.kl_fold.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint64]
     lib.kl_square2_fold.argtypes = [P, P, C.c_uint64, C.c_uint64]
+    lib.kl_square2_fold_units.argtypes = [P, P, C.c_uint64, C.c_uint64, P]
     lib.kl_square_fold_tiles.argtypes = [P, P, C.c_uint64, C.c_uint64, C.c_uint32]
     lib.kl_spread32p.restype = C.c_uint64
     lib.kl_spread32p.argtypes = [C.c_uint32]
@@ -173,6 +174,53 @@ def test_two_squarings_per_tile(kl):
             assert np.array_equal(out, one), (r, s, tp)
 
 
+def units(kl, a, r, s):
+    out = np.zeros(O.words(r), np.uint64)
+    st = np.zeros(3, np.uint64)
+    kl.kl_square2_fold_units(ptr(a), ptr(out), r, s, ptr(st))
+    return out, [int(x) for x in st]
+
+
+def test_two_squarings_chained_units(kl):
+    """The batched two-squaring kernel's work units: chains of tiles
+    delta = off2 - off3 pairs apart, each tile taking window 3 (and window 5
+    where the copy covers it) from the previous tile's copies.  Every read
+    is checked against the G word the LDS word holds and every output pair
+    must be formed once; s spans the unchained case (delta < 510), the
+    switch at delta = 510, one to many columns per period and chains cut
+    into several units, at both parities of r, s and off4 - off5 - delta."""
+    rng = np.random.default_rng(15)
+    cases = [(r, s) for r in (7, 131, 383, 4423) for s in (1, (r - 1) // 3, (r - 1) // 2)]
+    for r in (300007, 1000003, 1048573):
+        h = (r - 1) // 2
+        cases += [(r, s) for s in sorted({1, 2, 65279, 65280, 65281, 65407, 65408, 65409,
+                                          65535, 65536, 130559, 130560, 130561, r // 5, r // 4,
+                                          r // 3, h - 1, h} & set(range(1, h + 1)))]
+        cases += [(r, int(s)) for s in rng.integers(65000, h, 12)]
+    chained = set()
+    for r, s in cases:
+        a = rand_res(rng, r)
+        want = O.square_fold(O.square_fold(a, r, s), r, s)
+        out, (nu, nt, formed) = units(kl, a, r, s)
+        assert np.array_equal(out, want), (r, s)
+        assert nu > 0 and nt >= nu and formed >= 2 * nt
+        chained.add(s > 128 * 509)
+    assert chained == {False, True}
+
+
+def test_chained_units_form_fewer_windows(kl):
+    """In [2s, r) a chained tile forms windows 1 and 2 only: at s ~ r/4
+    the windows formed per tile fall from 3 + s/r (plain tiles) to about
+    2 + s/r, up to the one column in about 30 whose tiles straddle s, 2s
+    and r."""
+    rng = np.random.default_rng(16)
+    r = 8388593
+    s = r // 4
+    a = rand_res(rng, r)
+    _, (nu, nt, formed) = units(kl, a, r, s)
+    assert formed < 2.35 * nt, (nu, nt, formed)
+
+
 def test_two_squarings_full_size_sample(kl):
     rng = np.random.default_rng(14)
     r = 136279841
@@ -180,7 +228,13 @@ def test_two_squarings_full_size_sample(kl):
         a = rand_res(rng, r)
         out = np.zeros(O.words(r), np.uint64)
         kl.kl_square2_fold(ptr(a), ptr(out), r, s)
-        assert np.array_equal(out, O.square_fold(O.square_fold(a, r, s), r, s)), s
+        want = O.square_fold(O.square_fold(a, r, s), r, s)
+        assert np.array_equal(out, want), s
+        out, _ = units(kl, a, r, s)
+        assert np.array_equal(out, want), s
+    a = rand_res(rng, r)
+    out, _ = units(kl, a, r, 30000001)
+    assert np.array_equal(out, O.square_fold(O.square_fold(a, r, 30000001), r, 30000001))
 
 
 def test_fold_lengths(kl):
