@@ -204,9 +204,12 @@ __device__ __forceinline__ void square2_unit(const u64 *__restrict__ in, u64 *__
     }
 }
 
-// batched: blockIdx.y selects the residue, blockIdx.x the work unit
+// batched: blockIdx.y selects the residue, blockIdx.x the work unit.
+// Eight waves per SIMD (106 SGPRs would allow seven; 23 of them then live
+// in VGPR lanes, no scratch): about 2% faster per launch.
 template <bool NT>
-__global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_batch(
+__global__ __launch_bounds__(SQ_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void
+k_square2_fold_batch(
         const SqDesc *__restrict__ desc) {
     __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
     const SqDesc d = desc[blockIdx.y];
