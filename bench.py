@@ -503,6 +503,21 @@ def main():
     certified = int(np.count_nonzero(best[1:] != np.uint64(0xFFFFFFFFFFFFFFFF)))
     survivors = SMAX - certified
     per_step_ms = 1e3 * wall / args.steps
+
+    # after the timed region: the library's 16-byte stream copy over the
+    # chain's live bytes (read + write = 16 nw n), the rate a pure
+    # streaming kernel reaches on this box
+    big_a = DeviceBuffer.words(nw * n)
+    big_b = DeviceBuffer.words(nw * n)
+    nbytes16 = (8 * nw * n) & ~15
+    check(lib.gf2t_stream_copy(big_b.ptr, big_a.ptr, nbytes16, None))
+    ec0, ec1 = Event(), Event()
+    ec0.record()
+    check(lib.gf2t_stream_copy(big_b.ptr, big_a.ptr, nbytes16, None))
+    ec1.record()
+    device.synchronize()
+    copy_gbs = 2 * nbytes16 / (ec0.elapsed_ms(ec1) * 1e-3) / 1e9
+    del big_a, big_b
     value = certified / (wall / args.steps)
 
     kernel, launches, per = chain_launches(args.chain_steps, args.sq_fuse != 0)
@@ -539,6 +554,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "copy_peak_measured": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs,
             "traffic": SQ_TRAFFIC_256.get(kernel) if (n == 256 and default_tuning) else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "squarings_per_launch": per,
