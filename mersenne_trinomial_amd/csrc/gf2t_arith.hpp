@@ -476,14 +476,18 @@ GF2T_HD int sq_tile_region(const SqConst &C, size_t i0, size_t i1) {
 // A tile of output pairs [i0, i0 + SQ2_TP) of H reads G only through
 // the windows of its pairs: window k covers G words [i0 + off_k,
 // i0 + off_k + SQ2_TP] (the last one for the funnel shift).  With
-// SQ2_TP = 510 those words lie inside the 256 G pairs from
+// SQ2_TP <= 510 those words lie inside the 256 G pairs from
 // (i0 + off_k) >> 1 on, so each of the 256 threads computes one G pair
 // per window, straight from F, into that window's copy (512 words).
 // Tiles inside one region need three or four of the five windows; a
 // tile that straddles s, 2s or r takes all five.  The tile's H pairs are
 // then formed from those copies: F is read once per two squarings.
+// SQ2_TP = 504 = 63 * 8 leaves room for the H phase to map its threads
+// from i0 rounded down to 8 pairs (one 128-byte line), so that each
+// wave's 1 KiB of non-temporal stores covers whole lines: the block's
+// 512 pair slots cover (i0 & 7) + width <= 7 + 504 pairs.
 constexpr int SQ2_WINS = 5;
-constexpr u32 SQ2_TP = 510;
+constexpr u32 SQ2_TP = 504;
 constexpr u32 SQ2_GP = 256;          // G pairs per window
 constexpr u32 SQ2_LEN = 2 * SQ2_GP;  // G words per window copy
 GF2T_HD u32 sq2_off(const SqConst &C, int k) {   // word offset of window k

@@ -118,7 +118,7 @@ template <bool NT, bool LOADFIRST = true>
 __device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__restrict__ out,
                                             const SqConst &c, size_t i0, size_t i1, int treg,
                                             const Sq2Plan &pl, u64 *gw) {
-    static_assert(SQ_THREADS == SQ2_GP && 2 * SQ_THREADS >= SQ2_TP, "tile shape");
+    static_assert(SQ_THREADS == SQ2_GP && 2 * SQ_THREADS >= SQ2_TP + 7, "tile shape");
     typedef u64 v2u __attribute__((ext_vector_type(2)));
     const SqArray fsrc{in, c.nw};
     int greg[SQ2_WINS];
@@ -149,11 +149,15 @@ __device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__r
     }
     __syncthreads();
     const Sq2Tile src = sq2_tile(gw, pl);
+    // pair slots from i0 rounded down to a 128-byte line: each wave stores
+    // whole lines (non-temporal stores of part of a line go to memory
+    // as partial writes); the slots below i0 idle
+    const size_t ib = i0 & ~(size_t)7;
     if (treg < 3) {
         sq_variant(treg, c, [&](auto v) {
             for (int j = 0; j < 2; j++) {
-                const size_t i = i0 + (size_t)j * SQ_THREADS + threadIdx.x;
-                if (i < i1) {
+                const size_t i = ib + (size_t)j * SQ_THREADS + threadIdx.x;
+                if (i >= i0 && i < i1) {
                     u64 o0, o1;
                     v.pair(src, c, (u32)i, o0, o1);
                     store_pair<NT>(out, c.nw, i, o0, o1);
@@ -162,8 +166,8 @@ __device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__r
         });
     } else {
         for (int j = 0; j < 2; j++) {
-            const size_t i = i0 + (size_t)j * SQ_THREADS + threadIdx.x;
-            if (i < i1) {
+            const size_t i = ib + (size_t)j * SQ_THREADS + threadIdx.x;
+            if (i >= i0 && i < i1) {
                 u64 o0, o1;
                 square_fold_pair_fast_src(src, c, i, o0, o1);
                 store_pair<NT>(out, c.nw, i, o0, o1);

@@ -111,7 +111,7 @@ void kl_square2_fold_units(const u64 *in, u64 *out, u64 r, u64 s, u64 *stats) {
     for (u64 u = 0; u < nu; u++) {
         Sq2Unit w;
         if (!sq2_unit(h, (u32)u, w)) continue;
-        if (w.width == 0 || w.width > SQ2_TP) abort();
+        if (w.width == 0 || w.width > (h.chained ? SQ2_CW : SQ2_TP)) abort();
         units++;
         std::fill(gw.begin(), gw.end(), 0x5A5A5A5A5A5A5A5AULL);
         std::fill(holds.begin(), holds.end(), ~0ULL);
@@ -120,6 +120,8 @@ void kl_square2_fold_units(const u64 *in, u64 *out, u64 r, u64 s, u64 *stats) {
             const u32 i0 = w.start + j * h.period;
             if (i0 >= h.np) abort();
             const u32 i1 = i0 + w.width < h.np ? i0 + w.width : h.np;
+            // the H phase's 512 pair slots start at i0 rounded down to 8
+            if ((i0 & 7) + (i1 - i0) > 512) abort();
             const int treg = sq_tile_region(c, i0, i1);
             const u32 need = sq2_need(treg);
             const Sq2Plan pl = sq2_chain_plan(c, h, i0, w.width, need, j - w.j0, prev_need, prev_base4);

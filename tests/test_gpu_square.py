@@ -83,11 +83,12 @@ def test_two_squarings_every_s(eng, r):
 @pytest.mark.parametrize("r", [1048573, 2000003])
 def test_two_squarings_large_s(eng, r):
     """One two-squaring launch for s from about 128 tiles' worth of pairs
-    (s ~ 65000, where the window offsets off2 - off3 pass one tile width)
-    up to (r - 1)/2, at both parities, against the oracle."""
+    (s ~ 64256, where the window offsets off2 - off3 pass one chained tile
+    width) up to (r - 1)/2, at both parities, against the oracle."""
     rng = np.random.default_rng(r + 1)
     h = (r - 1) // 2
-    svals = {64900, 65023, 65024, 65025, 65151, 65152, 65153, 65279, 65280, 130000, h - 1, h}
+    svals = {64127, 64128, 64255, 64256, 64257, 64383, 64384, 64900, 65024, 65152, 65280,
+             130000, h - 1, h}
     svals |= {int(x) for x in rng.integers(65000, h, 40)}
     for s in sorted(svals):
         a = rand_res(rng, r)

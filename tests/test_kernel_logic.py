@@ -186,16 +186,18 @@ def test_two_squarings_chained_units(kl):
     delta = off2 - off3 pairs apart, each tile taking window 3 (and window 5
     where the copy covers it) from the previous tile's copies.  Every read
     is checked against the G word the LDS word holds and every output pair
-    must be formed once; s spans the unchained case (delta < 510), the
-    switch at delta = 510, one to many columns per period and chains cut
-    into several units, at both parities of r, s and off4 - off5 - delta."""
+    must be formed once; s spans the unchained case (delta < 502), the
+    switch at delta = 502 (s ~ 64256), one to many columns per period and
+    chains cut into several units, at both parities of r, s and
+    off4 - off5 - delta."""
     rng = np.random.default_rng(15)
     cases = [(r, s) for r in (7, 131, 383, 4423) for s in (1, (r - 1) // 3, (r - 1) // 2)]
     for r in (300007, 1000003, 1048573):
         h = (r - 1) // 2
-        cases += [(r, s) for s in sorted({1, 2, 65279, 65280, 65281, 65407, 65408, 65409,
-                                          65535, 65536, 130559, 130560, 130561, r // 5, r // 4,
-                                          r // 3, h - 1, h} & set(range(1, h + 1)))]
+        cases += [(r, s) for s in sorted({1, 2, 64127, 64128, 64255, 64256, 64257, 64383,
+                                          64384, 65279, 65280, 65281, 65535, 65536, 130559,
+                                          130560, 130561, r // 5, r // 4, r // 3, h - 1, h}
+                                         & set(range(1, h + 1)))]
         cases += [(r, int(s)) for s in rng.integers(65000, h, 12)]
     chained = set()
     for r, s in cases:
@@ -204,7 +206,7 @@ def test_two_squarings_chained_units(kl):
         out, (nu, nt, formed) = units(kl, a, r, s)
         assert np.array_equal(out, want), (r, s)
         assert nu > 0 and nt >= nu and formed >= 2 * nt
-        chained.add(s > 128 * 509)
+        chained.add(s > 128 * 503)
     assert chained == {False, True}
 
 
