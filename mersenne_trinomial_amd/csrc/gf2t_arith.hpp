@@ -498,18 +498,13 @@ GF2T_HD u32 sq2_need(int region) {   // windows read by a tile of this region
 }
 // first G pair of window k for the tile at i0
 GF2T_HD u64 sq2_p0(const SqConst &C, size_t i0, int k) { return ((u64)i0 + sq2_off(C, k)) >> 1; }
-// G pairs p0 + j (j < SQ2_GP) of one window: words 2(p0 + j) and
-// 2(p0 + j) + 1 of G, zero at and beyond nw.  greg = the region of the
-// window's G pairs, sq_tile_region(C, p0, p0 + SQ2_GP).
+// G pair p, words 2p and 2p + 1 of G (zero at and beyond nw), by the
+// clipped path: for a window whose G pairs straddle s, 2s or r
+// (sq_tile_region(C, p0, p0 + SQ2_GP) == 3)
 GF2T_HD void sq2_g_pair_clipped(const u64 *F, const SqConst &C, u64 p, u64 &o0, u64 &o1) {
     o0 = o1 = 0;
     if (2 * p < C.nw) square_fold_pair_fast(F, C, (size_t)p, o0, o1);
     if (2 * p + 1 >= C.nw) o1 = 0;
-}
-GF2T_HD void sq2_g_pair(const u64 *F, const SqConst &C, u64 p0, u32 j, u64 &o0, u64 &o1) {
-    const int greg = sq_tile_region(C, p0, p0 + SQ2_GP);
-    if (greg < 3) sq_pair_region(SqArray{F, C.nw}, C, (u32)(p0 + j), greg, o0, o1);
-    else sq2_g_pair_clipped(F, C, p0 + j, o0, o1);
 }
 // the H phase reads G through the tile's window copies: window k's copy
 // holds G words from base_k = 2 p0_k on, at word at_k of the LDS area
@@ -557,9 +552,6 @@ GF2T_HD Sq2Tile sq2_tile(const u64 *win, const Sq2Plan &p) {
         t.at[k] = p.at[k];
     }
     return t;
-}
-GF2T_HD Sq2Tile sq2_tile(const u64 *win, const SqConst &C, size_t i0) {
-    return sq2_tile(win, sq2_plan(C, i0, 0x1Fu));
 }
 
 // ---- chained tiles of the two-squaring pass
