@@ -568,7 +568,7 @@ GF2T_HD Sq2Tile sq2_tile(const u64 *win, const Sq2Plan &p) {
 // wide, so one 512-word copy of window 4 can cover both its own words
 // and the next tile's window 5 (the copy's base is placed for both;
 // where it cannot cover, window 5 is formed).  Each period of delta
-// pairs is split into ncols columns of equal width.  In [2s, r) a tile
+// pairs is split into ncols columns of wlo or wlo + 1 pairs.  In [2s, r) a tile
 // then forms windows 1 and 2 only, in [s, 2s) likewise, where plain
 // tiles form three and four.  For delta < SQ2_CW (s below about
 // 128 SQ2_CW) units are the plain consecutive tiles of SQ2_TP pairs.
