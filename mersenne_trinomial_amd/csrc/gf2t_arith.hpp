@@ -257,16 +257,22 @@ GF2T_HD u64 interleave32(u32 e, u32 o) {
 // `win` names the fold window a read serves (0 = SP[0,r), 1..4 = the
 // four shifted windows): a plain array ignores it, the two-squaring
 // kernel's tile keeps one LDS copy per window.
-struct SqArray {
+template <bool W12>
+struct SqArrayT {
     const u64 *p;
     size_t nw;
     GF2T_HDM u64 operator()(int, size_t k) const { return k < nw ? p[k] : 0; }
     // 32-bit words q and q + 1 of the input, unchecked, and word q + 2
-    // when sh != 0 (word q + 1 again when sh = 0, where the funnel shift
-    // does not use it, so no read passes the window's last word).  No
-    // branch on q's alignment or on sh: the loads of all the windows of
-    // a pair stay in flight together.  32-bit byte offsets: residues are
-    // below 2^29 words.
+    // when sh != 0, the three as one 12-byte load when W12.  Without W12,
+    // word q + 1 is read again when sh = 0 (the funnel shift does not use
+    // it), so no read passes the window's last word.  W12 reads word
+    // q + 2 also when sh = 0 (its value unused): inside the region fast
+    // path a window's last bit is at most r - 1, so 32 (q + 2) <= r and
+    // word q + 2 lies inside the 2 nw words unless r is a multiple of 64
+    // (r = 64 (nw - 1) + t with t < 64 gives floor(r / 32) <= 2 nw - 1).
+    // No branch on q's alignment or on sh: the loads of all the windows
+    // of a pair stay in flight together.  32-bit byte offsets: residues
+    // are below 2^29 words.
     GF2T_HDM const GF2T_GLOBAL u32 *at(u32 q) const {
         return (const GF2T_GLOBAL u32 *)((const GF2T_GLOBAL char *)p + (u32)(q << 2));
     }
@@ -274,9 +280,12 @@ struct SqArray {
         const GF2T_GLOBAL u32 *w = at(q);
         a = w[0];
         b = w[1];
-        c = w[1 + (sh != 0)];
+        c = W12 ? w[2] : w[1 + (sh != 0)];
     }
 };
+using SqArray = SqArrayT<false>;
+// whether the 12-byte window loads are in bounds for degree r
+GF2T_HD bool sq_w12_ok(u64 r) { return (r & 63) != 0; }
 
 // 128-bit window of the virtual spread SP of the source at SP position
 // P, keeping only SP indices in [lo, hi).
@@ -464,8 +473,9 @@ GF2T_HD void square_fold_pair_fast_src(const Src &src, const SqConst &C, size_t 
     if (region == 3) square_fold_pair_src(src, C.r, C.s, i, o0, o1);
     else sq_pair_region(src, C, (u32)i, region, o0, o1);
 }
+template <bool W12 = false>
 GF2T_HD void square_fold_pair_fast(const u64 *in, const SqConst &C, size_t i, u64 &o0, u64 &o1) {
-    square_fold_pair_fast_src(SqArray{in, C.nw}, C, i, o0, o1);
+    square_fold_pair_fast_src(SqArrayT<W12>{in, C.nw}, C, i, o0, o1);
 }
 // region shared by the pairs [i0, i1) (3: mixed or clipped)
 GF2T_HD int sq_tile_region(const SqConst &C, size_t i0, size_t i1) {
@@ -501,9 +511,10 @@ GF2T_HD u64 sq2_p0(const SqConst &C, size_t i0, int k) { return ((u64)i0 + sq2_o
 // G pair p, words 2p and 2p + 1 of G (zero at and beyond nw), by the
 // clipped path: for a window whose G pairs straddle s, 2s or r
 // (sq_tile_region(C, p0, p0 + SQ2_GP) == 3)
-GF2T_HD void sq2_g_pair_clipped(const u64 *F, const SqConst &C, u64 p, u64 &o0, u64 &o1) {
+template <class Src>
+GF2T_HD void sq2_g_pair_clipped(const Src &F, const SqConst &C, u64 p, u64 &o0, u64 &o1) {
     o0 = o1 = 0;
-    if (2 * p < C.nw) square_fold_pair_fast(F, C, (size_t)p, o0, o1);
+    if (2 * p < C.nw) square_fold_pair_fast_src(F, C, (size_t)p, o0, o1);
     if (2 * p + 1 >= C.nw) o1 = 0;
 }
 // the H phase reads G through the tile's window copies: window k's copy

@@ -29,12 +29,16 @@ namespace {
 
 constexpr int SQ_THREADS = 256;
 
+// W12 (template argument of the squaring kernels): the window words are
+// read as one 12-byte load; the host sets it when no residue of the launch
+// has r a multiple of 64 (sq_w12_ok)
+template <bool W12>
 __global__ __launch_bounds__(SQ_THREADS) void k_square_fold(
         const u64 *__restrict__ in, u64 *__restrict__ out, const SqConst c) {
     size_t i = (size_t)blockIdx.x * SQ_THREADS + threadIdx.x;
     if (i >= (c.nw + 1) / 2) return;
     u64 o0, o1;
-    square_fold_pair_fast(in, c, i, o0, o1);
+    square_fold_pair_fast<W12>(in, c, i, o0, o1);
     out[2 * i] = o0;
     if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
 }
@@ -64,7 +68,7 @@ __device__ __forceinline__ void store_pair(u64 *out_, size_t nw, size_t i, u64 o
 
 // batched: blockIdx.y selects the residue, blockIdx.x a tile of
 // SQ_THREADS*PPT output pairs (pairs j*SQ_THREADS + lane for j < PPT)
-template <int PPT, bool NT>
+template <int PPT, bool NT, bool W12>
 __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
         const SqDesc *__restrict__ desc) {
     const SqDesc d = desc[blockIdx.y];
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
     if (t0 >= np) return;
     // one region for the whole tile (all but a few tiles per residue)
     const int treg = sq_tile_region(d.c, t0, t0 + SQ_THREADS * PPT < np ? t0 + SQ_THREADS * PPT : np);
-    const SqArray src{d.in, d.c.nw};
+    const SqArrayT<W12> src{d.in, d.c.nw};
     // streaming stores: F^2 is not re-read before the next launch (one
     // 16-byte store per pair; residue buffers are 16-byte aligned)
     if (treg < 3) {
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
             const size_t i = t0 + (size_t)j * SQ_THREADS + threadIdx.x;
             if (i < np) {
                 u64 o0, o1;
-                square_fold_pair_fast(d.in, d.c, i, o0, o1);
+                square_fold_pair_fast<W12>(d.in, d.c, i, o0, o1);
                 store_pair<NT>(d.out, d.c.nw, i, o0, o1);
             }
         }
@@ -114,13 +118,13 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_batch(
 // every tile forms, go first and the others are loaded as they are
 // formed, which keeps the work-unit loop at 56 VGPRs (7 waves per SIMD)
 // where the full load-first form needed 84.
-template <bool NT, bool LOADFIRST = true>
+template <bool NT, bool W12, bool LOADFIRST = true>
 __device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__restrict__ out,
                                             const SqConst &c, size_t i0, size_t i1, int treg,
                                             const Sq2Plan &pl, u64 *gw) {
     static_assert(SQ_THREADS == SQ2_GP && 2 * SQ_THREADS >= SQ2_TP + 7, "tile shape");
     typedef u64 v2u __attribute__((ext_vector_type(2)));
-    const SqArray fsrc{in, c.nw};
+    const SqArrayT<W12> fsrc{in, c.nw};
     int greg[SQ2_WINS];
     SqRawPair raw[SQ2_WINS];
 #pragma unroll
@@ -143,7 +147,7 @@ __device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__r
                 raw[k] = sq_raw_pair(fsrc, c, (u32)((pl.base[k] >> 1) + threadIdx.x), greg[k]);
             sq_combine_pair(raw[k], c, greg[k], o0, o1);
         } else {
-            sq2_g_pair_clipped(in, c, (pl.base[k] >> 1) + threadIdx.x, o0, o1);
+            sq2_g_pair_clipped(fsrc, c, (pl.base[k] >> 1) + threadIdx.x, o0, o1);
         }
         *(v2u *)(gw + pl.at[k] + 2 * threadIdx.x) = v2u{o0, o1};
     }
@@ -176,7 +180,7 @@ __device__ __forceinline__ void square2_run(const u64 *__restrict__ in, u64 *__r
     }
 }
 // the plain tile number `tile` (pairs from tile * SQ2_TP on)
-template <bool NT>
+template <bool NT, bool W12>
 __device__ __forceinline__ void square2_tile(const u64 *__restrict__ in, u64 *__restrict__ out,
                                              const SqConst &c, size_t tile, u64 *gw) {
     const size_t np = (c.nw + 1) / 2;
@@ -184,11 +188,11 @@ __device__ __forceinline__ void square2_tile(const u64 *__restrict__ in, u64 *__
     if (i0 >= np) return;   // whole block
     const size_t i1 = i0 + SQ2_TP < np ? i0 + SQ2_TP : np;
     const int treg = sq_tile_region(c, i0, i1);
-    square2_run<NT, false>(in, out, c, i0, i1, treg, sq2_plan(c, i0, sq2_need(treg)), gw);
+    square2_run<NT, W12, false>(in, out, c, i0, i1, treg, sq2_plan(c, i0, sq2_need(treg)), gw);
 }
 // work unit u: its tiles one after another, each taking windows 3 and 5
 // from the copies of the tile before it where the plan says so
-template <bool NT>
+template <bool NT, bool W12>
 __device__ __forceinline__ void square2_unit(const u64 *__restrict__ in, u64 *__restrict__ out,
                                              const SqConst &c, const Sq2Chain &h, u32 u,
                                              u64 *gw) {
@@ -202,7 +206,7 @@ __device__ __forceinline__ void square2_unit(const u64 *__restrict__ in, u64 *__
         const int treg = sq_tile_region(c, i0, i1);
         const u32 need = sq2_need(treg);
         const Sq2Plan pl = sq2_chain_plan(c, h, i0, w.width, need, j - w.j0, prev_need, prev_base4);
-        square2_run<NT, false>(in, out, c, i0, i1, treg, pl, gw);
+        square2_run<NT, W12, false>(in, out, c, i0, i1, treg, pl, gw);
         prev_need = need;
         prev_base4 = pl.base[3];
     }
@@ -211,18 +215,19 @@ __device__ __forceinline__ void square2_unit(const u64 *__restrict__ in, u64 *__
 // batched: blockIdx.y selects the residue, blockIdx.x the work unit.
 // Eight waves per SIMD (106 SGPRs would allow seven; 23 of them then live
 // in VGPR lanes, no scratch): about 2% faster per launch.
-template <bool NT>
+template <bool NT, bool W12>
 __global__ __launch_bounds__(SQ_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void
 k_square2_fold_batch(
         const SqDesc *__restrict__ desc) {
     __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
     const SqDesc d = desc[blockIdx.y];
-    square2_unit<NT>(d.in, d.out, d.c, d.h, blockIdx.x, gw);
+    square2_unit<NT, W12>(d.in, d.out, d.c, d.h, blockIdx.x, gw);
 }
+template <bool W12>
 __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold(
         const u64 *__restrict__ in, u64 *__restrict__ out, const SqConst c, const Sq2Chain h) {
     __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
-    square2_unit<false>(in, out, c, h, blockIdx.x, gw);
+    square2_unit<false, W12>(in, out, c, h, blockIdx.x, gw);
 }
 
 // slab of n residues (pitch words apart), residue y with s = svec[y / group],
@@ -236,6 +241,7 @@ __device__ __forceinline__ u64 uniform_u64(u64 v) {
 // blockIdx.y (a loop over residues here held its whole state across
 // iterations, at 160-210 VGPRs and 2-3 waves per SIMD, against 47 and 8
 // without it).
+template <bool W12>
 __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_slab_g(
         const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
         const u64 *__restrict__ svec, unsigned n, unsigned group) {
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_slab_g(
     const unsigned y = blockIdx.z * gridDim.y + blockIdx.y;
     if (y >= n) return;   // whole block
     const SqConst c = sq_const(r, uniform_u64(svec[y / group]));
-    square2_tile<false>(in + (u64)y * pitch, out + (u64)y * pitch, c, blockIdx.x, gw);
+    square2_tile<false, W12>(in + (u64)y * pitch, out + (u64)y * pitch, c, blockIdx.x, gw);
 }
 
 // slab: residue y at in + y*pitch, its s in svec[y]
@@ -361,8 +367,12 @@ extern "C" int gf2t_square_fold(const uint64_t *din, uint64_t *dout, uint64_t r,
         return GF2T_EINVAL;
     }
     SqConst c = sq_const(r, s);
-    hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((c.nw + 1) / 2, SQ_THREADS)),
-                       dim3(SQ_THREADS), 0, (hipStream_t)stream, din, dout, c);
+    if (sq_w12_ok(r))
+        hipLaunchKernelGGL(k_square_fold<true>, dim3(blocks_for((c.nw + 1) / 2, SQ_THREADS)),
+                           dim3(SQ_THREADS), 0, (hipStream_t)stream, din, dout, c);
+    else
+        hipLaunchKernelGGL(k_square_fold<false>, dim3(blocks_for((c.nw + 1) / 2, SQ_THREADS)),
+                           dim3(SQ_THREADS), 0, (hipStream_t)stream, din, dout, c);
     GF2T_CHECK_LAUNCH("k_square_fold");
     return GF2T_OK;
 }
@@ -377,17 +387,22 @@ extern "C" int gf2t_square_fold_steps(uint64_t *dres, uint64_t *dtmp, uint64_t r
     size_t nw = c.nw;
     hipStream_t st = (hipStream_t)stream;
     u64 *a = dres, *b = dtmp;
-    const bool fuse = sq_fuse();
+    const bool fuse = sq_fuse(), w12 = sq_w12_ok(r);
     const Sq2Chain h2 = sq2_chain(c);
+    const dim3 g2((unsigned)sq2_units(h2)), g1(blocks_for((nw + 1) / 2, SQ_THREADS));
     u64 launches = 0;
     for (u64 k = 0; k < steps; launches++) {
         if (fuse && steps - k >= 2) {
-            hipLaunchKernelGGL(k_square2_fold, dim3((unsigned)sq2_units(h2)), dim3(SQ_THREADS), 0,
-                               st, a, b, c, h2);
+            if (w12)
+                hipLaunchKernelGGL(k_square2_fold<true>, g2, dim3(SQ_THREADS), 0, st, a, b, c, h2);
+            else
+                hipLaunchKernelGGL(k_square2_fold<false>, g2, dim3(SQ_THREADS), 0, st, a, b, c, h2);
             k += 2;
         } else {
-            hipLaunchKernelGGL(k_square_fold, dim3(blocks_for((nw + 1) / 2, SQ_THREADS)),
-                               dim3(SQ_THREADS), 0, st, a, b, c);
+            if (w12)
+                hipLaunchKernelGGL(k_square_fold<true>, g1, dim3(SQ_THREADS), 0, st, a, b, c);
+            else
+                hipLaunchKernelGGL(k_square_fold<false>, g1, dim3(SQ_THREADS), 0, st, a, b, c);
             k += 1;
         }
         std::swap(a, b);
@@ -407,14 +422,23 @@ struct BatchScratch {
     int dev = -1;
 } g_batch;
 
-template <int PPT>
+template <int PPT, bool W12>
 void launch_batch(unsigned gx, unsigned active, const SqDesc *tab, hipStream_t st) {
     if (sq_nt())
-        hipLaunchKernelGGL((k_square_fold_batch<PPT, true>), dim3(gx, active), dim3(SQ_THREADS), 0,
-                           st, tab);
+        hipLaunchKernelGGL((k_square_fold_batch<PPT, true, W12>), dim3(gx, active),
+                           dim3(SQ_THREADS), 0, st, tab);
     else
-        hipLaunchKernelGGL((k_square_fold_batch<PPT, false>), dim3(gx, active), dim3(SQ_THREADS),
+        hipLaunchKernelGGL((k_square_fold_batch<PPT, false, W12>), dim3(gx, active),
+                           dim3(SQ_THREADS), 0, st, tab);
+}
+template <bool W12>
+void launch_batch2(unsigned gx, unsigned active, const SqDesc *tab, hipStream_t st) {
+    if (sq_nt())
+        hipLaunchKernelGGL((k_square2_fold_batch<true, W12>), dim3(gx, active), dim3(SQ_THREADS),
                            0, st, tab);
+    else
+        hipLaunchKernelGGL((k_square2_fold_batch<false, W12>), dim3(gx, active),
+                           dim3(SQ_THREADS), 0, st, tab);
 }
 }  // namespace
 
@@ -536,7 +560,11 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
     // value is in dtmp when that count is odd).
     const bool fuse = sq_fuse();
     u64 gx2 = 0;   // work units of the residue with the most
-    for (int i = 0; i < n; i++) gx2 = std::max(gx2, sq2_units(sq2_chain(sq_const(r[i], s[i]))));
+    bool w12 = true;   // 12-byte window loads: no r of the batch a multiple of 64
+    for (int i = 0; i < n; i++) {
+        gx2 = std::max(gx2, sq2_units(sq2_chain(sq_const(r[i], s[i]))));
+        w12 = w12 && sq_w12_ok(r[i]);
+    }
     std::vector<u64> launches(n, 0);
     int active = n;
     u64 nl = 0;
@@ -545,17 +573,16 @@ extern "C" int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dt
         if (!active) break;
         const SqDesc *tab = g_batch.d + ((nl & 1) ? n : 0);
         if (fuse && steps[ord[active - 1]] - k >= 2) {
-            if (sq_nt())
-                hipLaunchKernelGGL(k_square2_fold_batch<true>, dim3((unsigned)gx2, active),
-                                   dim3(SQ_THREADS), 0, st, tab);
-            else
-                hipLaunchKernelGGL(k_square2_fold_batch<false>, dim3((unsigned)gx2, active),
-                                   dim3(SQ_THREADS), 0, st, tab);
+            if (w12) launch_batch2<true>((unsigned)gx2, active, tab, st);
+            else launch_batch2<false>((unsigned)gx2, active, tab, st);
             k += 2;
         } else {
-            if (ppt == 4) launch_batch<4>((unsigned)gx, active, tab, st);
-            else if (ppt == 2) launch_batch<2>((unsigned)gx, active, tab, st);
-            else launch_batch<1>((unsigned)gx, active, tab, st);
+            if (ppt == 4 && w12) launch_batch<4, true>((unsigned)gx, active, tab, st);
+            else if (ppt == 4) launch_batch<4, false>((unsigned)gx, active, tab, st);
+            else if (ppt == 2 && w12) launch_batch<2, true>((unsigned)gx, active, tab, st);
+            else if (ppt == 2) launch_batch<2, false>((unsigned)gx, active, tab, st);
+            else if (w12) launch_batch<1, true>((unsigned)gx, active, tab, st);
+            else launch_batch<1, false>((unsigned)gx, active, tab, st);
             k += 1;
         }
     }
@@ -667,9 +694,14 @@ extern "C" int gf2t_square2_fold_slab_g(const uint64_t *din, uint64_t *dout, siz
     const unsigned gx = (unsigned)(((nw + 1) / 2 + SQ2_TP - 1) / SQ2_TP);
     const unsigned gy = n < 65535 ? (unsigned)n : 65535u;
     const unsigned gz = (unsigned)((n + gy - 1) / gy);
-    hipLaunchKernelGGL(k_square2_fold_slab_g, dim3(gx, gy, gz), dim3(SQ_THREADS), 0,
-                       (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
-                       (unsigned)group);
+    if (sq_w12_ok(r))
+        hipLaunchKernelGGL(k_square2_fold_slab_g<true>, dim3(gx, gy, gz), dim3(SQ_THREADS), 0,
+                           (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
+                           (unsigned)group);
+    else
+        hipLaunchKernelGGL(k_square2_fold_slab_g<false>, dim3(gx, gy, gz), dim3(SQ_THREADS), 0,
+                           (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
+                           (unsigned)group);
     GF2T_CHECK_LAUNCH("k_square2_fold_slab_g");
     return GF2T_OK;
 }

@@ -25,6 +25,26 @@ struct CheckedTile {
     }
 };
 
+// the kernels' global source with every 32-bit read checked against the
+// 2 nw words of the residue; w12 as in SqArrayT<true> (word q + 2 read
+// for every window), used where sq_w12_ok(r) lets the kernels use it
+struct CheckedArray {
+    const u64 *p;
+    size_t nw;
+    bool w12;
+    u64 operator()(int, size_t k) const { return k < nw ? p[k] : 0; }
+    u32 w32(u32 q) const {
+        if ((size_t)q >= 2 * nw) abort();
+        return (q & 1) ? (u32)(p[q >> 1] >> 32) : (u32)p[q >> 1];
+    }
+    void raw3(int, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
+        if (sh > 31) abort();
+        a = w32(q);
+        b = w32(q + 1);
+        c = w32(w12 ? q + 2 : q + 1 + (sh != 0));
+    }
+};
+
 extern "C" {
 void kl_square_fold(const u64 *in, u64 *out, u64 r, u64 s) {
     size_t nw = (size_t)((r + 63) >> 6);
@@ -39,14 +59,15 @@ void kl_square_fold_fast(const u64 *in, u64 *out, u64 r, u64 s) {
     SqConst c = sq_const(r, s);
     for (size_t i = 0; i < (c.nw + 1) / 2; i++) {
         u64 o0, o1;
-        square_fold_pair_fast(in, c, i, o0, o1);
+        square_fold_pair_fast_src(CheckedArray{in, c.nw, sq_w12_ok(r)}, c, i, o0, o1);
         out[2 * i] = o0;
         if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
     }
 }
 // the G phase of one tile: the windows of pl.compute, one G pair per
 // thread, into the plan's slots
-static void g_phase(const u64 *in, const SqConst &c, const Sq2Plan &pl, std::vector<u64> &gw,
+static void g_phase(const CheckedArray &in, const SqConst &c, const Sq2Plan &pl,
+                    std::vector<u64> &gw,
                     std::vector<u64> &holds, u64 &formed) {
     for (int k = 0; k < SQ2_WINS; k++) {
         if (!((pl.compute >> k) & 1)) continue;
@@ -56,7 +77,7 @@ static void g_phase(const u64 *in, const SqConst &c, const Sq2Plan &pl, std::vec
         for (u32 t = 0; t < SQ2_GP; t++) {
             u64 o0, o1;
             if (greg < 3) {
-                SqRawPair w = sq_raw_pair(SqArray{in, c.nw}, c, (u32)(p0 + t), greg);
+                SqRawPair w = sq_raw_pair(in, c, (u32)(p0 + t), greg);
                 sq_combine_pair(w, c, greg, o0, o1);
             } else {
                 sq2_g_pair_clipped(in, c, p0 + t, o0, o1);
@@ -94,7 +115,7 @@ void kl_square2_fold(const u64 *in, u64 *out, u64 r, u64 s) {
         const int treg = sq_tile_region(c, i0, i1);
         const Sq2Plan pl = sq2_plan(c, i0, sq2_need(treg));
         std::fill(holds.begin(), holds.end(), ~0ULL);
-        g_phase(in, c, pl, gw, holds, formed);
+        g_phase(CheckedArray{in, c.nw, sq_w12_ok(r)}, c, pl, gw, holds, formed);
         h_phase(c, i0, i1, treg, pl, gw, holds, out, hits);
     }
 }
@@ -126,7 +147,7 @@ void kl_square2_fold_units(const u64 *in, u64 *out, u64 r, u64 s, u64 *stats) {
             const u32 need = sq2_need(treg);
             const Sq2Plan pl = sq2_chain_plan(c, h, i0, w.width, need, j - w.j0, prev_need, prev_base4);
             if ((pl.compute & ~need) || !(pl.compute & 2u)) abort();
-            g_phase(in, c, pl, gw, holds, formed);
+            g_phase(CheckedArray{in, c.nw, sq_w12_ok(r)}, c, pl, gw, holds, formed);
             h_phase(c, i0, i1, treg, pl, gw, holds, out, hits);
             tiles++;
             prev_need = need;
@@ -143,14 +164,14 @@ void kl_square2_fold_units(const u64 *in, u64 *out, u64 r, u64 s, u64 *stats) {
 void kl_square_fold_tiles(const u64 *in, u64 *out, u64 r, u64 s, u32 tile_pairs) {
     SqConst c = sq_const(r, s);
     const size_t np = (c.nw + 1) / 2;
-    const SqArray src{in, c.nw};
+    const CheckedArray src{in, c.nw, sq_w12_ok(r)};
     for (size_t t0 = 0; t0 < np; t0 += tile_pairs) {
         const size_t t1 = t0 + tile_pairs < np ? t0 + tile_pairs : np;
         const int treg = sq_tile_region(c, t0, t1);
         for (size_t i = t0; i < t1; i++) {
             u64 o0, o1;
             if (treg < 3) sq_pair_region(src, c, (u32)i, treg, o0, o1);
-            else square_fold_pair_fast(in, c, i, o0, o1);
+            else square_fold_pair_fast_src(src, c, i, o0, o1);
             out[2 * i] = o0;
             if (2 * i + 1 < c.nw) out[2 * i + 1] = o1;
         }

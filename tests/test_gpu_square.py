@@ -187,6 +187,37 @@ def test_batch_full_size_two_squarings(eng):
         assert np.array_equal(b.download(nw), O.square_fold_steps(a, R136, s, 2)), s
 
 
+def test_r_multiple_of_64(eng):
+    """r a multiple of 64 takes the window loads that never read past a
+    window's last word (the 12-byte form could read one word past the
+    residue there): single chains of one to five steps, and a batch that
+    mixes such an r with odd ones."""
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+    import ctypes as C
+    rng = np.random.default_rng(64)
+    for r in (128, 4160, 8192, 262144):
+        for s in sorted({1, 63, 64, r // 3, r // 2 - 1} & set(range(1, (r + 1) // 2))):
+            a = rand_res(rng, r)
+            for k in (1, 2, 5):
+                assert np.array_equal(eng.square_fold(a, r, s, k), O.square_fold_steps(a, r, s, k)), (r, s, k)
+    cases = [(4160, 1000, 4), (4423, 1200, 4), (262144, 70001, 2), (8192, 4095, 3)]
+    ins = [rand_res(rng, r) for r, _, _ in cases]
+    bufs = [DeviceBuffer.from_array(x) for x in ins]
+    tmps = [DeviceBuffer.words(O.words(r)) for r, _, _ in cases]
+    n = len(cases)
+    P = C.POINTER(C.c_uint64)
+    rr = np.array([c[0] for c in cases], dtype=np.uint64)
+    ss = np.array([c[1] for c in cases], dtype=np.uint64)
+    kk = np.array([c[2] for c in cases], dtype=np.uint64)
+    check(lib.gf2t_square_fold_batch((C.c_void_p * n)(*[b.ptr for b in bufs]),
+                                     (C.c_void_p * n)(*[t.ptr for t in tmps]),
+                                     rr.ctypes.data_as(P), ss.ctypes.data_as(P),
+                                     kk.ctypes.data_as(P), n, None))
+    for (r, s, k), x, b in zip(cases, ins, bufs):
+        assert np.array_equal(b.download(O.words(r)), O.square_fold_steps(x, r, s, k)), (r, s, k)
+
+
 @pytest.mark.parametrize("r,s", [(4423, 1200), (4097, 1), (4097, 2048), (131, 65), (132049, 777)])
 def test_fold_products(eng, r, s):
     from mersenne_trinomial_amd._capi import check, lib

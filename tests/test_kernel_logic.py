@@ -174,6 +174,32 @@ def test_two_squarings_per_tile(kl):
             assert np.array_equal(out, one), (r, s, tp)
 
 
+def test_window_loads_every_r_mod_64(kl):
+    """r = 4097..4160 covers every r mod 64.  Where r is not a multiple of
+    64 the kernels read each window's three words as one 12-byte load,
+    word q + 2 also when the shift is 0; the emulation's source aborts on
+    any read past the 2 nw words, so this checks that those reads stay in
+    bounds (and r = 4160 takes the other form), through the one-step
+    tiles, the two-squaring tiles and the work units."""
+    rng = np.random.default_rng(17)
+    for r in range(4097, 4161):
+        h = (r - 1) // 2
+        for s in sorted({1, 2, 63, 64, 65, r // 3, h - 1, h, int(rng.integers(1, h + 1))}):
+            a = rand_res(rng, r)
+            one = O.square_fold(a, r, s)
+            two = O.square_fold(one, r, s)
+            out = np.zeros(O.words(r), np.uint64)
+            kl.kl_square_fold_fast(ptr(a), ptr(out), r, s)
+            assert np.array_equal(out, one), (r, s)
+            for tp in (1, 512):
+                kl.kl_square_fold_tiles(ptr(a), ptr(out), r, s, tp)
+                assert np.array_equal(out, one), (r, s, tp)
+            kl.kl_square2_fold(ptr(a), ptr(out), r, s)
+            assert np.array_equal(out, two), (r, s)
+            out2, _ = units(kl, a, r, s)
+            assert np.array_equal(out2, two), (r, s)
+
+
 def units(kl, a, r, s):
     out = np.zeros(O.words(r), np.uint64)
     st = np.zeros(3, np.uint64)
