@@ -526,13 +526,15 @@ struct Sq2Tile {
     GF2T_HDM u64 operator()(int k, size_t w) const {
         return win[at[k] + (u32)(w - base[k])];
     }
-    // as SqArray::raw3, from the copy's 32-bit words: q, q + 1 and
-    // q + 1 + (sh != 0)
+    // as SqArray::raw3, from the copy's 32-bit words q, q + 1 and q + 2
+    // for every shift: the copy holds the word after a window's last one
+    // (the funnel shift's word when sh != 0), so q + 2 is always inside it
     GF2T_HDM void raw3(int k, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
+        (void)sh;
         const u32 *w = (const u32 *)win + (2 * at[k] + (u32)(q - 2 * base[k]));
         a = w[0];
         b = w[1];
-        c = w[1 + (sh != 0)];
+        c = w[2];
     }
 };
 // what the G phase of one tile computes, and where each window's copy

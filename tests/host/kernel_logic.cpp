@@ -21,7 +21,7 @@ struct CheckedTile {
     }
     void raw3(int k, u32 q, u32 sh, u32 &a, u32 &b, u32 &c) const {
         if (sh > 31) abort();
-        a = ld32(k, q); b = ld32(k, q + 1); c = ld32(k, q + 1 + (sh != 0));
+        a = ld32(k, q); b = ld32(k, q + 1); c = ld32(k, q + 2);
     }
 };
 
