@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_square_fold_batch_r136279841_256res.md and
 # profiles/k_square2_fold_batch_r136279841_256res.md.
 SQ_TRAFFIC_256 = {
-    "k_square_fold_batch": 6731434803 + 4360962867,    # one squaring per launch
-    "k_square2_fold_batch": 8156244582 + 4366682624,   # two squarings per launch
+    "k_square_fold_batch": 6732439347 + 4360962560,    # one squaring per launch
+    "k_square2_fold_batch": 8157166592 + 4366682419,   # two squarings per launch
 }
 
 
