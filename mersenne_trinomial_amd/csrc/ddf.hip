@@ -25,7 +25,9 @@ constexpr int DDF_MAX_M = 64;
 unsigned blocks_for(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // squaring of a slab in groups: residue y (grid-stride over y) uses
-// svec[y / group]
+// svec[y / group]; W12 as in the squaring kernels (12-byte window loads,
+// for r not a multiple of 64)
+template <bool W12>
 __global__ __launch_bounds__(DDF_THREADS) void k_square_fold_slab_g(
         const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
         const u64 *__restrict__ svec, unsigned n, unsigned group) {
@@ -36,7 +38,7 @@ __global__ __launch_bounds__(DDF_THREADS) void k_square_fold_slab_g(
         const u64 *src = in + (u64)y * pitch;
         u64 *dst = out + (u64)y * pitch;
         u64 o0, o1;
-        square_fold_pair_fast(src, c, i, o0, o1);
+        square_fold_pair_fast<W12>(src, c, i, o0, o1);
         dst[2 * i] = o0;
         if (2 * i + 1 < c.nw) dst[2 * i + 1] = o1;
     }
@@ -148,9 +150,14 @@ extern "C" int gf2t_square_fold_slab_g(const uint64_t *din, uint64_t *dout, size
     }
     if (n == 0) return GF2T_OK;
     unsigned gx = blocks_for((nw + 1) / 2, DDF_THREADS);
-    hipLaunchKernelGGL(k_square_fold_slab_g, dim3(gx, grid_y((unsigned)n)), dim3(DDF_THREADS), 0,
-                       (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
-                       (unsigned)group);
+    if (sq_w12_ok(r))
+        hipLaunchKernelGGL(k_square_fold_slab_g<true>, dim3(gx, grid_y((unsigned)n)),
+                           dim3(DDF_THREADS), 0, (hipStream_t)stream, din, dout, (u64)pitch, r,
+                           ds, (unsigned)n, (unsigned)group);
+    else
+        hipLaunchKernelGGL(k_square_fold_slab_g<false>, dim3(gx, grid_y((unsigned)n)),
+                           dim3(DDF_THREADS), 0, (hipStream_t)stream, din, dout, (u64)pitch, r,
+                           ds, (unsigned)n, (unsigned)group);
     GF2T_CHECK_LAUNCH("k_square_fold_slab_g");
     return GF2T_OK;
 }
@@ -255,8 +262,12 @@ extern "C" int gf2t_ddf_init_slab(uint64_t *dh, uint64_t *dh_alt, size_t slot_pi
                 // h[i] ^= x h[i-1]
                 u64 *prev = dh + (size_t)(i - 1) * slot_pitch, *cur = dh + (size_t)i * slot_pitch;
                 u64 *tmp = dh_alt + (size_t)(i - 1) * slot_pitch;
-                hipLaunchKernelGGL(k_square_fold_slab_g, gp, dim3(DDF_THREADS), 0, st, prev, tmp,
-                                   (u64)rp, r, ds, (unsigned)n, 1u);
+                if (sq_w12_ok(r))
+                    hipLaunchKernelGGL(k_square_fold_slab_g<true>, gp, dim3(DDF_THREADS), 0, st,
+                                       prev, tmp, (u64)rp, r, ds, (unsigned)n, 1u);
+                else
+                    hipLaunchKernelGGL(k_square_fold_slab_g<false>, gp, dim3(DDF_THREADS), 0, st,
+                                       prev, tmp, (u64)rp, r, ds, (unsigned)n, 1u);
                 hipLaunchKernelGGL(k_copy_slab, gw, dim3(DDF_THREADS), 0, st, prev, (u64)rp, tmp,
                                    (u64)rp, nw, (unsigned)n);
                 hipLaunchKernelGGL(k_mulx_xor_slab, gw, dim3(DDF_THREADS), 0, st, cur, (u64)rp,
