@@ -97,41 +97,63 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         SCAN_CHECK(gf2t_memcpy_h2d(cur, host.data(), host.size() * 8, nullptr));
         SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)B, 1, P.k0, nullptr));
     }
+    // Work still queued on the device when the cohort ends (a speculative
+    // interval) must finish before the buffers above are freed; declared
+    // after them, this runs first.
+    struct SyncOnExit {
+        ~SyncOnExit() { gf2t_device_sync(); }
+    } sync_on_exit;
     int n = B;
     u64 k = P.k0, L = P.L0 ? P.L0 : 64;
     const u64 Lmax = P.Lmax ? P.Lmax : 4096;
     const int threads = P.gcd_threads > 0 ? P.gcd_threads : 8;
     std::vector<u64> acc_host;
-    while (n > 0) {
-        u64 ka = k + 1;
-        if (ka > rhigh) break;
-        u64 kend = std::min(k + L, rhigh);
-        u64 kb = kend;
-        SCAN_CHECK(gf2t_set_one_slab(ACC.w(), pitch, nw, n, nullptr));
+    // the device work of the interval after frontier k0 (length L0, on the
+    // first nslot slots): its products accumulate in ACC; returns its end
+    auto enqueue = [&](u64 k0, u64 L0, int nslot, u64 &kb) -> int {
+        const u64 ka = k0 + 1;
+        const u64 kend = std::min(k0 + L0, rhigh);
+        kb = kend;
+        SCAN_CHECK(gf2t_set_one_slab(ACC.w(), pitch, nw, nslot, nullptr));
         if (m > 1) {
             // whole blocks: the last may run past rhigh; factors found
             // beyond it are not reported
-            u64 nblk = (kend - k + (u64)m - 1) / (u64)m;
-            kb = k + nblk * (u64)m;
+            u64 nblk = (kend - k0 + (u64)m - 1) / (u64)m;
+            kb = k0 + nblk * (u64)m;
             for (u64 b = 0; b < nblk; b++) {
-                SCAN_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, n, nullptr));
+                SCAN_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, nslot, nullptr));
                 SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(),
-                                            pitch, n, WORK.p, nullptr));
-                SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)n * m, m,
+                                            pitch, nslot, WORK.p, nullptr));
+                SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)nslot * m, m,
                                              (u64)m, nullptr));
             }
         } else {
             for (u64 deg_k = ka; deg_k <= kb; deg_k++) {
-                SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), n, nullptr));
+                SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), nslot, nullptr));
                 std::swap(cur, oth);
                 SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, cur, pitch, 1, r, SV.w(), ACC.w(),
-                                            pitch, n, WORK.p, nullptr));
+                                            pitch, nslot, WORK.p, nullptr));
             }
         }
+        return GF2T_OK;
+    };
+    if (k + 1 > rhigh) return GF2T_OK;
+    u64 kb = 0;
+    SCAN_CHECK(enqueue(k, L, n, kb));
+    while (n > 0) {
+        // the interval (k, kb] of the n live slots is queued: wait for its
+        // products
         acc_host.assign((size_t)n * pitch, 0);
         SCAN_CHECK(gf2t_memcpy_d2h(acc_host.data(), ACC.p, acc_host.size() * 8, nullptr));
         double t1 = now_s();
         t_dev += t1 - t0;
+        // speculation (src/tsfactor.cu:1279-1331): the next interval goes
+        // to the device for every slot while the host takes this one's
+        // gcds; the slots that finish here simply drop its results
+        const u64 ka = k + 1, Ln = std::min(2 * L, Lmax);
+        const bool more = kb < rhigh;
+        u64 kb_next = 0;
+        if (more) SCAN_CHECK(enqueue(kb, Ln, n, kb_next));
         std::vector<Poly> g(n);
         parallel_for((size_t)n, threads, [&](size_t i) {
             Poly a = from_words(acc_host.data() + i * pitch, nw);
@@ -153,20 +175,25 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         });
         t0 = now_s();
         t_res += t0 - t2;
-        if (kb >= rhigh)
+        if (!more)
             for (int i = 0; i < n; i++)
                 if (!done[i]) {
                     gf2t_scan_result &R = rows[live[i].idx];
                     R.kind = limited ? GF2T_SCAN_GAVE_UP : GF2T_SCAN_PRIMITIVE;
                     done[i] = 1;
                 }
-        // compact the survivors of this interval to the front of the slabs
+        // compact the survivors to the front of the slabs: their advanced
+        // Frobenius state and, when the next interval is queued, its
+        // products (the copies queue behind that interval's launches)
         int nl = 0;
         for (int i = 0; i < n; i++) {
             if (done[i]) continue;
             if (nl != i) {
                 SCAN_CHECK(gf2t_memcpy_d2d(cur + (size_t)nl * tab, cur + (size_t)i * tab,
                                            tab * 8, nullptr));
+                if (more)
+                    SCAN_CHECK(gf2t_memcpy_d2d(ACC.w() + (size_t)nl * pitch,
+                                               ACC.w() + (size_t)i * pitch, pitch * 8, nullptr));
                 live[nl] = live[i];
                 svals[nl] = svals[i];
             }
@@ -176,15 +203,16 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)nl * 8, nullptr));
         if (P.verbose)
             fprintf(stderr, "  interval (%llu,%llu]: %d live, %zu hits, %d continue"
-                    " (m=%d; device %.3fs, gcd %.3fs, resolve %.3fs so far)\n",
+                    " (m=%d; device wait %.3fs, gcd %.3fs, resolve %.3fs so far)\n",
                     (unsigned long long)k, (unsigned long long)kb, n, hit.size(), nl, m, t_dev,
                     t_gcd, t_res);
         n = nl;
         k = kb;
-        L = std::min(2 * L, Lmax);
+        L = Ln;
+        kb = kb_next;
         if (n > 0 && P.progress) {
-            // every s still live is clear through degree kb
-            const u64 kdone = std::min(kb, rhigh);
+            // every s still live is clear through degree k
+            const u64 kdone = std::min(k, rhigh);
             if (P.progress(P.progress_user, kdone, svals.data(), (size_t)n)) {
                 for (int i = 0; i < n; i++) rows[live[i].idx].kind = GF2T_SCAN_STOPPED;
                 stopped = true;
