@@ -266,8 +266,14 @@ bool resolve_least(const Poly &g, u64 ka, u64 kb, const Poly &h0, u64 &d, Poly &
     Modulus G(g);
     Poly h = G.reduce(h0);
     const u64 B = 32;
-    for (u64 k0 = ka; k0 <= kb; k0 += B) {
-        const u64 k1 = std::min(kb, k0 + B - 1);
+    // With two or more irreducible factors (counted with multiplicity) the
+    // least has degree at most deg g / 2; with one, that factor is g.  So
+    // the walk stops at deg g / 2, and if it finds nothing g is irreducible
+    // (its degree then lies in [ka, kb] like every factor's).
+    const u64 gd = (u64)deg(g);
+    const u64 kmax = std::min(kb, gd / 2);
+    for (u64 k0 = ka; k0 <= kmax; k0 += B) {
+        const u64 k1 = std::min(kmax, k0 + B - 1);
         const Poly start = h;
         Poly P{1};
         for (u64 k = k0; k <= k1; k++) {
@@ -286,6 +292,11 @@ bool resolve_least(const Poly &g, u64 ka, u64 kb, const Poly &h0, u64 &d, Poly &
             }
         }
         return false;   // the block's product hit but no single degree did
+    }
+    if (gd >= ka && gd <= kb) {   // no factor up to deg g / 2: g is irreducible
+        d = gd;
+        mask = g;
+        return true;
     }
     return false;
 }

@@ -50,7 +50,9 @@ private:
 
 // least factor degree d in (ka-1, kb] of g, whose irreducible factors all
 // have degrees in [ka, kb], and the numerically least degree-d factor.
-// h0 = x^(2^(ka-1)) mod g.  Returns false if nothing is found.
+// h0 = x^(2^(ka-1)) mod g.  Degrees are tried up to deg g / 2 only (a
+// least factor of a reducible g lies there; else g itself is the factor).
+// Returns false if nothing is found.
 bool resolve_least(const Poly &g, u64 ka, u64 kb, const Poly &h0, u64 &d, Poly &mask);
 
 // numerically least irreducible factor of t, all of whose irreducible
