@@ -74,18 +74,35 @@ inline void clmul(u64 a, u64 b, u64 &lo, u64 &hi) {
     hi = (u64)_mm_extract_epi64(p, 1);
 }
 
-// out[0, na+nb) ^= a[0,na) * b[0,nb)
+// out[0, na+nb) ^= a[0,na) * b[0,nb), column by column: column k sums the
+// 128-bit products a[i] b[k-i] in registers (two per pair of 16-byte
+// loads, into two accumulators) and passes its high half to column k+1,
+// so each output word is written once
 void school(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
-    for (size_t i = 0; i < na; i++) {
-        u64 ai = a[i];
-        if (!ai) continue;
-        for (size_t j = 0; j < nb; j++) {
-            u64 lo, hi;
-            clmul(ai, b[j], lo, hi);
-            out[i + j] ^= lo;
-            out[i + j + 1] ^= hi;
+    if (!na || !nb) return;
+    const size_t nk = na + nb - 1;
+    u64 carry = 0;
+    for (size_t k = 0; k < nk; k++) {
+        const size_t i0 = k >= nb ? k - nb + 1 : 0;
+        const size_t i1 = k < na ? k : na - 1;        // last i, inclusive
+        __m128i s0 = _mm_setzero_si128(), s1 = _mm_setzero_si128();
+        size_t i = i0;
+        for (; i < i1; i += 2) {
+            // (a[i], a[i+1]) against (b[k-i-1], b[k-i])
+            const __m128i A = _mm_loadu_si128((const __m128i *)(a + i));
+            const __m128i B = _mm_loadu_si128((const __m128i *)(b + (k - i - 1)));
+            s0 = _mm_xor_si128(s0, _mm_clmulepi64_si128(A, B, 0x10));
+            s1 = _mm_xor_si128(s1, _mm_clmulepi64_si128(A, B, 0x01));
         }
+        if (i == i1)
+            s0 = _mm_xor_si128(s0, _mm_clmulepi64_si128(_mm_loadl_epi64((const __m128i *)(a + i)),
+                                                       _mm_loadl_epi64((const __m128i *)(b + (k - i))),
+                                                       0x00));
+        s0 = _mm_xor_si128(s0, s1);
+        out[k] ^= (u64)_mm_cvtsi128_si64(s0) ^ carry;
+        carry = (u64)_mm_extract_epi64(s0, 1);
     }
+    out[nk] ^= carry;
 }
 
 // out[0, 2n) = a[0,n) * b[0,n); ws holds at least 4n words per level

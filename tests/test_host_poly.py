@@ -139,3 +139,20 @@ def test_cert_formats(tmp_path):
     assert certs.parse_line("12 u") == (12, "u", None, None)
     seeded = certs.best_from_survivors([2, 5], 6, 30)
     assert seeded[2] == certs.NONE and seeded[5] == certs.NONE and seeded[3] == np.uint64(29 << 48)
+
+
+def test_host_products_vs_bitserial(tmp_path):
+    """The gcd's host products (column schoolbook, Karatsuba, unbalanced
+    blocks, parallel halves) against a bit-serial product, with sizes
+    around every switch point, zero words and all-ones words."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    csrc = os.path.join(root, "mersenne_trinomial_amd", "csrc")
+    exe = tmp_path / "host_mul_check"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-mpclmul", "-msse4.1", "-pthread",
+                           "-I" + os.path.join(root, "include"), "-I" + csrc, "-o", str(exe),
+                           os.path.join(root, "tests", "host", "host_mul_check.cpp"),
+                           os.path.join(csrc, "hgcd.cpp"), os.path.join(csrc, "host_poly.cpp")])
+    p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0 and "host products ok" in p.stdout, p.stdout + p.stderr
