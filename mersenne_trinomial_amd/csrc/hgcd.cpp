@@ -308,22 +308,29 @@ W2 hgcd_word(u64 c, u64 d, int stop, int &steps) {
     return W;
 }
 
-// out = w0 x + w1 y (word polynomials times multiword ones)
+// out = w0 x + w1 y (word polynomials times multiword ones), one pass:
+// word i is lo(w0 x[i] + w1 y[i]) plus the previous word's high half
 void lin2(u64 w0, const Poly &x, u64 w1, const Poly &y, Poly &out) {
-    size_t n = std::max(x.size(), y.size()) + 1;
-    out.assign(n, 0);
-    for (size_t i = 0; i < x.size(); i++) {
-        u64 lo, hi;
-        clmul(w0, x[i], lo, hi);
-        out[i] ^= lo;
-        out[i + 1] ^= hi;
+    const size_t nx = x.size(), ny = y.size(), n = std::max(nx, ny);
+    out.resize(n + 1);
+    const __m128i W = _mm_set_epi64x((long long)w1, (long long)w0);
+    u64 carry = 0;
+    size_t i = 0;
+    for (const size_t nb = std::min(nx, ny); i < nb; i++) {
+        const __m128i X = _mm_set_epi64x((long long)y[i], (long long)x[i]);
+        const __m128i p = _mm_xor_si128(_mm_clmulepi64_si128(W, X, 0x00),
+                                        _mm_clmulepi64_si128(W, X, 0x11));
+        out[i] = (u64)_mm_cvtsi128_si64(p) ^ carry;
+        carry = (u64)_mm_extract_epi64(p, 1);
     }
-    for (size_t i = 0; i < y.size(); i++) {
-        u64 lo, hi;
-        clmul(w1, y[i], lo, hi);
-        out[i] ^= lo;
-        out[i + 1] ^= hi;
+    const Poly &z = nx > ny ? x : y;
+    const __m128i Z = _mm_cvtsi64_si128((long long)(nx > ny ? w0 : w1));
+    for (; i < n; i++) {
+        const __m128i p = _mm_clmulepi64_si128(Z, _mm_cvtsi64_si128((long long)z[i]), 0x00);
+        out[i] = (u64)_mm_cvtsi128_si64(p) ^ carry;
+        carry = (u64)_mm_extract_epi64(p, 1);
     }
+    out[n] = carry;
     trim(out);
 }
 
