@@ -15,10 +15,12 @@
 // once more on the top of the new pair.
 #include "host_poly.hpp"
 #include "gf2t.h"
+#include <immintrin.h>
 #include <wmmintrin.h>
 #include <smmintrin.h>
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <future>
 #include <stdexcept>
@@ -63,7 +65,7 @@ std::future<void> spawn(F &&f) {
 }
 constexpr size_t PAR_WORDS = 512;   // products with both operands this large may go parallel
 
-constexpr size_t KBASE = 24;           // words: schoolbook below this
+constexpr size_t KBASE = 32;           // words: schoolbook below this
 constexpr int64_t HGCD_BASE = 1536;    // bits: step-by-step half-gcd below this
 constexpr int64_t GCD_BASE = 4096;     // bits: plain Euclid below this
 
@@ -78,7 +80,7 @@ inline void clmul(u64 a, u64 b, u64 &lo, u64 &hi) {
 // 128-bit products a[i] b[k-i] in registers (two per pair of 16-byte
 // loads, into two accumulators) and passes its high half to column k+1,
 // so each output word is written once
-void school(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
+void school128(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
     if (!na || !nb) return;
     const size_t nk = na + nb - 1;
     u64 carry = 0;
@@ -103,6 +105,66 @@ void school(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
         carry = (u64)_mm_extract_epi64(s0, 1);
     }
     out[nk] ^= carry;
+}
+
+// The same with 512-bit VPCLMULQDQ (four 64x64 products per instruction),
+// for nb <= 64: b is reversed once (rb[t] = b[nb-1-t]), so column k is the
+// lane-aligned sum of a[i0 + u] rb[t0 + u], u < len, eight words per step
+// (masked loads zero the tail).
+__attribute__((target("avx512f,vpclmulqdq")))
+void school512(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
+    u64 rb[64];
+    for (size_t t = 0; t < nb; t++) rb[t] = b[nb - 1 - t];
+    const size_t nk = na + nb - 1;
+    u64 carry = 0;
+    for (size_t k = 0; k < nk; k++) {
+        const size_t i0 = k >= nb ? k - nb + 1 : 0;
+        const size_t i1 = k < na ? k : na - 1;
+        const size_t len = i1 - i0 + 1, t0 = nb - 1 - (k - i0);
+        const u64 *pa = a + i0, *pr = rb + t0;
+        __m512i s = _mm512_setzero_si512();
+        size_t u = 0;
+        for (; u + 8 <= len; u += 8) {
+            const __m512i A = _mm512_loadu_si512(pa + u), R = _mm512_loadu_si512(pr + u);
+            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x00));
+            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x11));
+        }
+        if (u < len) {
+            const __mmask8 m = (__mmask8)((1u << (len - u)) - 1);
+            const __m512i A = _mm512_maskz_loadu_epi64(m, pa + u), R = _mm512_maskz_loadu_epi64(m, pr + u);
+            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x00));
+            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x11));
+        }
+        const __m256i h = _mm256_xor_si256(_mm512_castsi512_si256(s), _mm512_extracti64x4_epi64(s, 1));
+        const __m128i q = _mm_xor_si128(_mm256_castsi256_si128(h), _mm256_extracti128_si256(h, 1));
+        out[k] ^= (u64)_mm_cvtsi128_si64(q) ^ carry;
+        carry = (u64)_mm_extract_epi64(q, 1);
+    }
+    out[nk] ^= carry;
+}
+
+// The 512-bit form where the CPU has it (AVX-512F and VPCLMULQDQ), unless
+// GF2T_NO_VPCLMUL is set
+bool use512() {
+    static const bool yes = [] {
+        __builtin_cpu_init();
+        const char *e = getenv("GF2T_NO_VPCLMUL");
+        return !(e && *e && *e != '0') && __builtin_cpu_supports("avx512f") &&
+               __builtin_cpu_supports("vpclmulqdq");
+    }();
+    return yes;
+}
+
+void school(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
+    if (!na || !nb) return;
+    if (na < nb) {
+        std::swap(a, b);
+        std::swap(na, nb);
+    }
+    if (nb <= 64 && use512())
+        school512(a, na, b, nb, out);
+    else
+        school128(a, na, b, nb, out);
 }
 
 // out[0, 2n) = a[0,n) * b[0,n); ws holds at least 4n words per level

@@ -1,7 +1,7 @@
-// Host carry-less products of hgcd.cpp (column schoolbook below the
-// Karatsuba base, Karatsuba above it, block splitting for unbalanced
-// operands, parallel half products from 1024 words) against a bit-serial
-// shift-and-xor product.  Device products are off (gcd_dev_bits 0), so
+// Host carry-less products of hgcd.cpp (column schoolbook, 512-bit or
+// 128-bit, below the Karatsuba base, Karatsuba above it, block splitting
+// for unbalanced operands, parallel half products from 1024 words)
+// against a bit-serial shift-and-xor product.  Device products are off (gcd_dev_bits 0), so
 // the device entry points below are never reached; they abort if called.
 #include "host_poly.hpp"
 #include "gf2t.h"
@@ -47,11 +47,12 @@ int main() {
         if (n) p[n - 1] |= 1ULL << (rng() % 64);   // top word nonzero
         return p;
     };
-    const size_t sizes[][2] = {{1, 1},   {1, 2},     {2, 1},    {2, 2},    {3, 5},   {7, 7},
-                               {23, 24}, {24, 24},   {25, 25},  {24, 25},  {47, 49}, {48, 48},
-                               {49, 50}, {1, 300},   {300, 1},  {5, 200},  {23, 1000}, {25, 1000},
-                               {96, 97}, {130, 129}, {333, 77}, {512, 512}, {700, 900},
-                               {1100, 1100}};
+    const size_t sizes[][2] = {
+        {1, 1},     {1, 2},    {2, 1},     {2, 2},     {3, 5},     {7, 7},    {23, 24},
+        {24, 24},   {25, 25},  {31, 33},   {32, 32},   {33, 33},   {47, 49},  {48, 48},
+        {49, 50},   {64, 65},  {1, 300},   {300, 1},   {5, 200},   {23, 1000}, {31, 1000},
+        {32, 1000}, {33, 1000}, {96, 97},  {130, 129}, {333, 77},  {512, 512}, {700, 900},
+        {1100, 1100}};
     int bad = 0, n = 0;
     for (auto &s : sizes)
         for (int zero_every : {0, 3}) {
