@@ -280,6 +280,7 @@ constexpr int LP_MIN_D = 26;
 constexpr int LP_SB = 15;            // 2^15 values per placed slice (128 KiB)
 constexpr int LP_E = 16384;          // entries per block in passes 1 and 2
 constexpr int LP_T = 1024;           // threads per block in passes 1 and 2
+constexpr int LP_B1 = 9;            // level-1 partition bits (fewer when d - LP_SB is smaller)
 
 struct LpShape {
     int B1, B2;                      // partition bits, level 1 and 2
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(LP_T) void k_lp_part1(DegreeParams P, LpShape S,
                                                    unsigned long long *__restrict__ out,
                                                    unsigned *__restrict__ cursor,
                                                    unsigned *__restrict__ bad) {
-    __shared__ unsigned cnt[256], off[256], gbase[256], wtot[LP_T / 64];
+    __shared__ unsigned cnt[1 << LP_B1], off[1 << LP_B1], gbase[1 << LP_B1], wtot[LP_T / 64];
     __shared__ unsigned long long stage[LP_E];
     const unsigned nb = 1u << S.B1;
     const int shift = P.d - S.B1;
@@ -407,7 +408,7 @@ __global__ void k_fill_u32(unsigned *p, u64 n) {
 
 LpShape lp_shape(int d) {
     LpShape S;
-    S.B1 = std::min(8, d - LP_SB);
+    S.B1 = std::min(LP_B1, d - LP_SB);
     S.B2 = d - LP_SB - S.B1;
     return S;
 }
@@ -989,7 +990,7 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     if ((rc = narrow_mem(dmax, mem))) return rc;
     if (dmax >= LP_MIN_D) {
         LpShape S = lp_shape(dmax);
-        if ((rc = bCur.ensure(((size_t)1 << (S.B1 + S.B2)) * 4 + 256 * 4))) return rc;
+        if ((rc = bCur.ensure(((size_t)1 << (S.B1 + S.B2)) * 4 + (1 << LP_B1) * 4))) return rc;
         static std::once_flag lds_once;
         static hipError_t lds_err = hipSuccess;
         std::call_once(lds_once, [] {
@@ -1039,8 +1040,8 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         if (d >= LP_MIN_D) {
             const LpShape S = lp_shape(d);
             const unsigned nb1 = 1u << S.B1, nb2 = 1u << S.B2;
-            unsigned *cur1 = (unsigned *)bCur.p, *cur2 = cur1 + 256;
-            const u64 ncur = 256 + (u64)nb1 * nb2;
+            unsigned *cur1 = (unsigned *)bCur.p, *cur2 = cur1 + (1 << LP_B1);
+            const u64 ncur = (1 << LP_B1) + (u64)nb1 * nb2;
             hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((ncur + 255) / 256)), dim3(256), 0, 0,
                                cur1, ncur);
             hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
