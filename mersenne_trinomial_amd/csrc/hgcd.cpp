@@ -65,7 +65,7 @@ std::future<void> spawn(F &&f) {
 }
 constexpr size_t PAR_WORDS = 512;   // products with both operands this large may go parallel
 
-constexpr size_t KBASE = 32;           // words: schoolbook below this
+constexpr size_t KBASE = 48;           // words: schoolbook below this
 constexpr int64_t HGCD_BASE = 1536;    // bits: step-by-step half-gcd below this
 constexpr int64_t GCD_BASE = 4096;     // bits: plain Euclid below this
 
