@@ -165,13 +165,7 @@ int main(int argc, char **argv) {
     }
     std::string rpath = prefix + ".results.txt";
     std::set<u64> done;
-    if (FILE *f = fopen(rpath.c_str(), "r")) {
-        char line[256];
-        u64 s;
-        while (fgets(line, sizeof line, f))
-            if (sscanf(line, "%" SCNu64, &s) == 1) done.insert(s);
-        fclose(f);
-    }
+    scan_ckpt::read_done(rpath, done);
     std::vector<u64> todo;
     for (u64 s : surv)
         if (!done.count(s)) todo.push_back(s);
@@ -199,15 +193,23 @@ int main(int argc, char **argv) {
         scan_ckpt::Ckpt c;
         if (!W.path.empty() && scan_ckpt::load(W.path, c)) {
             if (c.p == W.p) {
-                for (u64 s : c.s)
-                    if (!done.count(s)) resume[s] = c.k_done;
-                fprintf(stderr, "resuming %zu s from %s (clear through degree %" PRIu64 ")\n",
-                        resume.size(), W.path.c_str(), c.k_done);
+                u64 klo = ~0ULL, khi = 0;
+                for (const auto &e : c.ent)
+                    if (!done.count(e.first)) {
+                        u64 &k = resume[e.first];
+                        k = std::max(k, e.second);
+                        klo = std::min(klo, e.second);
+                        khi = std::max(khi, e.second);
+                    }
+                if (!resume.empty())
+                    fprintf(stderr, "resuming %zu s from %s (clear through degree %" PRIu64
+                                    "..%" PRIu64 ")\n", resume.size(), W.path.c_str(), klo, khi);
             } else {
                 fprintf(stderr, "checkpoint %s is for other parameters; ignored\n", W.path.c_str());
             }
         }
     }
+    W.carry = resume;
     P.progress = &scan_ckpt::Writer::cb;
     P.progress_user = &W;
     scan_ckpt::install_signals();
@@ -217,19 +219,21 @@ int main(int argc, char **argv) {
     u64 nfound = 0, nu = 0, nrange = 0, nprim = 0;
     for (size_t at = 0; at < todo.size() && !W.stopped; at += chunk) {
         std::vector<u64> sv(todo.begin() + at, todo.begin() + std::min(todo.size(), at + chunk));
-        // resumed s values first (by their degree), then the rest from k0
+        // resumed s values first (deepest first), then the rest from k0
         std::map<u64, std::vector<u64>> by_k0;
         for (u64 s : sv) {
             auto it = resume.find(s);
             by_k0[it == resume.end() ? P.k0 : std::max(P.k0, it->second)].push_back(s);
         }
         std::vector<std::string> lines;
-        for (auto &g : by_k0) {
+        std::vector<u64> finished;
+        for (auto g = by_k0.rbegin(); g != by_k0.rend(); ++g) {
             if (W.stopped) break;
             gf2t_scan_params Q = P;
-            Q.k0 = g.first;
+            Q.k0 = g->first;
             std::vector<std::string> part;
-            gf2t::check(scan_lines(r, g.second, Q, part), "gf2t_scan_run");
+            gf2t::check(scan_lines(r, g->second, Q, part), "gf2t_scan_run");
+            for (const std::string &ln : part) finished.push_back(strtoull(ln.c_str(), 0, 10));
             lines.insert(lines.end(), part.begin(), part.end());
         }
         for (const std::string &ln : lines) {
@@ -240,6 +244,8 @@ int main(int argc, char **argv) {
             else nu++;
         }
         fflush(fres);
+        fsync(fileno(fres));
+        W.flushed(finished);
         fprintf(stderr, "%zu/%zu scanned (%.1fs)\n", std::min(todo.size(), at + chunk), todo.size(),
                 now_s() - t0);
     }

@@ -17,11 +17,13 @@
 // reference reports it.  -z gives up where the reference's interval
 // schedule (linear, or --sched opt) first exceeds Q blocks; -Z, --q0 and
 // the schedule otherwise only shape the intervals, which do not change
-// any line.  The per-s TSFC checkpoint (<out>.ckpt, not read here) is
-// replaced by <out>.gf2t.ckpt: the degree through which the s values of
-// the running cohort are clear, written every --ckpt-mins minutes and on
+// any line.  Checkpoints go to <out>.gf2t.ckpt: the degree through which
+// each unfinished s is clear, written every --ckpt-mins minutes and on
 // SIGINT / SIGTERM (the run then writes the results it has and exits with
-// status 3); rerunning the same command resumes them from there.
+// status 3); rerunning the same command resumes them from there.  The
+// reference's per-s TSFC checkpoint (<out>.ckpt and its rotations) of an
+// in-flight reference run is read too: its s resumes at the start of the
+// interval it was in.
 #include "gf2t.h"
 #include "gf2t_engine.hpp"
 #include "scan_ckpt.hpp"
@@ -221,8 +223,10 @@ static int tsfactor_lines(u64 r, const std::vector<u64> &sv, const Opts &O,
         G.past = !cut;
         G.idx.push_back(i);
     }
-    for (auto &kv : groups) {
-        Group &G = kv.second;
+    // resumed s (larger k0) first, so an interruption during the fresh
+    // cohort does not leave them waiting behind it
+    for (auto it = groups.rbegin(); it != groups.rend(); ++it) {
+        Group &G = it->second;
         if (W && W->stopped) break;   // the rest restart from the checkpoint or --skip
         gf2t_scan_params P = {};
         P.k0 = G.k0;
@@ -483,6 +487,35 @@ int main(int argc, char **argv) {
         fprintf(stderr, "-z and -Z are mutually exclusive\n");
         return 1;
     }
+    const bool m_given = O.m != 0;
+    const std::string res_path = out_prefix + ".results.txt";
+    std::set<u64> done;
+    if (FILE *f = fopen(res_path.c_str(), "r")) {
+        fclose(f);
+        scan_ckpt::read_done(res_path, done);
+        fprintf(stderr, "%zu already completed in %s\n", done.size(), res_path.c_str());
+    }
+    // the reference's own checkpoint of an in-flight run: its s resumes at
+    // the start of the interval it was in (every degree below is clear)
+    std::map<u64, u64> resume;
+    if (!no_ckpt) {
+        scan_ckpt::Tsfc T;
+        std::string which;
+        if (scan_ckpt::load_tsfc(out_prefix + ".ckpt", T, &which)) {
+            if (T.r != r || T.skip != O.skip || done.count(T.s)) {
+                fprintf(stderr, "checkpoint %s ignored (r/skip mismatch or s already done)\n",
+                        which.c_str());
+            } else if (m_given && (u64)O.m != T.m) {
+                fprintf(stderr, "checkpoint %s is for m=%" PRIu64 ", not --m %d; ignored\n",
+                        which.c_str(), T.m, O.m);
+            } else {
+                O.m = (int)T.m;
+                resume[T.s] = T.k_iv - 1;
+                fprintf(stderr, "resuming s=%" PRIu64 " from %s (clear through degree %" PRIu64
+                                ", m=%d)\n", T.s, which.c_str(), T.k_iv - 1, O.m);
+            }
+        }
+    }
     if (!O.m) O.m = default_m(r);
     if (O.m < 2 || O.m > 64 || 2 * (u64)O.m >= r) {
         fprintf(stderr, "--m must be in 2..64 with 2m < r\n");
@@ -515,22 +548,6 @@ int main(int argc, char **argv) {
         fclose(f);
     }
     fprintf(stderr, "%zu survivors from %s\n", svs.size(), pos[1].c_str());
-    const std::string res_path = out_prefix + ".results.txt";
-    std::set<u64> done;
-    if (FILE *f = fopen(res_path.c_str(), "r")) {
-        char line[256];
-        while (fgets(line, sizeof line, f)) {
-            u64 s = strtoull(line, 0, 10);
-            if (s) done.insert(s);
-        }
-        fclose(f);
-        fprintf(stderr, "%zu already completed in %s\n", done.size(), res_path.c_str());
-    }
-    if (FILE *f = fopen((out_prefix + ".ckpt").c_str(), "rb")) {
-        fclose(f);
-        fprintf(stderr, "note: %s.ckpt (the reference's own checkpoint) is not read here\n",
-                out_prefix.c_str());
-    }
     std::vector<u64> todo;
     for (u64 s : svs)
         if (!done.count(s)) todo.push_back(s);
@@ -551,20 +568,29 @@ int main(int argc, char **argv) {
     W.verbose = O.verbose;
     W.last = scan_ckpt::Writer::now();
     if (!no_ckpt) W.path = out_prefix + ".gf2t.ckpt";
-    std::map<u64, u64> resume;
     {
         scan_ckpt::Ckpt c;
         if (!W.path.empty() && scan_ckpt::load(W.path, c)) {
             if (c.p == W.p) {
-                for (u64 s : c.s)
-                    if (!done.count(s)) resume[s] = c.k_done;
-                fprintf(stderr, "resuming %zu s from %s (clear through degree %" PRIu64 ")\n",
-                        resume.size(), W.path.c_str(), c.k_done);
+                size_t nres = 0;
+                u64 klo = ~0ULL, khi = 0;
+                for (const auto &e : c.ent)
+                    if (!done.count(e.first)) {
+                        u64 &k = resume[e.first];
+                        k = std::max(k, e.second);
+                        nres++;
+                        klo = std::min(klo, e.second);
+                        khi = std::max(khi, e.second);
+                    }
+                if (nres)
+                    fprintf(stderr, "resuming %zu s from %s (clear through degree %" PRIu64
+                                    "..%" PRIu64 ")\n", nres, W.path.c_str(), klo, khi);
             } else {
                 fprintf(stderr, "checkpoint %s is for other parameters; ignored\n", W.path.c_str());
             }
         }
     }
+    W.carry = resume;
     scan_ckpt::install_signals();
     FILE *rf = fopen(res_path.c_str(), "a");
     if (!rf) {
@@ -578,9 +604,11 @@ int main(int argc, char **argv) {
         std::vector<u64> sv(todo.begin() + at, todo.begin() + std::min(todo.size(), at + chunk));
         std::vector<std::string> lines;
         gf2t::check(tsfactor_lines(r, sv, O, resume, &W, lines), "tsfactor scan");
+        std::vector<u64> finished;
         for (size_t i = 0; i < lines.size(); i++) {
             const std::string &ln = lines[i];
             if (ln.empty()) continue;   // stopped before it finished
+            finished.push_back(sv[i]);
             fprintf(rf, "%s\n", ln.c_str());
             printf("%s\n", ln.c_str());
             if (ln.find(" primitive") != std::string::npos) {
@@ -592,6 +620,8 @@ int main(int argc, char **argv) {
             else nu++;
         }
         fflush(rf);
+        fsync(fileno(rf));
+        W.flushed(finished);
         fflush(stdout);
         if (O.verbose)
             fprintf(stderr, "%zu/%zu scanned (%.1fs)\n", std::min(todo.size(), at + chunk),
