@@ -9,12 +9,20 @@ One step = one pass of the GPU hot path over one batch of synthetic input:
       advanced 512 squarings (one checkpoint interval) in the batched
       form, starting from F = x; two squarings per launch
       (k_square2_fold_batch: F is read once and F^4 written once).
-value = certified s-values / step wall time (sieve certificates only; the
-chain time is included in the step but certifies nothing by itself).
 roofline = the squaring kernel of the chain: algorithmic bytes 16*nw per
 residue per launch (read the residue, write its square or fourth power)
 times 256 residues, over its average launch time measured with HIP
 events on its stream.
+
+value = s-values certified per second on the deep-scan workload, one
+pass timed after the steps: 256 s drawn by SplitMix64 from
+[3, 68139920]; the coarse sieve to degree 32 certifies most of them, and
+its survivors go through the batched deep scan (blocked distinct-degree
+products on the device, an interval gcd every 512 degrees on host
+threads with device products, resolution of hits) to degree 4096, each s
+leaving at its first hit.  The same s values then run through the plain
+form (one s at a time, as the reference's per-s loop runs them) and
+their certificate lines must equal the batched ones (`c3` object).
 """
 from __future__ import annotations
 
@@ -77,8 +85,18 @@ def parse():
                     help="serve mode: longest s slice (the mix draws 10^3..10^6, capped here)")
     ap.add_argument("--serve-interval", type=int, default=2048,
                     help="serve mode: deep-scan interval length (gcds per s = 2048 / this)")
+    ap.add_argument("--c3", type=int, default=1,
+                    help="run the deep-scan workload (256 seeded s, sieve then deep scan) once "
+                         "after the timed steps; its rate is the line's value")
+    ap.add_argument("--c3-n", type=int, default=256)
+    ap.add_argument("--c3-seed", type=int, default=20250101)
+    ap.add_argument("--c3-maxd", type=int, default=4096)
+    ap.add_argument("--c3-interval", type=int, default=512)
+    ap.add_argument("--c3-plain", type=int, default=1,
+                    help="also run the same s one at a time (the plain per-s form)")
     ap.add_argument("--mode",
-                    choices=["full", "squaring", "fft", "scan", "pipeline", "serve", "sieve"],
+                    choices=["full", "squaring", "fft", "scan", "pipeline", "serve", "sieve",
+                             "c3"],
                     default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
@@ -115,6 +133,113 @@ def cpu_baseline(depth: int, chain_s: int):
                   f"oracle squaring+fold {1e3 * t_sq:.1f} ms per residue-step (1 thread)",
         "squaring_fold_residue_steps_per_s_1core": 1.0 / t_sq,
     }
+
+
+M64 = (1 << 64) - 1
+
+
+def splitmix_draw(seed: int, n: int, lo: int, hi: int) -> list[int]:
+    """n distinct integers in [lo, hi] from SplitMix64 (Steele, Lea and
+    Flood; the generator the reference seeds its random inputs with,
+    src/tsfactor.cu:1993-1998), in draw order."""
+    st, out, seen = seed & M64, [], set()
+    while len(out) < n:
+        st = (st + 0x9E3779B97F4A7C15) & M64
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+        z ^= z >> 31
+        v = lo + z % (hi - lo + 1)
+        if v not in seen:
+            seen.add(v)
+            out.append(v)
+    return out
+
+
+def c3_svals(seed: int, n: int = 256) -> list[int]:
+    """The deep-scan workload's s values: n distinct s drawn uniformly from
+    [3, 68139920] by SplitMix64 from `seed`."""
+    return splitmix_draw(seed, n, 3, SMAX)
+
+
+def c3_run(svals, depth: int, maxd: int, interval: int, batch: int, gcd_threads: int,
+           dbest=None):
+    """Certify every s of svals at r = 136279841: the coarse sieve over all
+    s <= 68139920 to `depth`, its lines for the s it certifies, then the
+    deep scan of the rest from degree depth+1 to maxd with an interval gcd
+    every `interval` degrees and per-s early exit.  batch = 0: one device
+    cohort (the batched form); batch = 1: one s at a time (the reference's
+    per-s loop).  Returns (lines by s, phase times)."""
+    import mersenne_trinomial_amd as M
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+    own = dbest is None
+    if own:
+        dbest = DeviceBuffer.words(SMAX + 1)
+    t0 = time.perf_counter()
+    dbest.fill(0xFF)
+    check(lib.gf2t_sieve(R, 2, depth, SMAX, dbest.ptr, None), "gf2t_sieve")
+    key = np.zeros(1, dtype=np.uint64)
+    lines, surv = {}, []
+    for s in svals:
+        check(lib.gf2t_memcpy_d2h(key.ctypes.data, dbest.ptr + 8 * s, 8, None))
+        k = int(key[0])
+        if k == 0xFFFFFFFFFFFFFFFF:
+            surv.append(s)
+        else:
+            lines[s] = f"{s} {k >> 48} p{k & 0xFFFFFFFFFFFF:x}"
+    device.synchronize()
+    t1 = time.perf_counter()
+    times = {}
+    out = M.scan(R, surv, k0=depth, maxd=maxd, L0=interval, Lmax=interval, block_m=0,
+                 gcd_threads=gcd_threads, batch=batch, times=times, verbose=1) if surv else []
+    t2 = time.perf_counter()
+    kinds = {}
+    for o in out:
+        lines[o.s] = o.line()
+        kinds[o.kind] = kinds.get(o.kind, 0) + 1
+    if own:
+        del dbest
+    return lines, {"seconds": t2 - t0, "sieve_s": t1 - t0, "scan_s": t2 - t1,
+                   "survivors": len(surv), "found": kinds.get(1, 0), "u": kinds.get(2, 0),
+                   **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in times.items()}}
+
+
+def c3_measure(args, dbest):
+    """The batched deep scan and the plain one-s-at-a-time form on the same
+    seeded s values; the certificate lines must be identical."""
+    svals = c3_svals(args.c3_seed, args.c3_n)
+    threads = host_threads()
+    kw = dict(depth=args.depth, maxd=args.c3_maxd, interval=args.c3_interval,
+              gcd_threads=threads, dbest=dbest)
+    print(f"c3: {len(svals)} s (seed {args.c3_seed}), batched form", file=sys.stderr, flush=True)
+    lines_b, tb = c3_run(svals, batch=0, **kw)
+    print(f"c3: batched {tb}", file=sys.stderr, flush=True)
+    res = {"n_s": len(svals), "seed": args.c3_seed, "depth": args.depth, "maxd": args.c3_maxd,
+           "interval": args.c3_interval, "gcd_threads": threads,
+           "s_per_s": len(svals) / tb["seconds"], **tb}
+    if args.c3_plain:
+        print("c3: plain form (one s at a time)", file=sys.stderr, flush=True)
+        lines_p, tp = c3_run(svals, batch=1, **kw)
+        print(f"c3: plain {tp}", file=sys.stderr, flush=True)
+        res["plain"] = {"s_per_s": len(svals) / tp["seconds"], **tp}
+        res["ratio"] = tp["seconds"] / tb["seconds"]
+        res["certificates_equal"] = lines_p == lines_b
+    import hashlib
+    text = "".join(lines_b[s] + "\n" for s in sorted(lines_b))
+    res["lines_md5"] = hashlib.md5(text.encode()).hexdigest()
+    res["lines_sample"] = [lines_b[s] for s in sorted(lines_b)
+                           if " p" in lines_b[s] and int(lines_b[s].split()[1]) > args.depth][:8]
+    return res
+
+
+def host_threads() -> int:
+    """Host threads this process is given: OMP_NUM_THREADS when set (the
+    GPU box sets it to the job's CPU share), else the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    aff = len(os.sched_getaffinity(0))
+    return min(aff, int(env)) if env.isdigit() and int(env) > 0 else aff
 
 
 def mode_squaring(args):
@@ -443,6 +568,11 @@ def main():
         return mode_serve(args)
     if args.mode == "sieve":
         return mode_sieve(args)
+    if args.mode == "c3":
+        from mersenne_trinomial_amd import device
+        device.init(0)
+        print(json.dumps({"mode": "c3", **c3_measure(args, None)}), flush=True)
+        return
 
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib
@@ -517,8 +647,10 @@ def main():
     ec1.record()
     device.synchronize()
     copy_gbs = 2 * nbytes16 / (ec0.elapsed_ms(ec1) * 1e-3) / 1e9
-    del big_a, big_b
-    value = certified / (wall / args.steps)
+    del big_a, big_b, res, tmp
+    sieve_rate = certified / (float(np.mean(sieve_ms)) * 1e-3)
+    c3 = c3_measure(args, dbest) if args.c3 else None
+    value = c3["s_per_s"] if c3 else sieve_rate
 
     kernel, launches, per = chain_launches(args.chain_steps, args.sq_fuse != 0)
     launch_ms = float(np.mean(chain_ms)) / (args.chain_steps / per)
@@ -541,12 +673,22 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {
-            "workload": f"r={R}: coarse sieve d=2..{args.depth} over all s in 1..{SMAX} + "
-                        f"{args.chain_steps} batched squaring+fold steps on {n} sieve survivors "
-                        f"(17 MB residues, {per} squarings per launch)",
+            "workload": (f"r={R} deep-scan workload: {args.c3_n} s drawn by SplitMix64 (seed "
+                         f"{args.c3_seed}) from [3, {SMAX}]; coarse sieve d=2..{args.depth} over "
+                         f"all s, then the batched deep scan of its survivors to degree "
+                         f"{args.c3_maxd} with an interval gcd every {args.c3_interval} degrees "
+                         f"and per-s early exit; value = s certified / wall seconds of that pass"
+                         if c3 else
+                         f"r={R}: coarse sieve d=2..{args.depth} over all s in 1..{SMAX}; value = "
+                         f"s certified by the sieve / sieve time"),
+            "steps": (f"each timed step: coarse sieve d=2..{args.depth} over all s in 1..{SMAX} + "
+                      f"{args.chain_steps} batched squaring+fold steps on {n} sieve survivors "
+                      f"(17 MB residues, {per} squarings per launch); the roofline is the "
+                      f"squaring kernel of these steps"),
             "r": R, "smax": SMAX, "depth": args.depth, "residues": n,
             "chain_steps": args.chain_steps, "seed": args.seed,
         },
+        "c3": c3,
         "roofline": {
             "bound": "hbm",
             "kernel": kernel,
@@ -563,7 +705,7 @@ def main():
             "avg_launch_ms": launch_ms,
         },
         "sieve": {"ms": float(np.mean(sieve_ms)), "certified": certified,
-                  "survivors": survivors, "per_degree": deg},
+                  "survivors": survivors, "s_certified_per_s": sieve_rate, "per_degree": deg},
         "squaring_fold": {"ms": float(np.mean(chain_ms)),
                           "residue_steps_per_s": n * args.chain_steps / (np.mean(chain_ms) * 1e-3),
                           "GB_per_s": achieved},

@@ -348,6 +348,13 @@ int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
 size_t gf2t_scan_results_count(const gf2t_scan_results *res);
 int gf2t_scan_results_get(const gf2t_scan_results *res, size_t i,
                           gf2t_scan_result *row, char *hexbuf, size_t cap);
+/* Wall seconds the run spent waiting for the device, in interval gcds
+ * and in resolving hits (summed over its cohorts; the device work of the
+ * next interval overlaps the gcds), and the number of interval gcds.
+ * Any pointer may be NULL.  The reference prints the same phases per s
+ * (src/tsfactor.cu:1334-1346). */
+int gf2t_scan_results_times(const gf2t_scan_results *res, double *dev_s, double *gcd_s,
+                            double *resolve_s, uint64_t *n_gcd);
 /* the certificate line of result i ("s d p<hex>", "s u", ...) */
 int gf2t_scan_line(const gf2t_scan_results *res, size_t i, char *buf,
                    size_t cap);

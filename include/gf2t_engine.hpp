@@ -212,6 +212,149 @@ struct TrinGPU {
     }
 };
 
+// Device context of the blocked distinct-degree scan (TsGPU,
+// src/tsfactor.cu:207-391): the h-table ping-pong dH[2] of m slots (slot
+// j at dH[hcur] + j*hpitch; hpitch is nw rounded up to 32 words so that
+// the two-squaring kernels can run on the table), the interval
+// accumulator dA, the block polynomial dC and a 2r-bit product buffer
+// dS0.  h_to_host / h_from_host exchange the table in the reference's
+// packed layout (slot j at word j*nw), the h section of its TSFC
+// checkpoint.  Every member has the reference's meaning:
+//   init_h_at(k)      slot j = e_(j+1)(x^(2^k), ..., x^(2^(k+m-1))) mod T
+//   square_all_step() every slot squared once (the table of block k+1)
+//   horner_C()        dC = x^m + sum_j h_j x^(m-1-j) mod T
+//                        = prod_t (x + x^(2^(k+t))) mod T
+//   mul_full(x, y)    dS0 = x*y (2r-1 bits);  reduce_into(dst): dS0 mod T
+//   mul_A_by_C()      dA = dA * dC mod T;     reset_A(): dA = 1
+// legacy_sq is accepted: the reference's three-kernel squaring gives the
+// same words as the single pass used here.
+struct TsGPU {
+    u64 r = 0, s = 0;
+    size_t nw = 0, nw2 = 0, nwc = 0, hpitch = 0;
+    int m = 0;
+    int fftm = 0;
+    CantorBasis Bb;
+    DevWords H0, H1, Hold, A, Cw, Tmp, S0, S1, W, Sdev;
+    u64 *dH[2] = {nullptr, nullptr};
+    int hcur = 0;
+    u64 *dHold = nullptr, *dA = nullptr, *dC = nullptr, *dTmp = nullptr, *dS0 = nullptr,
+        *dS1 = nullptr;
+    bool legacy_sq = false;
+    double t_sq = 0, t_horner = 0, t_mul = 0;
+
+    void init(u64 r_) {
+        r = r_;
+        nw = bits_to_words(r);
+        nw2 = 2 * nw;
+        hpitch = (nw + 31) & ~(size_t)31;
+        Bb.build();
+        if (!Bb.verify()) {
+            fprintf(stderr, "Cantor basis failure\n");
+            exit(1);
+        }
+        size_t ca = (size_t)((r + 31) >> 5);
+        fftm = 1;
+        while (((size_t)1 << fftm) < 2 * ca - 1) fftm++;
+        A.alloc(nw);
+        Cw.alloc(nw + 1);
+        Tmp.alloc(nw);
+        S0.alloc(mul_out_words(r, r));
+        S1.alloc(nw2);
+        W.alloc((gf2t_mul_workspace_bytes(r, r) + 7) / 8);
+        Sdev.alloc(1);
+        dA = A.p; dC = Cw.p; dTmp = Tmp.p; dS0 = S0.p; dS1 = S1.p;
+    }
+    void set_m(int m_) {
+        if (m_ < 2 || m_ > 64) {
+            fprintf(stderr, "need 2 <= m <= 64\n");
+            exit(1);
+        }
+        if (m == m_) return;
+        m = m_;
+        nwc = bits_to_words(r + (u64)m - 1);
+        H0.alloc((size_t)m * hpitch);
+        H1.alloc((size_t)m * hpitch);
+        Hold.release();
+        dHold = nullptr;
+        dH[0] = H0.p;
+        dH[1] = H1.p;
+        hcur = 0;
+    }
+    void need_hold() {
+        if (!dHold) {
+            Hold.alloc((size_t)m * hpitch);
+            dHold = Hold.p;
+        }
+    }
+    void set_s(u64 s_) {
+        if (!(s_ >= 1 && 2 * s_ < r)) {
+            fprintf(stderr, "need 1 <= s <= (r-1)/2 (got s=%llu)\n", (unsigned long long)s_);
+            exit(1);
+        }
+        s = s_;
+        Sdev.upload(&s, 1);
+    }
+    void fini() {
+        H0.release(); H1.release(); Hold.release(); A.release(); Cw.release(); Tmp.release();
+        S0.release(); S1.release(); W.release(); Sdev.release();
+        dH[0] = dH[1] = dHold = dA = dC = dTmp = dS0 = dS1 = nullptr;
+        m = 0;
+    }
+    u64 *hslot(int j) { return dH[hcur] + (size_t)j * hpitch; }
+    u64 *hslot_o(int j) { return dH[1 - hcur] + (size_t)j * hpitch; }
+
+    void square_slot(const u64 *src, u64 *dst) {
+        check(gf2t_square_fold(src, dst, r, s, nullptr), "gf2t_square_fold");
+    }
+    void square_all_step() {
+        check(gf2t_square_fold_slab_g(dH[hcur], dH[1 - hcur], hpitch, r, Sdev.p, (size_t)m, m,
+                                      nullptr), "gf2t_square_fold_slab_g");
+        hcur = 1 - hcur;
+    }
+    void horner_C() {
+        check(gf2t_horner_slab(dH[hcur], hpitch, m, r, Sdev.p, dC, nw + 1, 1, nullptr),
+              "gf2t_horner_slab");
+    }
+    void mul_full(const u64 *dx, const u64 *dy) {
+        check(gf2t_mul(dx, r, dy, r, dS0, W.p, nullptr), "gf2t_mul");
+    }
+    void reduce_into(u64 *dst) {
+        check(gf2t_fold(dS0, 2 * r - 1, dst, r, s, nullptr), "gf2t_fold");
+    }
+    void mul_A_by_C() {
+        mul_full(dA, dC);
+        reduce_into(dA);
+    }
+    void reset_A() { check(gf2t_set_one(dA, nw, nullptr), "gf2t_set_one"); }
+    void init_h_at(u64 kstart) {
+        check(gf2t_ddf_init_slab(dH[hcur], dH[1 - hcur], hpitch, m, r, Sdev.p, kstart, 1, nullptr),
+              "gf2t_ddf_init_slab");
+    }
+    void copy_h(u64 *dst_base, const u64 *src_base) {
+        check(gf2t_memcpy_d2d(dst_base, src_base, (size_t)m * hpitch * 8, nullptr), "d2d");
+    }
+    void h_to_host(std::vector<u64> &out) {
+        std::vector<u64> tab((size_t)m * hpitch);
+        check(gf2t_memcpy_d2h(tab.data(), dH[hcur], tab.size() * 8, nullptr), "d2h");
+        out.resize((size_t)m * nw);
+        for (int j = 0; j < m; j++)
+            memcpy(out.data() + (size_t)j * nw, tab.data() + (size_t)j * hpitch, nw * 8);
+    }
+    void h_from_host(const std::vector<u64> &in) {
+        std::vector<u64> tab((size_t)m * hpitch, 0);
+        for (int j = 0; j < m; j++)
+            memcpy(tab.data() + (size_t)j * hpitch, in.data() + (size_t)j * nw, nw * 8);
+        check(gf2t_memcpy_h2d(dH[hcur], tab.data(), tab.size() * 8, nullptr), "h2d");
+    }
+    void A_to_host(std::vector<u64> &out) {
+        out.resize(nw);
+        check(gf2t_memcpy_d2h(out.data(), dA, nw * 8, nullptr), "d2h");
+    }
+    void A_from_host(const std::vector<u64> &in) {
+        check(gf2t_memcpy_h2d(dA, in.data(), nw * 8, nullptr), "h2d");
+    }
+};
+
 // Device product service with the member functions of the reference's
 // GpuMulHook (src/gf2_gcd.h:155-172): mul() is the full product, and
 // mat2_apply() the 2x2 matrix-vector product in one set of six forward

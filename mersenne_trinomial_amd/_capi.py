@@ -98,6 +98,8 @@ _SIGS = {
     "gf2t_scan_results_get": (C.c_int, [_vp, _sz, C.POINTER(ScanResult), C.c_char_p, _sz]),
     "gf2t_scan_line": (C.c_int, [_vp, _sz, C.c_char_p, _sz]),
     "gf2t_scan_results_free": (None, [_vp]),
+    "gf2t_scan_results_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), _p64]),
     "gf2t_gcd_host": (C.c_int64, [_p64, _sz, _p64, _sz, _p64, _sz]),
     "gf2t_mul_host": (C.c_int, [_p64, _u64, _p64, _u64, _p64]),
     "gf2t_sieve_release": (C.c_int, []),

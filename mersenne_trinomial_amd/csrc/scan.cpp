@@ -42,6 +42,11 @@ using namespace gf2t_scan;
 struct gf2t_scan_results {
     std::vector<gf2t_scan_result> rows;
     std::vector<std::string> hex;
+    // wall seconds of the run's phases, summed over its cohorts: waiting
+    // for the device, interval gcds, resolution of hits; and the number
+    // of interval gcds taken
+    double t_dev = 0, t_gcd = 0, t_res = 0;
+    uint64_t n_gcd = 0;
 };
 
 namespace {
@@ -64,8 +69,9 @@ struct Live {
 // src/tsfactor.cu:1242-1341), one product per m degrees.  Intervals end
 // with a host gcd(ACC, T_s) per residue.
 int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
-               std::vector<Live> live, std::vector<gf2t_scan_result> &rows,
-               std::vector<std::string> &hexes, bool &stopped) {
+               std::vector<Live> live, gf2t_scan_results &out, bool &stopped) {
+    std::vector<gf2t_scan_result> &rows = out.rows;
+    std::vector<std::string> &hexes = out.hex;
     if (P.k0 >= rhigh) {   // nothing left to scan: the bound is at or below the sieve depth
         for (const Live &L : live)
             rows[L.idx].kind = limited ? GF2T_SCAN_GAVE_UP : GF2T_SCAN_PRIMITIVE;
@@ -88,6 +94,11 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, nullptr));
     uint64_t *cur = T0.w(), *oth = T1.w();
     double t_dev = 0, t_gcd = 0, t_res = 0, t0 = now_s();
+    struct AddTimes {   // the cohort's phase times go into the results on every exit
+        gf2t_scan_results &o;
+        double &a, &b, &c;
+        ~AddTimes() { o.t_dev += a; o.t_gcd += b; o.t_res += c; }
+    } add_times{out, t_dev, t_gcd, t_res};
     if (m > 1) {
         // h-table for the first block, degrees k0+1 .. k0+m
         SCAN_CHECK(gf2t_ddf_init_slab(cur, oth, pitch, m, r, SV.w(), P.k0 + 1, B, nullptr));
@@ -155,6 +166,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         u64 kb_next = 0;
         if (more) SCAN_CHECK(enqueue(kb, Ln, n, kb_next));
         std::vector<Poly> g(n);
+        out.n_gcd += (uint64_t)n;
         parallel_for((size_t)n, threads, [&](size_t i) {
             Poly a = from_words(acc_host.data() + i * pitch, nw);
             g[i] = gcd_fast(trinomial(r, live[i].s), a);
@@ -288,7 +300,7 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
                 continue;
             }
             try {
-                rc = run_cohort(r, P, G.rhigh, G.limited, part, res->rows, res->hex, stopped);
+                rc = run_cohort(r, P, G.rhigh, G.limited, part, *res, stopped);
             } catch (const std::exception &e) {
                 gf2t::set_error("gf2t_scan_run: %s", e.what());
                 rc = GF2T_EHIP;
@@ -302,6 +314,16 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
 
 extern "C" size_t gf2t_scan_results_count(const gf2t_scan_results *res) {
     return res ? res->rows.size() : 0;
+}
+
+extern "C" int gf2t_scan_results_times(const gf2t_scan_results *res, double *dev_s,
+                                       double *gcd_s, double *resolve_s, uint64_t *n_gcd) {
+    if (!res) return GF2T_EINVAL;
+    if (dev_s) *dev_s = res->t_dev;
+    if (gcd_s) *gcd_s = res->t_gcd;
+    if (resolve_s) *resolve_s = res->t_res;
+    if (n_gcd) *n_gcd = res->n_gcd;
+    return GF2T_OK;
 }
 
 extern "C" int gf2t_scan_results_get(const gf2t_scan_results *res, size_t i,

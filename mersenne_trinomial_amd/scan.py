@@ -46,10 +46,13 @@ def swan_bound(r: int, s: int):
 
 def scan(r: int, svals, k0: int, maxd: int = 0, L0: int = 64, Lmax: int = 4096,
          gcd_threads: int = 0, batch: int = 0, verbose: int = 0,
-         block_m: int = 0, past_bound: bool = False) -> list[ScanOut]:
+         block_m: int = 0, past_bound: bool = False, times: dict | None = None) -> list[ScanOut]:
     """Scan every s in svals (sieve survivors at depth k0).  past_bound:
     report a factor found in the last whole block beyond the degree bound
-    (tsfactor's semantics) instead of 'u' / 'primitive'."""
+    (tsfactor's semantics) instead of 'u' / 'primitive'.  batch = 1 runs
+    the s values one at a time (the reference's per-s loop); 0 puts as
+    many in one device cohort as memory allows.  times (a dict) receives
+    the run's phase seconds: device_s, gcd_s, resolve_s, and n_gcd."""
     init()
     sv = np.ascontiguousarray(np.asarray(svals, dtype=np.uint64))
     P = ScanParams(k0=k0, maxd=maxd, L0=L0, Lmax=Lmax, gcd_threads=gcd_threads, batch=batch,
@@ -58,6 +61,12 @@ def scan(r: int, svals, k0: int, maxd: int = 0, L0: int = 64, Lmax: int = 4096,
     check(lib.gf2t_scan_run(r, sv.ctypes.data_as(C.POINTER(C.c_uint64)), sv.size, C.byref(P),
                             C.byref(res)), "gf2t_scan_run")
     try:
+        if times is not None:
+            tv = [C.c_double() for _ in range(3)]
+            ng = C.c_uint64()
+            check(lib.gf2t_scan_results_times(res, *[C.byref(x) for x in tv], C.byref(ng)))
+            times.update(device_s=tv[0].value, gcd_s=tv[1].value, resolve_s=tv[2].value,
+                         n_gcd=int(ng.value))
         out = []
         n = lib.gf2t_scan_results_count(res)
         for i in range(n):
