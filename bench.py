@@ -63,7 +63,6 @@ def parse():
     ap.add_argument("--chain-steps", type=int, default=512)
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-depth", type=int, default=24)
     ap.add_argument("--sq-ppt", type=int, default=0, help="override the squaring pairs/lane")
     ap.add_argument("--sq-order", type=int, default=-1, help="override the squaring tile order")
     ap.add_argument("--sq-nt", type=int, default=-1, help="override streaming stores (0/1)")
@@ -94,9 +93,11 @@ def parse():
     ap.add_argument("--c3-interval", type=int, default=512)
     ap.add_argument("--c3-plain", type=int, default=1,
                     help="also run the same s one at a time (the plain per-s form)")
+    ap.add_argument("--gcd-conc", type=int, default=8,
+                    help="gcd mode: gcds run at once after the single one")
     ap.add_argument("--mode",
                     choices=["full", "squaring", "fft", "scan", "pipeline", "serve", "sieve",
-                             "c3"],
+                             "c3", "gcd"],
                     default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
@@ -107,32 +108,90 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(depth: int, chain_s: int):
-    """The oracle (CPU restatement) on a bounded sample of the same
-    workload: the sieve at r=136279841 over all s <= 68139920 for degrees
-    2..depth on all host threads, plus two squaring+fold steps of one
-    full-size residue on one thread."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(certified: int, chain_s: int, c3: dict | None):
+    """The oracle (the CPU restatement) on the same inputs as the sieve leg:
+    r = 136279841, every s <= 68139920, degrees 2..32.  On all the job's
+    cores: degrees 2..24 in one run, 25 and 26 one degree each, and 27..32
+    extrapolated from degree 26 (per-degree work is dominated by the 2^d
+    table entries, so each degree costs twice the one before).  On one
+    core: 2..20, then 21 and 22, and 23..32 extrapolated from 22.  Plus
+    two full-size squaring+fold steps on one core, and from those a lower
+    bound on the CPU time of the deep-scan workload (its squarings only)."""
     import oracle as O
-    cores = min(16, len(os.sched_getaffinity(0)))
-    t0 = time.perf_counter()
-    best, _ = O.sieve(R, depth, SMAX, nthreads=cores)
-    t_sieve = time.perf_counter() - t0
-    cert = int(np.count_nonzero(best[1:] != O.NONE))
-    del best
+    cores = host_threads()
+
+    def sieve_time(nthreads, d_all, d_each):
+        t0 = time.perf_counter()
+        best, _ = O.sieve(R, d_all, SMAX, nthreads=nthreads)
+        t_all = time.perf_counter() - t0
+        per = {}
+        for d in d_each:
+            t0 = time.perf_counter()
+            best, _ = O.sieve(R, d, SMAX, dmin=d, nthreads=nthreads, best=best)
+            per[d] = time.perf_counter() - t0
+        del best
+        last = d_each[-1]
+        extra = {d: per[last] * 2.0 ** (d - last) for d in range(last + 1, 33)}
+        total = t_all + sum(per.values()) + sum(extra.values())
+        return total, {"measured_2_to_%d_s" % d_all: round(t_all, 3),
+                       "measured_per_degree_s": {d: round(v, 3) for d, v in per.items()},
+                       "extrapolated_per_degree_s": {d: round(v, 2) for d, v in extra.items()},
+                       "total_s": round(total, 2)}
+
+    t_n, info_n = sieve_time(cores, 24, [25, 26])
+    t_1, info_1 = sieve_time(1, 20, [21, 22])
     x = O.x_residue(R)
     t0 = time.perf_counter()
     O.square_fold_steps(x, R, chain_s, 2)
     t_sq = (time.perf_counter() - t0) / 2
-    return {
-        "value": cert / t_sieve,
+    out = {
+        "value": certified / t_n,
         "unit": "s-values certified/sec",
         "cores": cores,
         "kind": "port",
-        "sample": f"oracle sieve r={R}, all s<={SMAX}, degrees 2..{depth} "
-                  f"({cert} certified in {t_sieve:.2f} s on {cores} threads); "
-                  f"oracle squaring+fold {1e3 * t_sq:.1f} ms per residue-step (1 thread)",
+        "sample": (f"oracle coarse sieve at r={R} over all s<=68139920, degrees 2..32 (the "
+                   f"sieve leg's inputs; {certified} certified): degrees 2..26 measured on "
+                   f"{cores} threads, 27..32 extrapolated (x2 per degree) = {t_n:.1f} s; "
+                   f"one core: {t_1:.1f} s (2..22 measured)"),
+        "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "cpu_model": cpu_model(),
+        "sieve_all_cores": info_n,
+        "sieve_one_core": info_1,
+        "one_core_value": certified / t_1,
         "squaring_fold_residue_steps_per_s_1core": 1.0 / t_sq,
     }
+    out["sieve_s_per_s"] = certified / t_n
+    if c3:
+        # the deep-scan workload: the same sieve, then for its survivors the
+        # squarings of the first interval (degrees 33..544) of the blocked
+        # scheme, 32 per degree, spread over the cores; the interval
+        # products and gcds are left out, so this bounds the CPU time from
+        # below and its rate from above
+        sq = c3["survivors"] * 512 * 32 * t_sq / cores
+        out["value"] = c3["n_s"] / (t_n + sq)
+        out["sample"] = (f"deep-scan workload ({c3['n_s']} s) on the oracle, {cores} threads: the "
+                         f"coarse sieve of all s<=68139920 to degree 32 ({t_n:.1f} s: degrees "
+                         f"2..26 measured, 27..32 extrapolated x2 per degree) + the "
+                         f"{c3['survivors']} survivors' first-interval squarings "
+                         f"({c3['survivors']} x 512 x 32 x {1e3 * t_sq:.1f} ms / {cores} = "
+                         f"{sq:.1f} s); interval products and gcds omitted, so value is an "
+                         f"upper bound on the CPU rate.  Sieve leg alone: "
+                         f"{certified / t_n:.3g} s/s on {cores} threads, "
+                         f"{certified / t_1:.3g} s/s on one core")
+        out["c3_cpu_lower_bound_s"] = round(t_n + sq, 1)
+    return out
 
 
 M64 = (1 << 64) - 1
@@ -369,6 +428,60 @@ def mode_scan(args):
                           "lines": [o.line() for o in out][:16]}), flush=True)
 
 
+def mode_gcd(args):
+    """Full-size interval gcds at r = 136279841: gcd(A, x^r + x^474 + 1) for
+    random residues A (SplitMix64 words), one alone and then --gcd-conc at
+    once on host threads (the deep scan's concurrency), with the half-gcd's
+    device products.  Prints one JSON line with the wall times."""
+    import threading
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import lib
+    device.init(0)
+    P64 = C.POINTER(C.c_uint64)
+    nw = (R + 63) // 64
+    t = np.zeros((R + 64) // 64, dtype=np.uint64)
+    for e in (0, 474, R):
+        t[e // 64] |= np.uint64(1 << (e % 64))
+
+    def residue(seed):
+        w = np.empty(nw, dtype=np.uint64)
+        rng = np.random.default_rng(seed)     # PCG64: not GF(2)-linear
+        w[:] = rng.integers(0, 2**64, nw, dtype=np.uint64)
+        w[-1] &= np.uint64((1 << (R % 64)) - 1)
+        return w
+
+    def one(a, out, i):
+        g = np.zeros(4, dtype=np.uint64)
+        t0 = time.perf_counter()
+        out[i] = (int(lib.gf2t_gcd_host(t.ctypes.data_as(P64), t.size, a.ctypes.data_as(P64),
+                                        a.size, g.ctypes.data_as(P64), 4)),
+                  time.perf_counter() - t0)
+
+    res = {}
+    a0 = residue(args.seed)
+    out = [None]
+    one(a0, out, 0)                               # warm plans and buffers
+    one(a0, out, 0)
+    res["one_s"] = out[0][1]
+    res["one_degree"] = out[0][0]
+    print(f"gcd: one alone {out[0][1]:.2f} s (degree {out[0][0]})", file=sys.stderr, flush=True)
+    n = args.gcd_conc
+    if n > 1:
+        As = [residue(args.seed + 1 + i) for i in range(n)]
+        outs = [None] * n
+        ths = [threading.Thread(target=one, args=(As[i], outs, i)) for i in range(n)]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        wall = time.perf_counter() - t0
+        res.update(concurrent=n, concurrent_wall_s=wall, concurrent_each_s=[o[1] for o in outs],
+                   concurrent_per_gcd_s=wall / n)
+        print(f"gcd: {n} at once {wall:.2f} s", file=sys.stderr, flush=True)
+    print(json.dumps({"mode": "gcd", "r": R, "s": 474, **res}), flush=True)
+
+
 def mode_sieve(args):
     """The coarse sieve alone: --warmup + --steps runs over every s <= (r-1)/2
     to --depth, each reported with its wall time and the device time of
@@ -568,6 +681,8 @@ def main():
         return mode_serve(args)
     if args.mode == "sieve":
         return mode_sieve(args)
+    if args.mode == "gcd":
+        return mode_gcd(args)
     if args.mode == "c3":
         from mersenne_trinomial_amd import device
         device.init(0)
@@ -713,7 +828,7 @@ def main():
         "device": device.device_name(),
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_depth, int(svals[0]))
+        line["cpu_baseline"] = cpu_baseline(certified, int(svals[0]), c3)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

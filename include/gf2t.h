@@ -61,6 +61,26 @@ int gf2t_event_create(void **event);
 int gf2t_event_destroy(void *event);
 int gf2t_event_record(void *event, void *stream);
 int gf2t_event_elapsed_ms(void *start, void *stop, float *ms);
+/* HIP graphs of call sequences.  gf2t_graph_begin starts capturing what
+ * the calling thread issues on `stream` (a stream from gf2t_stream_create,
+ * not NULL); gf2t_graph_end stops and instantiates it.  Each
+ * gf2t_graph_launch replays the same launches, with the same pointers and
+ * sizes, on any stream.  Only kernel-only calls may be captured: the
+ * residue-ring, transform, product and slab entry points (gf2t_square_*,
+ * gf2t_fold, gf2t_mulx_xor*, gf2t_set_*, gf2t_xor_*, gf2t_copy_slab,
+ * gf2t_horner_slab, gf2t_fft_fwd / _inv, gf2t_mul, gf2t_mulmod,
+ * gf2t_mulmod_slab); not the sieve, the host-buffer
+ * services, copies to or from the host, gf2t_memset or
+ * gf2t_ddf_init_slab.  The reference launches every kernel from the host
+ * (src/tsfactor.cu:278-336); the deep scan replays one graph per block of
+ * m degrees.  gf2t_graph_nodes reports the captured node count. */
+typedef struct gf2t_graph gf2t_graph;
+int gf2t_graph_begin(void *stream);
+int gf2t_graph_end(void *stream, gf2t_graph **graph);
+int gf2t_graph_launch(gf2t_graph *graph, void *stream);
+int gf2t_graph_nodes(const gf2t_graph *graph, size_t *nodes);
+void gf2t_graph_destroy(gf2t_graph *graph);
+
 /* 16-byte vector copy kernel (dst, src 16-byte aligned, bytes % 16 == 0):
  * the stream-copy reference the HBM roofline figures are calibrated
  * against; no reference counterpart. */
@@ -113,6 +133,15 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  * GF2T_NO_VPCLMUL=1 keeps the host gcds' schoolbook products on 128-bit
  * PCLMULQDQ where the CPU also has AVX-512 VPCLMULQDQ (read once). */
 int gf2t_tune(const char *key, int64_t value);
+/* The value a knob has now (the default until set): the keys above, and
+ * "fft_tile_<m>" -- the tile of the 2^m-point transforms' passes is 2^T
+ * elements, T in 6..13 (set with gf2t_tune("fft_tile_<m>", T), 0 = the
+ * default: 13 from m = 19 up, 11 for m = 14..18, m below 14).  A smaller
+ * tile spreads a small transform over more workgroups at the cost of more
+ * passes; the reference's split knobs are gf2c_taylor_lv / gf2c_bfly_lv
+ * (src/gf2_cantor_cuda.h:24-61).  The autotuner (mersenne_trinomial_amd/
+ * autotune.py) measures these and installs its choices. */
+int gf2t_tune_get(const char *key, int64_t *value);
 
 /* Slab form: residues i = 0..n-1 at din + i*pitch (pitch >= nw words),
  * each squared once mod x^r + x^s_i + 1 into dout + i*pitch; ds is a
@@ -255,6 +284,17 @@ int gf2t_mul_host(const uint64_t *ha, uint64_t abits, const uint64_t *hb,
 int gf2t_mat2_host(const uint64_t *const hm[4], const uint64_t mbits[4],
                    const uint64_t *ha, uint64_t abits, const uint64_t *hb,
                    uint64_t bbits, uint64_t *hoa, uint64_t *hob, size_t outw);
+/* The same matrix applied to npairs (1..4) vector pairs with one set of
+ * four matrix transforms: pair i is (hv[2i], hv[2i+1]) of vbits[2i],
+ * vbits[2i+1] bits, and its results go to hout[2i] = m0 a + m1 b and
+ * hout[2i+1] = m2 a + m3 b, each outw >= gf2t_mul_out_words(max mbits,
+ * max vbits) words, fully written.  A half-gcd node applies its second
+ * matrix to the remainder pair and to both columns of the first matrix
+ * this way.  No reference counterpart (it batches mat2_apply,
+ * src/tsfactor.cu:535-607). */
+int gf2t_mat2n_host(const uint64_t *const hm[4], const uint64_t mbits[4], int npairs,
+                    const uint64_t *const *hv, const uint64_t *vbits, uint64_t *const *hout,
+                    size_t outw);
 
 /* ------------------------------------------------------------------ *
  * (a) Zech-log coarse sieve

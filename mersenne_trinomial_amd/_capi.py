@@ -67,7 +67,13 @@ _SIGS = {
     "gf2t_event_record": (C.c_int, [_vp, _vp]),
     "gf2t_event_elapsed_ms": (C.c_int, [_vp, _vp, C.POINTER(C.c_float)]),
     "gf2t_stream_copy": (C.c_int, [_vp, _vp, _sz, _vp]),
+    "gf2t_graph_begin": (C.c_int, [_vp]),
+    "gf2t_graph_end": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "gf2t_graph_launch": (C.c_int, [_vp, _vp]),
+    "gf2t_graph_nodes": (C.c_int, [_vp, C.POINTER(_sz)]),
+    "gf2t_graph_destroy": (None, [_vp]),
     "gf2t_tune": (C.c_int, [C.c_char_p, C.c_int64]),
+    "gf2t_tune_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "gf2t_square_fold": (C.c_int, [_vp, _vp, _u64, _u64, _vp]),
     "gf2t_square_fold_steps": (C.c_int, [_vp, _vp, _u64, _u64, _u64, _vp]),
     "gf2t_square_fold_batch": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), _p64, _p64, _p64,
@@ -118,6 +124,8 @@ _SIGS = {
     "gf2t_set_one_slab": (C.c_int, [_vp, _sz, _sz, C.c_int, _vp]),
     "gf2t_copy_slab": (C.c_int, [_vp, _sz, _vp, _sz, _sz, C.c_int, _vp]),
     "gf2t_mat2_host": (C.c_int, [C.POINTER(_p64), _p64, _p64, _u64, _p64, _u64, _p64, _p64, _sz]),
+    "gf2t_mat2n_host": (C.c_int, [C.POINTER(_p64), _p64, C.c_int, C.POINTER(_p64), _p64,
+                                  C.POINTER(_p64), _sz]),
     "gf2t_edf_least": (C.c_int, [_p64, _sz, _u64, _p64, _sz]),
     "gf2t_swan_bound": (_u64, [_u64, _u64, C.POINTER(C.c_int)]),
 }

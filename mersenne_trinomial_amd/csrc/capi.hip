@@ -7,6 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <dlfcn.h>
+#include <mutex>
+#include <string>
 
 namespace gf2t {
 static thread_local char g_err[512] = "";
@@ -34,6 +37,48 @@ extern "C" int gf2t_device_count(int *count) {
     return GF2T_OK;
 }
 
+namespace {
+// The autotuner's choices: "key value" lines (gf2t_tune keys, '#' starts a
+// comment) from $GF2T_TUNE_FILE, or else gf2t_tune.txt beside this
+// library; GF2T_TUNE_FILE set to the empty string skips them.  A knob
+// whose environment variable is set keeps the environment's value.
+// Applied once per process, by the first gf2t_init.
+void load_tune_file() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::string path;
+        if (const char *e = getenv("GF2T_TUNE_FILE")) {
+            if (!*e) return;
+            path = e;
+        } else {
+            Dl_info info;
+            if (!dladdr((const void *)&gf2t_version, &info) || !info.dli_fname) return;
+            std::string lib = info.dli_fname;
+            size_t sl = lib.rfind('/');
+            path = (sl == std::string::npos ? std::string(".") : lib.substr(0, sl)) +
+                   "/gf2t_tune.txt";
+        }
+        FILE *f = fopen(path.c_str(), "r");
+        if (!f) return;
+        static const char *env_of[][2] = {{"sq_fuse", "GF2T_SQ_FUSE"}, {"sq_ppt", "GF2T_SQ_PPT"},
+                                          {"sq_order", "GF2T_SQ_ORDER"}, {"sq_nt", "GF2T_SQ_NT"},
+                                          {"gcd_dev_bits", "GF2T_GCD_DEV_BITS"}};
+        char line[256], key[96];
+        long long v;
+        while (fgets(line, sizeof line, f)) {
+            if (line[0] == '#' || sscanf(line, "%95s %lld", key, &v) != 2) continue;
+            bool env = false;
+            for (auto &kv : env_of)
+                if (!strcmp(key, kv[0]) && getenv(kv[1])) env = true;
+            if (!env && gf2t_tune(key, (int64_t)v) != GF2T_OK)
+                fprintf(stderr, "gf2t: %s: ignoring '%s %lld' (%s)\n", path.c_str(), key, v,
+                        gf2t_last_error());
+        }
+        fclose(f);
+    });
+}
+}  // namespace
+
 extern "C" int gf2t_init(int device) {
     int n = 0;
     GF2T_HIP(hipGetDeviceCount(&n));
@@ -43,6 +88,7 @@ extern "C" int gf2t_init(int device) {
     }
     GF2T_HIP(hipSetDevice(device));
     GF2T_HIP(hipFree(nullptr));   // create the context now
+    load_tune_file();
     // with a device present, large gcd products go to the device
     if (!gf2t_host::device_product_bits()) {
         const char *e = getenv("GF2T_GCD_DEV_BITS");
@@ -155,6 +201,56 @@ extern "C" int gf2t_stream_create(void **stream) {
 extern "C" int gf2t_stream_destroy(void *stream) {
     if (stream) GF2T_HIP(hipStreamDestroy((hipStream_t)stream));
     return GF2T_OK;
+}
+
+// ---- HIP graphs of captured call sequences
+struct gf2t_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int gf2t_graph_begin(void *stream) {
+    if (!stream) {
+        gf2t::set_error("gf2t_graph_begin: capture needs a stream from gf2t_stream_create");
+        return GF2T_EINVAL;
+    }
+    GF2T_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_graph_end(void *stream, gf2t_graph **out) {
+    if (!stream || !out) return GF2T_EINVAL;
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    GF2T_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (rc != hipSuccess) {
+        hipGraphDestroy(g);
+        return gf2t::hip_fail(rc, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    *out = new gf2t_graph{g, e};
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_graph_launch(gf2t_graph *g, void *stream) {
+    if (!g || !g->exec) return GF2T_EINVAL;
+    GF2T_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_graph_nodes(const gf2t_graph *g, size_t *nodes) {
+    if (!g || !nodes) return GF2T_EINVAL;
+    *nodes = 0;
+    GF2T_HIP(hipGraphGetNodes(g->graph, nullptr, nodes));
+    return GF2T_OK;
+}
+
+extern "C" void gf2t_graph_destroy(gf2t_graph *g) {
+    if (!g) return;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
 }
 
 extern "C" int gf2t_stream_sync(void *stream) {

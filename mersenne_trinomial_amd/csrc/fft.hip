@@ -30,13 +30,16 @@
 #include <condition_variable>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 
 namespace {
 
 constexpr int FFT_MAX_OPS = 64;
-constexpr int FFT_THREADS = 512;
-constexpr int FFT_TILE_BITS = 13;   // 8192 elements = 64 KiB of LDS
+constexpr int FFT_THREADS = 512;    // most threads per block (a tile of 2^T elements uses 2^(T-1))
+constexpr int FFT_TILE_BITS = 13;   // largest tile: 8192 elements = 64 KiB of LDS
+constexpr int FFT_MIN_TILE_BITS = 6;
 constexpr int FFT_LOW_BITS = 4;     // 16 contiguous elements = 128 bytes
+constexpr int PACK_LIST = 12;   // four matrix entries + four vector pairs
 
 enum : uint8_t { OP_TF = 0, OP_TI = 1, OP_BF = 2, OP_BI = 3 };
 
@@ -59,9 +62,35 @@ struct TwTables {
     u64 t[4][256];
 };
 
+// Where the first pass of a batch of transforms packs its input from:
+// 32-bit chunk k of an operand is element k (zero from nchunks on, the
+// last chunk masked to lastmask).  Stride form: batch element y packs
+// from src + y*stride.  List form (nlist > 0): element y < nlist packs
+// from lsrc[y] with its own chunk count and mask, so operands of
+// different lengths share one launch.  xor0 is XORed into element 0
+// (adds x to the operand).
+struct PackArg {
+    const u32 *src;
+    u64 stride;
+    u64 nchunks;
+    u32 lastmask, xor0;
+    int nlist;
+    const u32 *lsrc[PACK_LIST];
+    u64 lnch[PACK_LIST];
+    u32 lmask[PACK_LIST];
+};
+
+// W(t) for t < 2^(8 ntab): only the tables that exist are read
 GF2T_DEV u64 twiddle(const TwTables &tw, u64 t) {
-    return tw.t[0][t & 255] ^ tw.t[1][(t >> 8) & 255] ^ tw.t[2][(t >> 16) & 255] ^
-           tw.t[3][(t >> 24) & 255];
+    u64 w = tw.t[0][t & 255];
+    if (t >> 8) {
+        w ^= tw.t[1][(t >> 8) & 255];
+        if (t >> 16) {
+            w ^= tw.t[2][(t >> 16) & 255];
+            if (t >> 24) w ^= tw.t[3][(t >> 24) & 255];
+        }
+    }
+    return w;
 }
 
 GF2T_DEV u64 global_index(const PassDesc &pd, u64 li, u64 rest_lo, u64 rest_hi) {
@@ -80,25 +109,42 @@ GF2T_DEV u64 packed_chunk(const u32 *__restrict__ src, u64 nchunks, u32 lastmask
 }
 
 // blockIdx.y selects one transform of a batch: element y lives at
-// f + y*f_stride and packs from pack_src + y*pack_stride (u32 units);
-// pack_xor0 is XORed into element 0 (adds x to the packed operand).
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
-        u64 *__restrict__ f, const PassDesc pd, const u32 *__restrict__ pack_src,
-        u64 pack_nchunks, u32 pack_lastmask, u64 f_stride, u64 pack_stride, u32 pack_xor0) {
+// f + y*f_stride.  pack (first pass only) describes the input to pack.
+// The block has 2^(T-1) threads (at most FFT_THREADS) for its 2^T-element
+// tile; ntab byte tables of twiddles cover the twiddle indices t < 2^(8
+// ntab) this transform size uses.
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(u64 *__restrict__ f, const PassDesc pd,
+                                                          const PackArg pk, int pack, int ntab,
+                                                          u64 f_stride) {
     extern __shared__ u64 lds[];
-    f += (u64)blockIdx.y * f_stride;
-    if (pack_src) pack_src += (u64)blockIdx.y * pack_stride;
+    const unsigned y = blockIdx.y;
+    f += (u64)y * f_stride;
+    const u32 *psrc = nullptr;
+    u64 pn = 0;
+    u32 pmask = 0;
+    if (pack) {
+        if (pk.nlist) {
+            psrc = pk.lsrc[y];
+            pn = pk.lnch[y];
+            pmask = pk.lmask[y];
+        } else {
+            psrc = pk.src + (u64)y * pk.stride;
+            pn = pk.nchunks;
+            pmask = pk.lastmask;
+        }
+    }
     TwTables &tw = *reinterpret_cast<TwTables *>(lds);
     u64 *tile = lds + 4 * 256;
     const int T = pd.c + pd.w;
     const u64 tsize = 1ULL << T;
+    const unsigned nthr = blockDim.x;
     const u64 blk = blockIdx.x;
     const int nlo = pd.a - pd.c;
     const u64 rest_lo = blk & ((1ULL << nlo) - 1);
     const u64 rest_hi = blk >> nlo;
 
     // twiddle byte tables: T_k[v] = sum_{bits j of v} b[8k + j + 1]
-    for (int e = threadIdx.x; e < 1024; e += FFT_THREADS) {
+    for (int e = threadIdx.x; e < 256 * ntab; e += nthr) {
         int k = e >> 8, v = e & 255;
         u64 acc = 0;
         for (int j = 0; j < 8; j++)
@@ -109,10 +155,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
         tw.t[k][v] = acc;
     }
     // load the tile (low c bits contiguous in memory)
-    for (u64 li = threadIdx.x; li < tsize; li += FFT_THREADS) {
+    for (u64 li = threadIdx.x; li < tsize; li += nthr) {
         u64 g = global_index(pd, li, rest_lo, rest_hi);
-        u64 v = pack_src ? packed_chunk(pack_src, pack_nchunks, pack_lastmask, g) : f[g];
-        if (pack_src && g == 0) v ^= pack_xor0;
+        u64 v;
+        if (pack) {
+            v = packed_chunk(psrc, pn, pmask, g);
+            if (g == 0) v ^= pk.xor0;
+        } else {
+            v = f[g];
+        }
         tile[li] = v;
     }
     __syncthreads();
@@ -123,7 +174,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
         const u64 lo_mask = (1ULL << l) - 1;
         if (kind == OP_TF || kind == OP_TI) {
             const u64 st = 1ULL << l;
-            for (u64 qd = threadIdx.x; qd < (tsize >> 2); qd += FFT_THREADS) {
+            for (u64 qd = threadIdx.x; qd < (tsize >> 2); qd += nthr) {
                 u64 base = (qd & lo_mask) | ((qd >> l) << (l + 2));
                 u64 x01 = tile[base + st], x10 = tile[base + 2 * st], x11 = tile[base + 3 * st];
                 if (kind == OP_TF) { x10 ^= x11; x01 ^= x10; }
@@ -134,7 +185,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
         } else {
             const int L = pd.gbit[o];
             const u64 st = 1ULL << l;
-            for (u64 pr = threadIdx.x; pr < (tsize >> 1); pr += FFT_THREADS) {
+            for (u64 pr = threadIdx.x; pr < (tsize >> 1); pr += nthr) {
                 u64 i1 = (pr & lo_mask) | ((pr >> l) << (l + 1));
                 u64 g1 = global_index(pd, i1, rest_lo, rest_hi);
                 u64 w = twiddle(tw, g1 >> (L + 1));
@@ -152,7 +203,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
         }
         __syncthreads();
     }
-    for (u64 li = threadIdx.x; li < tsize; li += FFT_THREADS)
+    for (u64 li = threadIdx.x; li < tsize; li += nthr)
         f[global_index(pd, li, rest_lo, rest_hi)] = tile[li];
 }
 
@@ -212,10 +263,12 @@ std::vector<Op> op_sequence(int m, bool forward) {
     return ops;
 }
 
-std::vector<PassDesc> schedule(int m, bool forward) {
+// Pack the op sequence of a 2^m-point transform into passes over tiles of
+// 2^min(m, tile_bits) elements.
+std::vector<PassDesc> schedule(int m, bool forward, int tile_bits) {
     std::vector<Op> rem = op_sequence(m, forward);
     std::vector<PassDesc> passes;
-    int T = std::min(m, FFT_TILE_BITS);
+    int T = std::min(m, tile_bits);
     int c = std::min(FFT_LOW_BITS, T);
     int w = T - c;
     while (!rem.empty()) {
@@ -336,26 +389,49 @@ int ensure_basis() {
 }  // namespace
 
 struct gf2t_fft_plan {
-    int m;
+    int m, T;   // 2^m points, tiles of 2^T elements
     std::vector<PassDesc> fwd, inv;
 };
 
 namespace {
+constexpr int FFT_MAX_M = 27;
 std::mutex g_plan_mu;
-std::map<int, gf2t_fft_plan *> g_plans;
+std::map<int, gf2t_fft_plan *> g_plans;   // key m * 32 + T
 
-const gf2t_fft_plan *cached_plan(int m) {
-    std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plans.find(m);
-    if (it != g_plans.end()) return it->second;
-    gf2t_fft_plan *p = new gf2t_fft_plan{m, schedule(m, true), schedule(m, false)};
-    g_plans[m] = p;
-    return p;
+// Tile bits per transform size: whole 8192-element tiles from 2^19 points
+// up; below that smaller tiles, so that a transform still spreads over
+// tens of workgroups (a 2^15-point transform in 8192-element tiles
+// occupies four CUs, each running 15 dependent butterfly levels).
+// Overridden per size by gf2t_tune("fft_tile_<m>", T), which is how the
+// autotuner installs what it measured.
+int default_tile_bits(int m) {
+    if (m >= 19) return FFT_TILE_BITS;
+    if (m >= 14) return 11;
+    return std::min(m, FFT_TILE_BITS);
+}
+std::atomic<int> g_tile_bits[FFT_MAX_M + 1];   // 0: default_tile_bits
+
+int tile_bits_for(int m) {
+    int T = g_tile_bits[m].load(std::memory_order_relaxed);
+    return T ? T : default_tile_bits(m);
 }
 
-int run_passes(const std::vector<PassDesc> &ps, u64 *df, hipStream_t st, const u32 *pack_src,
-               u64 nchunks, u32 lastmask, unsigned nbatch = 1, u64 f_stride = 0,
-               u64 pack_stride = 0, u32 pack_xor0 = 0) {
+const gf2t_fft_plan *cached_plan_T(int m, int T) {
+    T = std::max(FFT_MIN_TILE_BITS, std::min(T, FFT_TILE_BITS));
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(m * 32 + T);
+    if (it != g_plans.end()) return it->second;
+    gf2t_fft_plan *p = new gf2t_fft_plan{m, T, schedule(m, true, T), schedule(m, false, T)};
+    g_plans[m * 32 + T] = p;
+    return p;
+}
+const gf2t_fft_plan *cached_plan(int m) { return cached_plan_T(m, tile_bits_for(m)); }
+
+// Run the passes of one transform (or of nbatch transforms f_stride words
+// apart, blockIdx.y selecting one); pk (may be null) packs the first
+// pass's input.
+int run_passes(const std::vector<PassDesc> &ps, u64 *df, hipStream_t st, const PackArg *pk,
+               unsigned nbatch = 1, u64 f_stride = 0) {
     static std::once_flag lds_once;
     static hipError_t lds_err = hipSuccess;
     std::call_once(lds_once, [] {
@@ -364,17 +440,34 @@ int run_passes(const std::vector<PassDesc> &ps, u64 *df, hipStream_t st, const u
                                       (int)((4 * 256 + ((size_t)1 << FFT_TILE_BITS)) * sizeof(u64)));
     });
     GF2T_HIP(lds_err);
+    if (nbatch == 0) return GF2T_OK;
+    if (pk && pk->nlist && nbatch > (unsigned)pk->nlist) {
+        gf2t::set_error("fft: %u transforms but %d pack sources", nbatch, pk->nlist);
+        return GF2T_EINTERNAL;
+    }
+    PackArg none{};
     for (size_t i = 0; i < ps.size(); i++) {
         const PassDesc &pd = ps[i];
-        int T = pd.c + pd.w;
-        unsigned blocks = (unsigned)(1ULL << (pd.m - T));
-        size_t lds = (4 * 256 + ((size_t)1 << T)) * sizeof(u64);
-        const u32 *src = (i == 0) ? pack_src : nullptr;
-        hipLaunchKernelGGL(k_fft_pass, dim3(blocks, nbatch), dim3(FFT_THREADS), lds, st, df, pd,
-                           src, nchunks, lastmask, f_stride, pack_stride, pack_xor0);
+        const int T = pd.c + pd.w;
+        const int ntab = std::max(1, std::min(4, (pd.m - 1 + 7) / 8));
+        const unsigned blocks = (unsigned)(1ULL << (pd.m - T));
+        const unsigned threads = (unsigned)std::max<u64>(64, std::min<u64>(FFT_THREADS, 1ULL << (T > 0 ? T - 1 : 0)));
+        const size_t lds = (4 * 256 + ((size_t)1 << T)) * sizeof(u64);
+        const bool pack = i == 0 && pk;
+        hipLaunchKernelGGL(k_fft_pass, dim3(blocks, nbatch), dim3(threads), lds, st, df, pd,
+                           pack ? *pk : none, pack ? 1 : 0, ntab, f_stride);
     }
     GF2T_CHECK_LAUNCH("k_fft_pass");
     return GF2T_OK;
+}
+
+PackArg pack_one(const u64 *src, u64 bits, u32 xor0 = 0) {
+    PackArg p{};
+    p.src = (const u32 *)src;
+    p.nchunks = (bits + 31) >> 5;
+    p.lastmask = (bits & 31) ? (u32)((1ULL << (bits & 31)) - 1) : 0xFFFFFFFFu;
+    p.xor0 = xor0;
+    return p;
 }
 
 int fft_m_for(u64 nchunks_total) {
@@ -383,8 +476,21 @@ int fft_m_for(u64 nchunks_total) {
     return m;
 }
 
-constexpr int FFT_MAX_M = 27;
 }  // namespace
+
+// gf2t_tune("fft_tile_<m>", T): tiles of 2^T elements (6..13) for 2^m-point
+// transforms from now on; 0 restores the default.  Plans already made
+// keep theirs.
+int gf2t_fft_set_tile_bits(int m, int64_t T) {
+    if (m < 1 || m > FFT_MAX_M || (T != 0 && (T < FFT_MIN_TILE_BITS || T > FFT_TILE_BITS))) {
+        gf2t::set_error("gf2t_tune: fft_tile_<m> needs 1 <= m <= %d and T = 0 or %d..%d",
+                        FFT_MAX_M, FFT_MIN_TILE_BITS, FFT_TILE_BITS);
+        return GF2T_EINVAL;
+    }
+    g_tile_bits[m].store((int)T);
+    return GF2T_OK;
+}
+int gf2t_fft_tile_bits(int m) { return (m >= 1 && m <= FFT_MAX_M) ? tile_bits_for(m) : 0; }
 
 extern "C" int gf2t_cantor_basis(uint64_t *hbasis, int *len) {
     build_basis_host();
@@ -403,7 +509,8 @@ extern "C" int gf2t_fft_plan_create(int m, gf2t_fft_plan **plan) {
         gf2t::set_error("gf2t_fft_plan_create: basis length %d < m", g_basis.len);
         return GF2T_EINTERNAL;
     }
-    *plan = new gf2t_fft_plan{m, schedule(m, true), schedule(m, false)};
+    const int T = std::max(FFT_MIN_TILE_BITS, std::min(tile_bits_for(m), FFT_TILE_BITS));
+    *plan = new gf2t_fft_plan{m, T, schedule(m, true, T), schedule(m, false, T)};
     return GF2T_OK;
 }
 
@@ -423,14 +530,14 @@ extern "C" int gf2t_fft_fwd(const gf2t_fft_plan *plan, uint64_t *df, void *strea
     if (!plan || !df) return GF2T_EINVAL;
     int rc = ensure_basis();
     if (rc) return rc;
-    return run_passes(plan->fwd, df, (hipStream_t)stream, nullptr, 0, 0);
+    return run_passes(plan->fwd, df, (hipStream_t)stream, nullptr);
 }
 
 extern "C" int gf2t_fft_inv(const gf2t_fft_plan *plan, uint64_t *df, void *stream) {
     if (!plan || !df) return GF2T_EINVAL;
     int rc = ensure_basis();
     if (rc) return rc;
-    return run_passes(plan->inv, df, (hipStream_t)stream, nullptr, 0, 0);
+    return run_passes(plan->inv, df, (hipStream_t)stream, nullptr);
 }
 
 extern "C" size_t gf2t_mul_out_words(uint64_t abits, uint64_t bbits) {
@@ -446,14 +553,24 @@ extern "C" size_t gf2t_mul_workspace_bytes(uint64_t abits, uint64_t bbits) {
 }
 
 namespace {
+// p[0, n) = 0 with a kernel (no memset node when the caller captures a graph)
+__global__ void k_zero_words(u64 *__restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+int zero_words(u64 *p, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)),
+                       dim3(256), 0, st, p, n);
+    GF2T_CHECK_LAUNCH("k_zero_words");
+    return GF2T_OK;
+}
+
 int mul_core(const u64 *da, u64 abits, const u64 *db, u64 bbits, u64 *dprod, void *dwork,
              hipStream_t st) {
     u64 ca = (abits + 31) >> 5, cb = (bbits + 31) >> 5;
     size_t ow = gf2t_mul_out_words(abits, bbits);
-    if (!ca || !cb) {
-        GF2T_HIP(hipMemsetAsync(dprod, 0, ow * 8, st));
-        return GF2T_OK;
-    }
+    if (!ca || !cb) return zero_words(dprod, ow, st);
     int m = fft_m_for(ca + cb - 1);
     if (m > FFT_MAX_M) {
         gf2t::set_error("gf2t_mul: product needs 2^%d points (max 2^%d)", m, FFT_MAX_M);
@@ -464,15 +581,17 @@ int mul_core(const u64 *da, u64 abits, const u64 *db, u64 bbits, u64 *dprod, voi
     const gf2t_fft_plan *plan = cached_plan(m);
     size_t n = (size_t)1 << m;
     u64 *FA = (u64 *)dwork, *FB = FA + n;
-    u32 ma = (abits & 31) ? (u32)((1ULL << (abits & 31)) - 1) : 0xFFFFFFFFu;
-    u32 mb = (bbits & 31) ? (u32)((1ULL << (bbits & 31)) - 1) : 0xFFFFFFFFu;
-    rc = run_passes(plan->fwd, FA, st, (const u32 *)da, ca, ma);
-    if (rc) return rc;
-    rc = run_passes(plan->fwd, FB, st, (const u32 *)db, cb, mb);
+    // both operands in one launch chain: FA and FB are n words apart
+    const PackArg pa = pack_one(da, abits), pb = pack_one(db, bbits);
+    PackArg two{};
+    two.nlist = 2;
+    two.lsrc[0] = pa.src; two.lnch[0] = pa.nchunks; two.lmask[0] = pa.lastmask;
+    two.lsrc[1] = pb.src; two.lnch[1] = pb.nchunks; two.lmask[1] = pb.lastmask;
+    rc = run_passes(plan->fwd, FA, st, &two, 2, n);
     if (rc) return rc;
     unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_pointwise, dim3(blocks), dim3(256), 0, st, FA, FB, n);
-    rc = run_passes(plan->inv, FA, st, nullptr, 0, 0);
+    rc = run_passes(plan->inv, FA, st, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0, st, FA,
                        (size_t)(ca + cb - 1), dprod, ow, (u64)0, (u64)0);
@@ -505,14 +624,18 @@ extern "C" int gf2t_mulmod(const uint64_t *da, uint64_t abits, const uint64_t *d
                         (unsigned long long)abits, (unsigned long long)bbits);
         return GF2T_EINVAL;
     }
+    if (!(r >= 3 && s >= 1 && 2 * s < r)) {
+        gf2t::set_error("gf2t_mulmod: need 1 <= s, 2s < r");
+        return GF2T_EINVAL;
+    }
     u64 ca = (abits + 31) >> 5, cb = (bbits + 31) >> 5;
-    size_t n = (ca && cb) ? ((size_t)1 << fft_m_for(ca + cb - 1)) : 0;
+    if (!ca || !cb)   // a zero operand: the workspace may be empty, dout is 0
+        return zero_words(dout, words_of(r), (hipStream_t)stream);
+    size_t n = (size_t)1 << fft_m_for(ca + cb - 1);
     u64 *prod = (u64 *)dwork + 2 * n;
     int rc = mul_core(da, abits, db, bbits, prod, dwork, (hipStream_t)stream);
     if (rc) return rc;
-    u64 pbits = abits + bbits ? abits + bbits - 1 : 0;
-    if (pbits < 1) pbits = 1;
-    return gf2t_fold(prod, pbits, dout, r, s, stream);
+    return gf2t_fold(prod, abits + bbits - 1, dout, r, s, stream);
 }
 
 // ------------------------------------------------------------------ slabs
@@ -560,15 +683,23 @@ extern "C" int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_
     u64 *FA = (u64 *)dwork, *FB = FA + N * n, *prod = FB + N * n;
     size_t ppitch = 2 * nw + 2;
     u32 lm = (r & 31) ? (u32)((1ULL << (r & 31)) - 1) : 0xFFFFFFFFu;
-    rc = run_passes(plan->fwd, FA, st, (const u32 *)da, ca, lm, (unsigned)n, N, 2 * apitch, 0);
+    PackArg pa{};
+    pa.src = (const u32 *)da;
+    pa.stride = 2 * apitch;
+    pa.nchunks = ca;
+    pa.lastmask = lm;
+    rc = run_passes(plan->fwd, FA, st, &pa, (unsigned)n, N);
     if (rc) return rc;
-    rc = run_passes(plan->fwd, FB, st, (const u32 *)db, ca, lm, (unsigned)n, N, 2 * bpitch,
-                    b_add_x ? 2u : 0u);
+    PackArg pb = pa;
+    pb.src = (const u32 *)db;
+    pb.stride = 2 * bpitch;
+    pb.xor0 = b_add_x ? 2u : 0u;
+    rc = run_passes(plan->fwd, FB, st, &pb, (unsigned)n, N);
     if (rc) return rc;
     size_t tot = N * (size_t)n;
     unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
     hipLaunchKernelGGL(k_pointwise, dim3(blocks), dim3(256), 0, st, FA, FB, tot);
-    rc = run_passes(plan->inv, FA, st, nullptr, 0, 0, (unsigned)n, N, 0, 0);
+    rc = run_passes(plan->inv, FA, st, nullptr, (unsigned)n, N);
     if (rc) return rc;
     size_t ow = 2 * nw;   // product of two r-bit operands: 2r-1 bits
     hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256), n), dim3(256), 0, st, FA,
@@ -585,26 +716,20 @@ extern "C" int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_
 // mutex-guarded context per process with a private stream and device
 // buffers grown on demand (never shrunk).
 namespace {
-__global__ void k_mat2_pointwise(u64 *__restrict__ f0, const u64 *__restrict__ f1,
-                                 u64 *__restrict__ f2, const u64 *__restrict__ f3,
-                                 const u64 *__restrict__ fa, const u64 *__restrict__ fb, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) {
-        u64 a = fa[i], b = fb[i];
-        f0[i] = gf64_mul(f0[i], a) ^ gf64_mul(f1[i], b);
-        f2[i] = gf64_mul(f2[i], a) ^ gf64_mul(f3[i], b);
-    }
-}
 
-// A pool of contexts, each with its own non-blocking stream and device
-// buffer, so products requested by different host threads (the gcds of
-// different s values) run concurrently on the device.
-constexpr int HM_CONTEXTS = 8;
+// A pool of contexts, each with its own non-blocking stream, device
+// buffer and pinned host staging buffer, so products requested by
+// different host threads (the gcds of different s values) run
+// concurrently on the device.  Operands are copied into the pinned
+// buffer and go to the device in one transfer; results come back the
+// same way.
+constexpr int HM_CONTEXTS = 16;
 struct HostMulCtx {
     hipStream_t st = nullptr;
     u64 *dbuf = nullptr;
     size_t dcap = 0;   // words
+    u64 *hbuf = nullptr;
+    size_t hcap = 0;   // words
     bool busy = false;
 };
 struct HostMulPool {
@@ -633,19 +758,33 @@ struct HmLease {
     }
 };
 
-int hm_reserve(HostMulCtx &c, size_t words) {
+int hm_reserve(HostMulCtx &c, size_t words, size_t host_words) {
     if (!c.st) GF2T_HIP(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
-    if (words <= c.dcap) return GF2T_OK;
-    if (c.dbuf) GF2T_HIP(hipFree(c.dbuf));
-    c.dbuf = nullptr;
-    c.dcap = 0;
-    size_t want = words + words / 4;
-    if (hipMalloc(&c.dbuf, want * 8) != hipSuccess) {
-        (void)hipGetLastError();
-        gf2t::set_error("gf2t host product service: cannot allocate %zu bytes", want * 8);
-        return GF2T_ENOMEM;
+    if (words > c.dcap) {
+        if (c.dbuf) GF2T_HIP(hipFree(c.dbuf));
+        c.dbuf = nullptr;
+        c.dcap = 0;
+        size_t want = words + words / 4;
+        if (hipMalloc(&c.dbuf, want * 8) != hipSuccess) {
+            (void)hipGetLastError();
+            gf2t::set_error("gf2t host product service: cannot allocate %zu bytes", want * 8);
+            return GF2T_ENOMEM;
+        }
+        c.dcap = want;
     }
-    c.dcap = want;
+    if (host_words > c.hcap) {
+        if (c.hbuf) GF2T_HIP(hipHostFree(c.hbuf));
+        c.hbuf = nullptr;
+        c.hcap = 0;
+        size_t want = host_words + host_words / 4;
+        if (hipHostMalloc((void **)&c.hbuf, want * 8, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            gf2t::set_error("gf2t host product service: cannot allocate %zu pinned bytes",
+                            want * 8);
+            return GF2T_ENOMEM;
+        }
+        c.hcap = want;
+    }
     return GF2T_OK;
 }
 
@@ -665,90 +804,135 @@ extern "C" int gf2t_mul_host(const uint64_t *ha, uint64_t abits, const uint64_t 
     }
     HmLease lease;
     HostMulCtx &hc = *lease.c;
-    size_t aw = words_of(abits), bw = words_of(bbits);
-    size_t ww = (gf2t_mul_workspace_bytes(abits, bbits) + 7) / 8;
-    int rc = hm_reserve(hc, aw + bw + ow + ww + 64);   // + room to align the workspace
+    const size_t aw = words_of(abits), bw = words_of(bbits);
+    const size_t ww = (gf2t_mul_workspace_bytes(abits, bbits) + 7) / 8;
+    int rc = hm_reserve(hc, aw + bw + ow + ww + 64, std::max(aw + bw, ow));
     if (rc) return rc;
     u64 *da = hc.dbuf, *db = da + aw, *dout = db + bw, *dw = dout + ow;
     dw = (u64 *)(((uintptr_t)dw + 255) & ~(uintptr_t)255);
     hipStream_t st = hc.st;
-    GF2T_HIP(hipMemcpyAsync(da, ha, aw * 8, hipMemcpyHostToDevice, st));
-    GF2T_HIP(hipMemcpyAsync(db, hb, bw * 8, hipMemcpyHostToDevice, st));
+    std::memcpy(hc.hbuf, ha, aw * 8);
+    std::memcpy(hc.hbuf + aw, hb, bw * 8);
+    GF2T_HIP(hipMemcpyAsync(da, hc.hbuf, (aw + bw) * 8, hipMemcpyHostToDevice, st));
     rc = mul_core(da, abits, db, bbits, dout, dw, st);
     if (rc) return rc;
-    GF2T_HIP(hipMemcpyAsync(hout, dout, ow * 8, hipMemcpyDeviceToHost, st));
+    GF2T_HIP(hipMemcpyAsync(hc.hbuf, dout, ow * 8, hipMemcpyDeviceToHost, st));
     GF2T_HIP(hipStreamSynchronize(st));
+    std::memcpy(hout, hc.hbuf, ow * 8);
+    return GF2T_OK;
+}
+
+namespace {
+// Pair p of a batched matrix-vector product: its two transforms (at F + (4
+// + 2p) n and F + (5 + 2p) n) become (m0 a + m1 b, m2 a + m3 b), the
+// matrix transforms at F .. F + 3n left as they are.
+__global__ void k_mat2n_pointwise(u64 *__restrict__ F, size_t n, int npairs) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const u64 m0 = F[i], m1 = F[n + i], m2 = F[2 * n + i], m3 = F[3 * n + i];
+        for (int p = 0; p < npairs; p++) {
+            u64 *fa = F + (4 + 2 * (size_t)p) * n, *fb = fa + n;
+            const u64 a = fa[i], b = fb[i];
+            fa[i] = gf64_mul(m0, a) ^ gf64_mul(m1, b);
+            fb[i] = gf64_mul(m2, a) ^ gf64_mul(m3, b);
+        }
+    }
+}
+}  // namespace
+
+extern "C" int gf2t_mat2n_host(const uint64_t *const hm[4], const uint64_t mbits[4], int npairs,
+                               const uint64_t *const *hv, const uint64_t *vbits,
+                               uint64_t *const *hout, size_t outw) {
+    if (!hm || !mbits || !hv || !vbits || !hout || npairs < 1 || npairs > (PACK_LIST - 4) / 2) {
+        gf2t::set_error("gf2t_mat2n_host: bad arguments (1 <= npairs <= %d)", (PACK_LIST - 4) / 2);
+        return GF2T_EINVAL;
+    }
+    const int nv = 2 * npairs;
+    u64 mb = 0, vb = 0;
+    for (int i = 0; i < 4; i++) {
+        if (!hm[i] && mbits[i]) { gf2t::set_error("gf2t_mat2n_host: null entry"); return GF2T_EINVAL; }
+        mb = std::max<u64>(mb, mbits[i]);
+    }
+    for (int i = 0; i < nv; i++) {
+        if ((!hv[i] && vbits[i]) || !hout[i]) {
+            gf2t::set_error("gf2t_mat2n_host: null operand");
+            return GF2T_EINVAL;
+        }
+        vb = std::max<u64>(vb, vbits[i]);
+    }
+    if (outw < gf2t_mul_out_words(mb, vb)) {
+        gf2t::set_error("gf2t_mat2n_host: outw %zu below %zu", outw, gf2t_mul_out_words(mb, vb));
+        return GF2T_EINVAL;
+    }
+    if (!mb || !vb) {
+        for (int i = 0; i < nv; i++) std::memset(hout[i], 0, outw * 8);
+        return GF2T_OK;
+    }
+    u64 cm = (mb + 31) >> 5, cv = (vb + 31) >> 5;
+    int m = fft_m_for(cm + cv - 1);
+    if (m > FFT_MAX_M) { gf2t::set_error("gf2t_mat2n_host: product too large"); return GF2T_ERANGE; }
+    int rc = ensure_basis();
+    if (rc) return rc;
+    HmLease lease;
+    HostMulCtx &hc = *lease.c;
+    const gf2t_fft_plan *plan = cached_plan(m);
+    const int nt = 4 + nv;                    // transforms: matrix, then the vectors
+    size_t n = (size_t)1 << m, ow = gf2t_mul_out_words(mb, vb);
+    const u64 *src[PACK_LIST];
+    u64 bits[PACK_LIST];
+    size_t in_w[PACK_LIST], stage = 0;
+    for (int i = 0; i < nt; i++) {
+        src[i] = i < 4 ? hm[i] : hv[i - 4];
+        bits[i] = i < 4 ? mbits[i] : vbits[i - 4];
+        in_w[i] = words_of(bits[i]);
+        stage += (in_w[i] + 3) & ~(size_t)3;
+    }
+    rc = hm_reserve(hc, nt * n + nv * ow + stage + 64, std::max(stage, nv * ow));
+    if (rc) return rc;
+    hipStream_t st = hc.st;
+    u64 *F = hc.dbuf;                         // nt transforms, n words apart
+    u64 *dout = F + nt * n, *dstage = dout + nv * ow;
+    // the operands, 32-byte aligned, in one pinned buffer and one copy
+    PackArg pk{};
+    pk.nlist = nt;
+    size_t off = 0;
+    for (int i = 0; i < nt; i++) {
+        if (in_w[i]) std::memcpy(hc.hbuf + off, src[i], in_w[i] * 8);
+        pk.lsrc[i] = (const u32 *)(dstage + off);
+        pk.lnch[i] = (bits[i] + 31) >> 5;     // 0 for a zero operand: its transform is all zero
+        pk.lmask[i] = last_mask(bits[i]);
+        off += (in_w[i] + 3) & ~(size_t)3;
+    }
+    if (off) GF2T_HIP(hipMemcpyAsync(dstage, hc.hbuf, off * 8, hipMemcpyHostToDevice, st));
+    rc = run_passes(plan->fwd, F, st, &pk, (unsigned)nt, n);
+    if (rc) return rc;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_mat2n_pointwise, dim3(blocks), dim3(256), 0, st, F, n, npairs);
+    rc = run_passes(plan->inv, F + 4 * n, st, nullptr, (unsigned)nv, n);
+    if (rc) return rc;
+    size_t nch = (size_t)(cm + cv - 1);
+    hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256), nv), dim3(256), 0, st,
+                       F + 4 * n, nch, dout, ow, (u64)n, (u64)ow);
+    GF2T_CHECK_LAUNCH("gf2t_mat2n_host");
+    GF2T_HIP(hipMemcpyAsync(hc.hbuf, dout, nv * ow * 8, hipMemcpyDeviceToHost, st));
+    GF2T_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nv; i++) {
+        std::memcpy(hout[i], hc.hbuf + (size_t)i * ow, ow * 8);
+        if (outw > ow) std::memset(hout[i] + ow, 0, (outw - ow) * 8);
+    }
     return GF2T_OK;
 }
 
 extern "C" int gf2t_mat2_host(const uint64_t *const hm[4], const uint64_t mbits[4],
                               const uint64_t *ha, uint64_t abits, const uint64_t *hb,
                               uint64_t bbits, uint64_t *hoa, uint64_t *hob, size_t outw) {
-    if (!hm || !mbits || !hoa || !hob) {
+    const uint64_t *hv[2] = {ha, hb};
+    const uint64_t vbits[2] = {abits, bbits};
+    uint64_t *hout[2] = {hoa, hob};
+    if (!hoa || !hob) {
         gf2t::set_error("gf2t_mat2_host: null pointer");
         return GF2T_EINVAL;
     }
-    u64 mb = 0;
-    for (int i = 0; i < 4; i++) {
-        if (!hm[i] && mbits[i]) { gf2t::set_error("gf2t_mat2_host: null entry"); return GF2T_EINVAL; }
-        mb = std::max<u64>(mb, mbits[i]);
-    }
-    u64 vb = std::max(abits, bbits);
-    if ((!ha && abits) || (!hb && bbits)) { gf2t::set_error("gf2t_mat2_host: null operand"); return GF2T_EINVAL; }
-    if (outw < gf2t_mul_out_words(mb, vb)) {
-        gf2t::set_error("gf2t_mat2_host: outw %zu below %zu", outw, gf2t_mul_out_words(mb, vb));
-        return GF2T_EINVAL;
-    }
-    std::memset(hoa, 0, outw * 8);
-    std::memset(hob, 0, outw * 8);
-    if (!mb || !vb) return GF2T_OK;
-    u64 cm = (mb + 31) >> 5, cv = (vb + 31) >> 5;
-    int m = fft_m_for(cm + cv - 1);
-    if (m > FFT_MAX_M) { gf2t::set_error("gf2t_mat2_host: product too large"); return GF2T_ERANGE; }
-    int rc = ensure_basis();
-    if (rc) return rc;
-    HmLease lease;
-    HostMulCtx &hc = *lease.c;
-    const gf2t_fft_plan *plan = cached_plan(m);
-    size_t n = (size_t)1 << m, ow = gf2t_mul_out_words(mb, vb);
-    size_t in_w[6] = {words_of(mbits[0]), words_of(mbits[1]), words_of(mbits[2]),
-                      words_of(mbits[3]), words_of(abits), words_of(bbits)};
-    size_t stage = 0;
-    for (size_t w : in_w) stage += w + 4;
-    rc = hm_reserve(hc, 6 * n + 2 * ow + stage + 64);
-    if (rc) return rc;
-    hipStream_t st = hc.st;
-    u64 *F[6];
-    for (int i = 0; i < 6; i++) F[i] = hc.dbuf + (size_t)i * n;
-    u64 *dout = hc.dbuf + 6 * n, *dst = dout + 2 * ow;
-    const u64 *src[6] = {hm[0], hm[1], hm[2], hm[3], ha, hb};
-    const u64 bits[6] = {mbits[0], mbits[1], mbits[2], mbits[3], abits, bbits};
-    for (int i = 0; i < 6; i++) {
-        u64 c = (bits[i] + 31) >> 5;
-        if (!c) {
-            GF2T_HIP(hipMemsetAsync(F[i], 0, n * 8, st));
-            continue;
-        }
-        GF2T_HIP(hipMemcpyAsync(dst, src[i], in_w[i] * 8, hipMemcpyHostToDevice, st));
-        rc = run_passes(plan->fwd, F[i], st, (const u32 *)dst, c, last_mask(bits[i]));
-        if (rc) return rc;
-        dst += in_w[i] + 4;
-    }
-    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_mat2_pointwise, dim3(blocks), dim3(256), 0, st, F[0], F[1], F[2], F[3],
-                       F[4], F[5], n);
-    rc = run_passes(plan->inv, F[0], st, nullptr, 0, 0);
-    if (rc) return rc;
-    rc = run_passes(plan->inv, F[2], st, nullptr, 0, 0);
-    if (rc) return rc;
-    size_t nch = (size_t)(cm + cv - 1);
-    hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0, st, F[0],
-                       nch, dout, ow, (u64)0, (u64)0);
-    hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0, st, F[2],
-                       nch, dout + ow, ow, (u64)0, (u64)0);
-    GF2T_CHECK_LAUNCH("gf2t_mat2_host");
-    GF2T_HIP(hipMemcpyAsync(hoa, dout, ow * 8, hipMemcpyDeviceToHost, st));
-    GF2T_HIP(hipMemcpyAsync(hob, dout + ow, ow * 8, hipMemcpyDeviceToHost, st));
-    GF2T_HIP(hipStreamSynchronize(st));
-    return GF2T_OK;
+    return gf2t_mat2n_host(hm, mbits, 1, hv, vbits, hout, outw);
 }

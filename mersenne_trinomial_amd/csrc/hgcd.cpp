@@ -26,6 +26,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <sched.h>
 
 namespace gf2t_host {
 
@@ -34,13 +35,13 @@ namespace {
 std::atomic<u64> g_dev_bits{0};   // 0: host products only
 
 // Helper threads for independent products inside one gcd: a process-wide
-// budget of (hardware threads - 1), taken without blocking; a product
-// that finds the budget empty runs on the calling thread.
+// budget of (host threads - 1), taken without blocking; a product that
+// finds the budget empty runs on the calling thread.
 std::atomic<int> g_helpers{-1};
 bool take_helper() {
     int cur = g_helpers.load();
     if (cur < 0) {
-        int n = (int)std::thread::hardware_concurrency() - 1;
+        int n = host_threads() - 1;
         g_helpers.compare_exchange_strong(cur, n > 0 ? n : 0);
         cur = g_helpers.load();
     }
@@ -323,6 +324,47 @@ void apply(const HMat &M, const Poly &x, const Poly &y, Poly &ox, Poly &oy) {
     oy = add(mul_fast(M.m10, x), mul_fast(M.m11, y));
 }
 
+// (ox_i, oy_i) = M (x_i, y_i) for i < n (n <= 3) with one set of matrix
+// transforms on the device when the largest pair qualifies for it, else
+// pair by pair
+void applyN(const HMat &M, int n, const Poly *const *x, const Poly *const *y, Poly *const *ox,
+            Poly *const *oy) {
+    int64_t dm = std::max(std::max(deg(M.m00), deg(M.m01)), std::max(deg(M.m10), deg(M.m11)));
+    int64_t dv = -1;
+    for (int i = 0; i < n; i++) dv = std::max(dv, std::max(deg(*x[i]), deg(*y[i])));
+    u64 lim = g_dev_bits.load(std::memory_order_relaxed);
+    if (n > 1 && lim && dm + 1 >= (int64_t)lim && dv + 1 >= (int64_t)lim) {
+        const Poly *e[4] = {&M.m00, &M.m01, &M.m10, &M.m11};
+        const uint64_t *hm[4];
+        uint64_t mb[4];
+        for (int i = 0; i < 4; i++) {
+            hm[i] = e[i]->data();
+            mb[i] = (uint64_t)(deg(*e[i]) + 1);
+        }
+        const uint64_t *hv[6];
+        uint64_t vb[6];
+        uint64_t *ho[6];
+        size_t ow = gf2t_mul_out_words((u64)dm + 1, (u64)dv + 1);
+        for (int i = 0; i < n; i++) {
+            hv[2 * i] = x[i]->data();
+            vb[2 * i] = (uint64_t)(deg(*x[i]) + 1);
+            hv[2 * i + 1] = y[i]->data();
+            vb[2 * i + 1] = (uint64_t)(deg(*y[i]) + 1);
+            ox[i]->assign(ow, 0);
+            oy[i]->assign(ow, 0);
+            ho[2 * i] = ox[i]->data();
+            ho[2 * i + 1] = oy[i]->data();
+        }
+        check_rc(gf2t_mat2n_host(hm, mb, n, hv, vb, ho, ow), "gf2t_mat2n_host");
+        for (int i = 0; i < n; i++) {
+            trim(*ox[i]);
+            trim(*oy[i]);
+        }
+        return;
+    }
+    for (int i = 0; i < n; i++) apply(M, *x[i], *y[i], *ox[i], *oy[i]);
+}
+
 // ---- word-level base case --------------------------------------------
 inline int deg64(u64 v) { return v ? 63 - __builtin_clzll(v) : -1; }
 
@@ -463,6 +505,37 @@ void xor_shift_words(u64 *a, const u64 *b, size_t bw, u64 sh) {
     a[bw + ws] ^= carry;
 }
 
+int host_threads() {
+    static const int n = [] {
+        int k = (int)std::thread::hardware_concurrency();
+        cpu_set_t cs;
+        if (sched_getaffinity(0, sizeof cs, &cs) == 0 && CPU_COUNT(&cs) > 0)
+            k = std::min(k > 0 ? k : CPU_COUNT(&cs), CPU_COUNT(&cs));
+        for (const char *v : {"GF2T_HOST_THREADS", "OMP_NUM_THREADS"}) {
+            const char *e = getenv(v);
+            if (e && atoi(e) > 0) {
+                k = std::min(k > 0 ? k : atoi(e), atoi(e));
+                break;
+            }
+        }
+        return std::max(1, k);
+    }();
+    return n;
+}
+
+// deep-scan block size per r range: g_block_m[k] for 2^k <= r < 2^(k+1)
+// (0 = the built-in choice)
+static std::atomic<int> g_block_m[64];
+int tuned_block_m(u64 r) {
+    int k = 63 - __builtin_clzll(r | 1);
+    return g_block_m[k].load(std::memory_order_relaxed);
+}
+bool set_tuned_block_m(int k, int m) {
+    if (k < 2 || k > 40 || m < 0 || m > 64) return false;
+    g_block_m[k].store(m);
+    return true;
+}
+
 void set_device_product_bits(u64 bits) { g_dev_bits.store(bits); }
 u64 device_product_bits() { return g_dev_bits.load(); }
 
@@ -481,7 +554,7 @@ Poly mul_fast(const Poly &a, const Poly &b) {
     return host_mul(a.data(), na, b.data(), nb);
 }
 
-void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d) {
+void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d, bool want_M) {
     int64_t n = deg(a);
     int64_t m = (n + 1) / 2;
     if (deg(b) < m) {
@@ -527,14 +600,21 @@ void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d) {
     HMat S;
     Poly c3, d3;
     hgcd(shr(b1, (u64)k), shr(rem, (u64)k), S, c3, d3);
-    apply(S, low(b1, (u64)k), low(rem, (u64)k), c, d);
+    // S applied to the low parts of (b1, rem) and, for M = S * R2, to the
+    // columns (R2.m00, R2.m10) and (R2.m01, R2.m11) of R2, with one set of
+    // transforms of S
+    const Poly lb = low(b1, (u64)k), lr = low(rem, (u64)k);
+    if (want_M) {
+        const Poly *xs[3] = {&lb, &R2.m00, &R2.m01}, *ys[3] = {&lr, &R2.m10, &R2.m11};
+        Poly *oxs[3] = {&c, &M.m00, &M.m01}, *oys[3] = {&d, &M.m10, &M.m11};
+        applyN(S, 3, xs, ys, oxs, oys);
+    } else {
+        apply(S, lb, lr, c, d);
+    }
     xor_shift(c, c3, (u64)k);
     xor_shift(d, d3, (u64)k);
     trim(c);
     trim(d);
-    // M = S * R2, by columns: S applied to (R2.m00, R2.m10) and (R2.m01, R2.m11)
-    apply(S, R2.m00, R2.m10, M.m00, M.m10);
-    apply(S, R2.m01, R2.m11, M.m01, M.m11);
 }
 
 Poly gcd_fast(const Poly &a0, const Poly &b0) {
@@ -551,7 +631,7 @@ Poly gcd_fast(const Poly &a0, const Poly &b0) {
     while (deg(b) >= GCD_BASE) {
         HMat M;
         Poly c, d;
-        hgcd(a, b, M, c, d);
+        hgcd(a, b, M, c, d, false);   // only the remainders are needed here
         if (deg(d) < 0) return c;
         Poly q, r;
         divmod(c, d, q, r);

@@ -79,7 +79,19 @@ struct HMat {
 };
 // deg a > deg b >= -1: M and (c, d) = M (a, b), consecutive remainders
 // of (a, b) with deg c >= ceil(deg a / 2) > deg d
-void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d);
+// Host threads this process may use: the affinity mask, capped by
+// GF2T_HOST_THREADS or else OMP_NUM_THREADS when set (the job's CPU share
+// on a shared machine)
+int host_threads();
+
+// Deep-scan block size installed for r (gf2t_tune "block_m_<k>" covers
+// 2^k <= r < 2^(k+1)); 0 = none
+int tuned_block_m(u64 r);
+bool set_tuned_block_m(int k, int m);
+
+// want_M = false: the caller needs only (c, d), and the node's final
+// matrix product M = S R2 is skipped (M is then unspecified)
+void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d, bool want_M = true);
 Poly gcd_fast(const Poly &a, const Poly &b);      // monic; same as gcd()
 
 }  // namespace gf2t_host

@@ -51,6 +51,16 @@ struct gf2t_scan_results {
 
 namespace {
 
+// GF2T_SCAN_GRAPHS=0 issues every block's launches directly (for timing
+// the graphs against plain launches); default on
+bool scan_graphs_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("GF2T_SCAN_GRAPHS");
+        return !(e && *e == '0');
+    }();
+    return on;
+}
+
 #define SCAN_CHECK(call)                                                       \
     do {                                                                       \
         int rc__ = (call);                                                     \
@@ -89,9 +99,20 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     if (m > 1) SCAN_CHECK(C.alloc(pitch * B * 8));
     SCAN_CHECK(SV.alloc(B * 8));
     SCAN_CHECK(WORK.alloc(gf2t_mulmod_slab_workspace_bytes(r, B)));
+    // the cohort's own stream: the device work of a block can then be
+    // captured once as a HIP graph and replayed (see issue_block)
+    void *st = nullptr;
+    SCAN_CHECK(gf2t_stream_create(&st));
+    struct StreamGuard {
+        void *s;
+        ~StreamGuard() {
+            gf2t_stream_sync(s);
+            gf2t_stream_destroy(s);
+        }
+    } stream_guard{st};
     std::vector<u64> svals(B);
     for (int i = 0; i < B; i++) svals[i] = live[i].s;
-    SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, nullptr));
+    SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, st));
     uint64_t *cur = T0.w(), *oth = T1.w();
     double t_dev = 0, t_gcd = 0, t_res = 0, t0 = now_s();
     struct AddTimes {   // the cohort's phase times go into the results on every exit
@@ -101,13 +122,81 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     } add_times{out, t_dev, t_gcd, t_res};
     if (m > 1) {
         // h-table for the first block, degrees k0+1 .. k0+m
-        SCAN_CHECK(gf2t_ddf_init_slab(cur, oth, pitch, m, r, SV.w(), P.k0 + 1, B, nullptr));
+        SCAN_CHECK(gf2t_ddf_init_slab(cur, oth, pitch, m, r, SV.w(), P.k0 + 1, B, st));
     } else {
         std::vector<u64> host((size_t)B * pitch, 0);
         for (int i = 0; i < B; i++) host[(size_t)i * pitch] = 2;   // F = x
-        SCAN_CHECK(gf2t_memcpy_h2d(cur, host.data(), host.size() * 8, nullptr));
-        SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)B, 1, P.k0, nullptr));
+        SCAN_CHECK(gf2t_memcpy_h2d(cur, host.data(), host.size() * 8, st));
+        SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)B, 1, P.k0, st));
     }
+    // One block of the interval on the first nslot slots: m degrees of the
+    // blocked scheme (Horner product C, ACC *= C, m squarings of the
+    // h-table), or one degree of the plain one (F squared, ACC *= F + x).
+    // Its launches depend only on (cur, nslot), so after the first block
+    // it is captured as a HIP graph per (cur, nslot) and replayed: one
+    // graph launch instead of ~60 kernel launches per block.
+    auto issue_block = [&](uint64_t *&c, uint64_t *&o, int nslot) -> int {
+        if (m > 1) {
+            SCAN_CHECK(gf2t_horner_slab(c, pitch, m, r, SV.w(), C.w(), pitch, nslot, st));
+            SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(), pitch,
+                                        nslot, WORK.p, st));
+            SCAN_CHECK(slab_square_steps(c, o, pitch, r, SV.w(), (size_t)nslot * m, m, (u64)m, st));
+        } else {
+            SCAN_CHECK(gf2t_square_fold_slab(c, o, pitch, r, SV.w(), nslot, st));
+            std::swap(c, o);
+            SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, c, pitch, 1, r, SV.w(), ACC.w(), pitch,
+                                        nslot, WORK.p, st));
+        }
+        return GF2T_OK;
+    };
+    struct BlockGraph {
+        uint64_t *cin, *cout, *oout;
+        int nslot;
+        gf2t_graph *g;
+    };
+    std::vector<BlockGraph> graphs;
+    struct GraphGuard {
+        std::vector<BlockGraph> &v;
+        ~GraphGuard() {
+            for (auto &x : v) gf2t_graph_destroy(x.g);
+        }
+    } graph_guard{graphs};
+    const bool use_graphs = scan_graphs_enabled();
+    bool warmed = false;
+    auto run_block = [&](int nslot) -> int {
+        if (!use_graphs || !warmed) {   // the first block runs eagerly (plans, constants)
+            warmed = true;
+            return issue_block(cur, oth, nslot);
+        }
+        for (auto it = graphs.begin(); it != graphs.end();)
+            if (it->nslot != nslot) {   // the cohort was compacted: older graphs are stale
+                gf2t_graph_destroy(it->g);
+                it = graphs.erase(it);
+            } else {
+                ++it;
+            }
+        BlockGraph *G = nullptr;
+        for (auto &x : graphs)
+            if (x.cin == cur) G = &x;
+        if (!G) {
+            uint64_t *c = cur, *o = oth;
+            SCAN_CHECK(gf2t_graph_begin(st));
+            int rc = issue_block(c, o, nslot);
+            gf2t_graph *g = nullptr;
+            int rc2 = gf2t_graph_end(st, &g);
+            if (rc) {
+                gf2t_graph_destroy(g);
+                return rc;
+            }
+            if (rc2) return rc2;
+            graphs.push_back(BlockGraph{cur, c, o, nslot, g});
+            G = &graphs.back();
+        }
+        SCAN_CHECK(gf2t_graph_launch(G->g, st));
+        cur = G->cout;
+        oth = G->oout;
+        return GF2T_OK;
+    };
     // Work still queued on the device when the cohort ends (a speculative
     // interval) must finish before the buffers above are freed; declared
     // after them, this runs first.
@@ -125,26 +214,15 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         const u64 ka = k0 + 1;
         const u64 kend = std::min(k0 + L0, rhigh);
         kb = kend;
-        SCAN_CHECK(gf2t_set_one_slab(ACC.w(), pitch, nw, nslot, nullptr));
+        SCAN_CHECK(gf2t_set_one_slab(ACC.w(), pitch, nw, nslot, st));
         if (m > 1) {
             // whole blocks: the last may run past rhigh; factors found
             // beyond it are not reported
             u64 nblk = (kend - k0 + (u64)m - 1) / (u64)m;
             kb = k0 + nblk * (u64)m;
-            for (u64 b = 0; b < nblk; b++) {
-                SCAN_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, nslot, nullptr));
-                SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(),
-                                            pitch, nslot, WORK.p, nullptr));
-                SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)nslot * m, m,
-                                             (u64)m, nullptr));
-            }
+            for (u64 b = 0; b < nblk; b++) SCAN_CHECK(run_block(nslot));
         } else {
-            for (u64 deg_k = ka; deg_k <= kb; deg_k++) {
-                SCAN_CHECK(gf2t_square_fold_slab(cur, oth, pitch, r, SV.w(), nslot, nullptr));
-                std::swap(cur, oth);
-                SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, cur, pitch, 1, r, SV.w(), ACC.w(),
-                                            pitch, nslot, WORK.p, nullptr));
-            }
+            for (u64 deg_k = ka; deg_k <= kb; deg_k++) SCAN_CHECK(run_block(nslot));
         }
         return GF2T_OK;
     };
@@ -155,7 +233,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         // the interval (k, kb] of the n live slots is queued: wait for its
         // products
         acc_host.assign((size_t)n * pitch, 0);
-        SCAN_CHECK(gf2t_memcpy_d2h(acc_host.data(), ACC.p, acc_host.size() * 8, nullptr));
+        SCAN_CHECK(gf2t_memcpy_d2h(acc_host.data(), ACC.p, acc_host.size() * 8, st));
         double t1 = now_s();
         t_dev += t1 - t0;
         // speculation (src/tsfactor.cu:1279-1331): the next interval goes
@@ -202,17 +280,17 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             if (done[i]) continue;
             if (nl != i) {
                 SCAN_CHECK(gf2t_memcpy_d2d(cur + (size_t)nl * tab, cur + (size_t)i * tab,
-                                           tab * 8, nullptr));
+                                           tab * 8, st));
                 if (more)
                     SCAN_CHECK(gf2t_memcpy_d2d(ACC.w() + (size_t)nl * pitch,
-                                               ACC.w() + (size_t)i * pitch, pitch * 8, nullptr));
+                                               ACC.w() + (size_t)i * pitch, pitch * 8, st));
                 live[nl] = live[i];
                 svals[nl] = svals[i];
             }
             nl++;
         }
         if (nl != n && nl > 0)
-            SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)nl * 8, nullptr));
+            SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)nl * 8, st));
         if (P.verbose)
             fprintf(stderr, "  interval (%llu,%llu]: %d live, %zu hits, %d continue"
                     " (m=%d; device wait %.3fs, gcd %.3fs, resolve %.3fs so far)\n",

@@ -140,7 +140,8 @@ inline int slab_square_steps(u64 *&cur, u64 *&oth, size_t pitch, u64 r, const u6
 // r ~ 2^19 on (ratios of about 300 and 1100, from the squaring rate of
 // 4.7 TB/s and slab product times of ~5 us and ~8 ms per residue).
 inline int block_for(int requested, u64 r) {
-    int m = requested > 0 ? requested : (r < (1u << 14) ? 4 : r < (1u << 19) ? 16 : 32);
+    int m = requested > 0 ? requested : gf2t_host::tuned_block_m(r);
+    if (m <= 0) m = r < (1u << 14) ? 4 : r < (1u << 19) ? 16 : 32;
     if (m > 64) m = 64;
     while (m > 1 && 2 * (u64)m >= r) m /= 2;
     return m;

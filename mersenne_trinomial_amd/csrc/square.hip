@@ -446,8 +446,23 @@ namespace gf2t_host {
 bool square_fuse_enabled() { return sq_fuse(); }
 }
 
+int gf2t_fft_set_tile_bits(int m, int64_t T);   // fft.hip
+int gf2t_fft_tile_bits(int m);
+
 extern "C" int gf2t_tune(const char *key, int64_t value) {
     if (!key) return GF2T_EINVAL;
+    if (!strncmp(key, "fft_tile_", 9) && key[9]) {
+        char *end = nullptr;
+        long m = strtol(key + 9, &end, 10);
+        if (*end == 0) return gf2t_fft_set_tile_bits((int)m, value);
+    }
+    if (!strncmp(key, "block_m_", 8) && key[8]) {
+        char *end = nullptr;
+        long k = strtol(key + 8, &end, 10);
+        if (*end == 0 && gf2t_host::set_tuned_block_m((int)k, (int)value)) return GF2T_OK;
+        gf2t::set_error("gf2t_tune: block_m_<k> needs 2 <= k <= 40 and 0 <= m <= 64");
+        return GF2T_EINVAL;
+    }
     if (!strcmp(key, "sq_ppt")) { g_sq_ppt = (int)value; return GF2T_OK; }
     if (!strcmp(key, "sq_order")) { g_sq_order = (int)value; return GF2T_OK; }
     if (!strcmp(key, "sq_nt")) { g_sq_nt = (int)value; return GF2T_OK; }
@@ -457,6 +472,38 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
         return GF2T_OK;
     }
     gf2t::set_error("gf2t_tune: unknown key '%s'", key);
+    return GF2T_EINVAL;
+}
+
+extern "C" int gf2t_tune_get(const char *key, int64_t *value) {
+    if (!key || !value) return GF2T_EINVAL;
+    int ppt = 0, order = 0;
+    sq_tuning(ppt, order);
+    if (!strcmp(key, "sq_ppt")) { *value = ppt; return GF2T_OK; }
+    if (!strcmp(key, "sq_order")) { *value = order; return GF2T_OK; }
+    if (!strcmp(key, "sq_nt")) { *value = sq_nt() ? 1 : 0; return GF2T_OK; }
+    if (!strcmp(key, "sq_fuse")) { *value = sq_fuse() ? 1 : 0; return GF2T_OK; }
+    if (!strcmp(key, "gcd_dev_bits")) {
+        *value = (int64_t)gf2t_host::device_product_bits();
+        return GF2T_OK;
+    }
+    if (!strncmp(key, "fft_tile_", 9) && key[9]) {
+        char *end = nullptr;
+        long m = strtol(key + 9, &end, 10);
+        if (*end == 0 && gf2t_fft_tile_bits((int)m)) {
+            *value = gf2t_fft_tile_bits((int)m);
+            return GF2T_OK;
+        }
+    }
+    if (!strncmp(key, "block_m_", 8) && key[8]) {
+        char *end = nullptr;
+        long k = strtol(key + 8, &end, 10);
+        if (*end == 0 && k >= 2 && k <= 40) {
+            *value = gf2t_host::tuned_block_m(1ULL << k);
+            return GF2T_OK;
+        }
+    }
+    gf2t::set_error("gf2t_tune_get: unknown key '%s'", key);
     return GF2T_EINVAL;
 }
 

@@ -18,6 +18,11 @@ extern "C" int gf2t_mat2_host(const uint64_t *const *, const uint64_t *, const u
                               const uint64_t *, uint64_t, uint64_t *, uint64_t *, size_t) {
     abort();
 }
+extern "C" int gf2t_mat2n_host(const uint64_t *const *, const uint64_t *, int,
+                               const uint64_t *const *, const uint64_t *, uint64_t *const *,
+                               size_t) {
+    abort();
+}
 extern "C" size_t gf2t_mul_out_words(uint64_t, uint64_t) { abort(); }
 extern "C" const char *gf2t_last_error(void) { return "no device in this check"; }
 namespace gf2t_host {

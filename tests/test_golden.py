@@ -83,3 +83,23 @@ def test_scan_selftest_counts_r4423():
              for s in range(k["s_range"][0], k["s_range"][1] + 1)]
     assert sum(1 for x in kinds if x) == k["survivors"]
     assert kinds.count(1) == k["factors"] and kinds.count(2) == k["u"]
+
+
+def test_scan_one_reproduces_sieve_ref_lines_r4423():
+    """Per-value pin of the oracle's per-degree scan: for every s <= 2211 at
+    r = 4423, scan_one over degrees 2..16 gives exactly the line the
+    reference's sieve_ref printed for that s (degree and least mask), and
+    'no factor' for every s the sieve left."""
+    want = {}
+    import golden_data
+    with open(golden_data.GOLDEN + "/sieve_ref_r4423_d16_s2211.cert") as f:
+        for ln in f:
+            s, d, hx = ln.split()
+            want[int(s)] = (int(d), hx[1:])
+    assert len(want) > 1500
+    for s in range(1, 2212):
+        kind, d, hx, _ = O.scan_one(4423, s, 1, 16)
+        if s in want:
+            assert kind == 1 and (d, hx) == want[s], s
+        else:
+            assert kind == 2, s

@@ -105,3 +105,32 @@ def test_trin_gpu_interval_product(eng):
     assert np.array_equal(G.A(), F)
     assert np.array_equal(G.ACC(), acc)
     G.fini()
+
+
+@pytest.mark.parametrize("m", [6, 9, 12, 14, 15, 17])
+def test_every_tile_size_bit_identical(eng, m):
+    """The transform is the same whatever tile the passes use: every tile
+    size from 2^6 to 2^13 elements (gf2t_tune "fft_tile_<m>"), forward
+    against the oracle and back, and a product through the same plans."""
+    from mersenne_trinomial_amd._capi import lib
+    import ctypes as C
+    rng = np.random.default_rng(100 + m)
+    f = rng.integers(0, 2**64, 1 << m, dtype=np.uint64)
+    want = O.fft_fwd(f, m)
+    bits = (1 << (m + 4)) - 77         # a product that needs 2^m points
+    a, b = rand_bits(rng, bits), rand_bits(rng, 29)
+    pwant = O.fft_mul(a, bits, b, 29)
+    old = C.c_int64()
+    assert lib.gf2t_tune_get(f"fft_tile_{m}".encode(), C.byref(old)) == 0
+    try:
+        for T in range(6, 14):
+            print(f"m={m} tile 2^{T}", flush=True)
+            assert lib.gf2t_tune(f"fft_tile_{m}".encode(), T) == 0
+            F = eng.fft_fwd(f, m)
+            assert np.array_equal(F, want), T
+            assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f), T
+            assert np.array_equal(eng.mul(a, bits, b, 29), pwant), T
+    finally:
+        assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 0) == 0
+    assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 5) != 0
+    assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 14) != 0

@@ -44,6 +44,39 @@ def test_mul_host_vs_oracle(lib, abits, bbits):
     assert np.array_equal(out, O.mul_comb(a, abits, b, bbits))
 
 
+@pytest.mark.parametrize("npairs,mb,vb", [(1, 300, 7000), (2, 40000, 100000), (3, 70001, 260000),
+                                          (4, 64, 64)])
+def test_mat2n_host_vs_products(lib, npairs, mb, vb):
+    """One matrix applied to npairs vector pairs of different lengths (one
+    set of matrix transforms) equals the schoolbook products."""
+    rng = np.random.default_rng(npairs * 1000 + mb)
+    mbits = [mb - 3 * i if mb > 16 else mb for i in range(4)]
+    m = [rp(rng, x) for x in mbits]
+    vbits = [max(1, vb - 1000 * i) for i in range(2 * npairs)]
+    v = [rp(rng, x) for x in vbits]
+    ow = lib.gf2t_mul_out_words(max(mbits), max(vbits)) + 3
+    outs = [np.full(ow, 0xDEAD, dtype=np.uint64) for _ in range(2 * npairs)]
+    hm = (P64 * 4)(*[ptr(x) for x in m])
+    hv = (P64 * (2 * npairs))(*[ptr(x) for x in v])
+    ho = (P64 * (2 * npairs))(*[ptr(x) for x in outs])
+    mb_arr = np.array(mbits, dtype=np.uint64)
+    vb_arr = np.array(vbits, dtype=np.uint64)
+    assert lib.gf2t_mat2n_host(hm, ptr(mb_arr), npairs, hv, ptr(vb_arr), ho, ow) == 0
+
+    def prod(x, xb, y, yb):
+        p = O.fft_mul(x, xb, y, yb)
+        out = np.zeros(ow, dtype=np.uint64)
+        out[:len(p)] = p
+        return out
+
+    for p in range(npairs):
+        a, ab, b, bb = v[2 * p], vbits[2 * p], v[2 * p + 1], vbits[2 * p + 1]
+        assert np.array_equal(outs[2 * p], prod(m[0], mbits[0], a, ab) ^ prod(m[1], mbits[1], b, bb))
+        assert np.array_equal(outs[2 * p + 1],
+                              prod(m[2], mbits[2], a, ab) ^ prod(m[3], mbits[3], b, bb))
+    assert lib.gf2t_mat2n_host(hm, ptr(mb_arr), 5, hv, ptr(vb_arr), ho, ow) != 0
+
+
 @pytest.mark.parametrize("mb,vb", [(100, 5000), (30000, 90000), (1, 1)])
 def test_mat2_host_vs_products(lib, mb, vb):
     rng = np.random.default_rng(mb + vb)

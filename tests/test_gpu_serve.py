@@ -103,3 +103,49 @@ def test_request_loop_mix_equals_solo(M):
         solo = run([mk(i, sp)])
         assert solo[i] == mixed[i], sp
         assert len(mixed[i]) == sp[2] - sp[1] + 1
+
+
+def test_request_loop_serving_mix_shape(M):
+    """The serving mix's shape: jobs at r = 756839, 3021377 and 136279841
+    (short slices), sieve bound 32, deep bound 2048 with one interval gcd
+    per s.  Every job's certificate is the same in the mix as alone, and
+    every factor line passes the oracle's check (divides x^r + x^s + 1,
+    irreducible, of the stated degree)."""
+    from mersenne_trinomial_amd.serve import Job, RequestLoop
+    specs = [(756839, 1000, 1300), (3021377, 500000, 500200), (136279841, 20000000, 20000039)]
+
+    def run(jobs):
+        loop = RequestLoop(mem_fraction=0.5, L0=2048, Lmax=2048)
+        for j in jobs:
+            loop.submit(j)
+        done = {}
+        while loop.busy():
+            for j in loop.pump(2):
+                done[j.job_id] = j
+        for j in jobs:
+            done[j.job_id] = j
+        loop.close()
+        return {k: v.certificate() for k, v in done.items()}
+
+    def mk(i, spec):
+        r, lo, hi = spec
+        return Job(job_id=i, r=r, s_lo=lo, s_hi=hi, depth=32, maxd=2048)
+
+    print("serving mix: all jobs together", flush=True)
+    mixed = run([mk(i, sp) for i, sp in enumerate(specs)])
+    for i, sp in enumerate(specs):
+        r, lo, hi = sp
+        print(f"serving mix: r={r} alone", flush=True)
+        solo = run([mk(i, sp)])
+        assert solo[i] == mixed[i], sp
+        assert len(mixed[i]) == hi - lo + 1
+        deep = 0
+        for ln in mixed[i]:
+            f = ln.split()
+            assert int(f[0]) in range(lo, hi + 1)
+            if f[1] == "u":
+                continue
+            d, P = int(f[1]), O.poly_from_hex(f[2][1:])
+            assert O.deg(P) == d and O.verify_factor(r, int(f[0]), P), ln
+            deep += d > 32
+        print(f"r={r}: {len(mixed[i])} lines, {deep} from the deep scan", flush=True)
