@@ -69,7 +69,7 @@ int gf2t_event_elapsed_ms(void *start, void *stop, float *ms);
  * residue-ring, transform, product and slab entry points (gf2t_square_*,
  * gf2t_fold, gf2t_mulx_xor*, gf2t_set_*, gf2t_xor_*, gf2t_copy_slab,
  * gf2t_horner_slab, gf2t_fft_fwd / _inv, gf2t_mul, gf2t_mulmod,
- * gf2t_mulmod_slab); not the sieve, the host-buffer
+ * gf2t_mulmod_slab, gf2t_mulmod_noalloc); not the sieve, the host-buffer
  * services, copies to or from the host, gf2t_memset or
  * gf2t_ddf_init_slab.  The reference launches every kernel from the host
  * (src/tsfactor.cu:278-336); the deep scan replays one graph per block of
@@ -213,6 +213,22 @@ int gf2t_mul(const uint64_t *da, uint64_t abits, const uint64_t *db,
 int gf2t_mulmod(const uint64_t *da, uint64_t abits, const uint64_t *db,
                 uint64_t bbits, uint64_t r, uint64_t s, uint64_t *dout,
                 void *dwork, void *stream);
+
+/* dout (nw words) = a*b mod x^r + x^s + 1, as gf2t_mulmod, but with no
+ * workspace argument and no allocation: the transforms and the product
+ * live in a library-owned device array of 80 MiB per GPU
+ * (2 x 2^22 transform points + a 2^27-bit product), reserved when the
+ * library is loaded and kept until the process ends.  Limit: the two
+ * operands together hold at most 2^22 + 1 32-bit chunks (abits + bbits
+ * up to about 2^27, e.g. r up to 2^26 for two r-bit operands); larger
+ * products return GF2T_ERANGE.  Calls that overlap on one GPU share the
+ * array and race; calls ordered on one stream, or synchronized, do not.
+ * dout may alias da or db.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that must not call hipMalloc). */
+int gf2t_mulmod_noalloc(const uint64_t *da, uint64_t abits, const uint64_t *db,
+                        uint64_t bbits, uint64_t r, uint64_t s, uint64_t *dout,
+                        void *stream);
 
 /* Slab form of the interval accumulation: for i = 0..n-1,
  * out_i = a_i * (b_i + [b_add_x] x) mod x^r + x^s_i + 1, with a_i at

@@ -83,6 +83,7 @@ _SIGS = {
     "gf2t_mulmod_slab": (C.c_int, [_vp, _sz, _vp, _sz, C.c_int, _u64, _vp, _vp, _sz, C.c_int,
                                    _vp, _vp]),
     "gf2t_fold": (C.c_int, [_vp, _u64, _vp, _u64, _u64, _vp]),
+    "gf2t_mulmod_noalloc": (C.c_int, [_vp, _u64, _vp, _u64, _u64, _u64, _vp, _vp]),
     "gf2t_mulx_xor": (C.c_int, [_vp, _vp, _u64, _u64, _vp]),
     "gf2t_set_x": (C.c_int, [_vp, _sz, _vp]),
     "gf2t_set_one": (C.c_int, [_vp, _sz, _vp]),

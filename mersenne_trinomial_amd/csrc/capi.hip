@@ -37,6 +37,8 @@ extern "C" int gf2t_device_count(int *count) {
     return GF2T_OK;
 }
 
+int gf2t_noalloc_setup();   // fft.hip: per-device scratch of gf2t_mulmod_noalloc
+
 namespace {
 // The autotuner's choices: "key value" lines (gf2t_tune keys, '#' starts a
 // comment) from $GF2T_TUNE_FILE, or else gf2t_tune.txt beside this
@@ -88,6 +90,8 @@ extern "C" int gf2t_init(int device) {
     }
     GF2T_HIP(hipSetDevice(device));
     GF2T_HIP(hipFree(nullptr));   // create the context now
+    int rc = gf2t_noalloc_setup();
+    if (rc) return rc;
     load_tune_file();
     // with a device present, large gcd products go to the device
     if (!gf2t_host::device_product_bits()) {

@@ -638,6 +638,106 @@ extern "C" int gf2t_mulmod(const uint64_t *da, uint64_t abits, const uint64_t *d
     return gf2t_fold(prod, abits + bbits - 1, dout, r, s, stream);
 }
 
+// ------------------------------------------------ modmul without allocation
+// gf2t_mulmod_noalloc: gf2t_mulmod with its workspace in a library-owned
+// device array instead of a caller's buffer, for callers that cannot
+// allocate.  The array holds the two 2^22-point transforms and the
+// product (up to 2^27 bits): 80 MiB, which every GPU the library runs on
+// keeps until the process ends.  The pack pass writes every transform
+// element and the overlap-add every product word, so nothing in it needs
+// clearing between calls.  Calls that overlap on one GPU share the array
+// and race.
+namespace {
+constexpr int NA_MAX_M = 22;
+constexpr size_t NA_FFT_WORDS = (size_t)1 << NA_MAX_M;
+constexpr size_t NA_PROD_WORDS = ((size_t)1 << 21) + 2;   // gf2t_mul_out_words(a, b), a+b <= 2^27+64
+constexpr size_t NA_WORDS = 2 * NA_FFT_WORDS + NA_PROD_WORDS + 30;
+constexpr int NA_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) u64 scratch[NA_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+// device address of `scratch` on each device, from hipGetSymbolAddress
+// (the name itself is only a host-side stand-in)
+u64 *g_scratch_ptr[NA_MAX_DEVICES] = {};
+
+}  // namespace
+
+// Once per device, with that device current (gf2t_init calls it; the
+// entry point calls it on its first use on a device otherwise): record
+// the scratch array's device address, upload the Cantor basis and build
+// the transform plans the entry point can need, so that its calls
+// allocate nothing.
+int gf2t_noalloc_setup() {
+    int dev = 0;
+    GF2T_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) {
+        gf2t::set_error("gf2t_noalloc_setup: device %d beyond %d", dev, NA_MAX_DEVICES);
+        return GF2T_ERANGE;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *p = nullptr;
+            GF2T_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch_ptr[dev] = (u64 *)p;
+        }
+    }
+    for (int m = 1; m <= NA_MAX_M; m++) cached_plan(m);
+    return ensure_basis();
+}
+
+extern "C" int gf2t_mulmod_noalloc(const uint64_t *da, uint64_t abits, const uint64_t *db,
+                                   uint64_t bbits, uint64_t r, uint64_t s, uint64_t *dout,
+                                   void *stream) {
+    if (!da || !db || !dout) {
+        gf2t::set_error("gf2t_mulmod_noalloc: null pointer");
+        return GF2T_EINVAL;
+    }
+    if (!(r >= 3 && s >= 1 && 2 * s < r) || abits + bbits > 2 * r) {
+        gf2t::set_error("gf2t_mulmod_noalloc: need 1 <= s, 2s < r and a product of at most "
+                        "2r-1 bits (r=%llu s=%llu, %llu+%llu bits)", (unsigned long long)r,
+                        (unsigned long long)s, (unsigned long long)abits,
+                        (unsigned long long)bbits);
+        return GF2T_EINVAL;
+    }
+    const u64 ca = (abits + 31) >> 5, cb = (bbits + 31) >> 5;
+    if (ca && cb && ca + cb - 1 > NA_FFT_WORDS) {
+        gf2t::set_error("gf2t_mulmod_noalloc: a %llu x %llu-bit product needs more than the "
+                        "2^%d transform points its scratch holds (at most %llu 32-bit chunks "
+                        "in both operands together); use gf2t_mulmod with a workspace",
+                        (unsigned long long)abits, (unsigned long long)bbits, NA_MAX_M,
+                        (unsigned long long)(NA_FFT_WORDS + 1));
+        return GF2T_ERANGE;
+    }
+    int dev = 0;
+    GF2T_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) {
+        gf2t::set_error("gf2t_mulmod_noalloc: device %d beyond %d", dev, NA_MAX_DEVICES);
+        return GF2T_ERANGE;
+    }
+    u64 *ptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ptr = g_scratch_ptr[dev];
+    }
+    if (!ptr) {
+        int rc = gf2t_noalloc_setup();
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ptr = g_scratch_ptr[dev];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (!ca || !cb) return zero_words(dout, words_of(r), st);   // a zero operand
+    const size_t n = (size_t)1 << fft_m_for(ca + cb - 1);
+    u64 *prod = ptr + 2 * n;
+    int rc = mul_core(da, abits, db, bbits, prod, ptr, st);
+    if (rc) return rc;
+    return gf2t_fold(prod, abits + bbits - 1, dout, r, s, stream);
+}
+
 // ------------------------------------------------------------------ slabs
 namespace {
 __global__ void k_fold_slab(const u64 *__restrict__ p, size_t pw, u64 p_stride, u64 L,
