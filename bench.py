@@ -70,9 +70,11 @@ def parse():
                     help="override two squarings per launch (0/1; default on)")
     ap.add_argument("--scan-r", type=int, default=R, help="scan mode: trinomial degree r")
     ap.add_argument("--scan-s", type=str, default="474",
-                    help="scan mode: comma-separated s values")
+                    help="scan mode: comma-separated s values, or survivors:N for the first N "
+                         "sieve survivors at depth --scan-k0")
     ap.add_argument("--scan-k0", type=int, default=32, help="scan mode: depth already sieved")
     ap.add_argument("--scan-maxd", type=int, default=64, help="scan mode: degree bound")
+    ap.add_argument("--scan-l0", type=int, default=64, help="scan mode: first interval length")
     ap.add_argument("--scan-block", type=str, default="16,1",
                     help="scan mode: comma-separated block sizes m to run (1 = plain)")
     ap.add_argument("--out", type=str, default="",
@@ -103,7 +105,7 @@ def parse():
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
                          "fft: the Cantor-product batch (64 seeded operand pairs of 2^16..2^27 "
                          "bits, each product folded mod x^r + x^s + 1); sieve: --steps full-range "
-                         "coarse sieves at r = 136279841 to --depth (up to 34), per-degree "
+                         "coarse sieves at r = 136279841 to --depth (up to 36), per-degree "
                          "device times")
     return ap.parse_args()
 
@@ -377,12 +379,18 @@ def c4_pairs(seed: int):
 
 
 def mode_fft(args):
+    """The Cantor-product batch (64 seeded pairs of 2^16..2^27 bits, each
+    product folded mod x^136279841 + x^s + 1): one pair at a time
+    (gf2t_mulmod) and as one variable-size batch (gf2t_mulmod_batch); then
+    full 136279841 x 136279841-bit products (2^24 points) with the forward
+    and inverse transforms timed apart, for the transform pass's rate."""
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import check, lib
     from mersenne_trinomial_amd.device import DeviceBuffer, Event
     device.init(0)
     nw = (R + 63) // 64
     pairs = c4_pairs(args.seed)
+    n = len(pairs)
     dev = []
     wsmax = 0
     for a, ab, b, bb, s in pairs:
@@ -390,24 +398,78 @@ def mode_fft(args):
                     DeviceBuffer.words(nw)))
         wsmax = max(wsmax, int(lib.gf2t_mul_workspace_bytes(ab, bb)))
     work = DeviceBuffer(wsmax)
+    P64 = C.POINTER(C.c_uint64)
+    abits = np.array([p[1] for p in dev], dtype=np.uint64)
+    bbits = np.array([p[3] for p in dev], dtype=np.uint64)
+    svec = np.array([p[4] for p in dev], dtype=np.uint64)
+    vp = C.c_void_p * n
+    pa, pb, po = (vp(*[p[0].ptr for p in dev]), vp(*[p[2].ptr for p in dev]),
+                  vp(*[p[5].ptr for p in dev]))
+    bwork = DeviceBuffer(lib.gf2t_mulmod_batch_workspace_bytes(n, abits.ctypes.data_as(P64),
+                                                               bbits.ctypes.data_as(P64)))
 
-    def run():
+    def one_by_one():
         for da, ab, db, bb, s, do in dev:
             check(lib.gf2t_mulmod(da.ptr, ab, db.ptr, bb, R, s, do.ptr, work.ptr, None),
                   "gf2t_mulmod")
 
-    for _ in range(args.warmup):
-        run()
-    device.synchronize()
-    e0, e1 = Event(), Event()
-    e0.record()
-    for _ in range(args.steps):
-        run()
-    e1.record()
-    ms = e0.elapsed_ms(e1) / args.steps
+    def batch():
+        check(lib.gf2t_mulmod_batch(n, pa, abits.ctypes.data_as(P64), pb,
+                                    bbits.ctypes.data_as(P64), R, svec.ctypes.data_as(P64), po,
+                                    bwork.ptr, None), "gf2t_mulmod_batch")
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        device.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record()
+        for _ in range(args.steps):
+            fn()
+        e1.record()
+        device.synchronize()
+        return e0.elapsed_ms(e1) / args.steps
+
+    ms_one = timed(one_by_one)
+    ms_batch = timed(batch)
     bits = sum(ab + bb for _, ab, _, bb, _ in pairs)
-    print(json.dumps({"mode": "fft", "products": 64, "ms_per_batch": ms,
-                      "products_per_s": 64 / (ms * 1e-3), "operand_bits": bits}), flush=True)
+    # full-size products: forward pair, pointwise, inverse timed with events
+    plan = C.c_void_p()
+    check(lib.gf2t_fft_plan_create(24, C.byref(plan)))
+    fp, ip = C.c_int(), C.c_int()
+    check(lib.gf2t_fft_plan_passes(plan, C.byref(fp), C.byref(ip)))
+    f = DeviceBuffer.words(1 << 24)
+    f.fill(0x3C)
+    e = [Event() for _ in range(3)]
+    reps = max(3, args.steps)
+    check(lib.gf2t_fft_fwd(plan, f.ptr, None))
+    check(lib.gf2t_fft_inv(plan, f.ptr, None))
+    device.synchronize()
+    e[0].record()
+    for _ in range(reps):
+        check(lib.gf2t_fft_fwd(plan, f.ptr, None))
+    e[1].record()
+    for _ in range(reps):
+        check(lib.gf2t_fft_inv(plan, f.ptr, None))
+    e[2].record()
+    device.synchronize()
+    fwd_ms, inv_ms = e[0].elapsed_ms(e[1]) / reps, e[1].elapsed_ms(e[2]) / reps
+    lib.gf2t_fft_plan_destroy(plan)
+    pass_bytes = 16 * (1 << 24)
+    fwd_pass_ms = fwd_ms / fp.value
+    mults = 24 * (1 << 23)            # GF(2^64) butterfly products per transform
+    print(json.dumps({"mode": "fft", "products": n, "operand_bits": bits,
+                      "one_by_one_ms": ms_one, "one_by_one_products_per_s": n / (ms_one * 1e-3),
+                      "batch_ms": ms_batch, "batch_products_per_s": n / (ms_batch * 1e-3),
+                      "batch_speedup": ms_one / ms_batch,
+                      "transform_2p24": {"fwd_ms": fwd_ms, "inv_ms": inv_ms,
+                                         "fwd_passes": fp.value, "inv_passes": ip.value,
+                                         "pass_ms": fwd_pass_ms,
+                                         "pass_GB_per_s": pass_bytes / (fwd_pass_ms * 1e-3) / 1e9,
+                                         "hbm_frac_per_pass": pass_bytes / (fwd_pass_ms * 1e-3)
+                                         / 1e9 / HBM_PEAK_GBS,
+                                         "gf64_mul_per_s": mults / (fwd_ms * 1e-3)}}),
+          flush=True)
 
 
 def mode_scan(args):
@@ -416,15 +478,25 @@ def mode_scan(args):
     import mersenne_trinomial_amd as M
     from mersenne_trinomial_amd import device
     device.init(0)
-    svals = [int(x) for x in args.scan_s.split(",") if x]
+    if args.scan_s.startswith("survivors:"):
+        # the first N sieve survivors at depth k0 (s >= 3)
+        n = int(args.scan_s.split(":")[1])
+        smax = (args.scan_r - 1) // 2
+        best, _ = M.run_gpu(args.scan_r, args.scan_k0, smax)
+        sv = np.nonzero(best == np.uint64(0xFFFFFFFFFFFFFFFF))[0]
+        svals = [int(x) for x in sv[sv >= 3][:n]]
+    else:
+        svals = [int(x) for x in args.scan_s.split(",") if x]
     for bm in [int(x) for x in args.scan_block.split(",") if x]:
+        times = {}
         t0 = time.perf_counter()
-        out = M.scan(args.scan_r, svals, k0=args.scan_k0, maxd=args.scan_maxd, L0=64,
-                     Lmax=4096, verbose=1, block_m=bm)
+        out = M.scan(args.scan_r, svals, k0=args.scan_k0, maxd=args.scan_maxd, L0=args.scan_l0,
+                     Lmax=4096, verbose=1, block_m=bm, times=times)
         dt = time.perf_counter() - t0
         print(json.dumps({"mode": "scan", "r": args.scan_r, "k0": args.scan_k0,
                           "maxd": args.scan_maxd, "block_m": bm, "n_s": len(svals),
-                          "seconds": dt, "s_per_second": len(svals) / dt,
+                          "graphs": os.environ.get("GF2T_SCAN_GRAPHS", "1") != "0",
+                          "seconds": dt, "s_per_second": len(svals) / dt, **times,
                           "lines": [o.line() for o in out][:16]}), flush=True)
 
 
@@ -485,15 +557,15 @@ def mode_gcd(args):
 def mode_sieve(args):
     """The coarse sieve alone: --warmup + --steps runs over every s <= (r-1)/2
     to --depth, each reported with its wall time and the device time of
-    the top degrees (log table / classes+marking).  Depths 33..34 build the
-    wide tables (about 201 GB of HBM at 34)."""
+    the top degrees (log table / classes+marking).  Depths 33..36 build the
+    wide tables (about 205 GB of HBM at 34; 35 and 36 in 2 and 4 slices of that size)."""
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import SieveStats, check, lib
     from mersenne_trinomial_amd.device import DeviceBuffer
 
     device.init(0)
     dbest = DeviceBuffer.words(SMAX + 1)
-    stats = (SieveStats * 35)()
+    stats = (SieveStats * 37)()
     walls = []
     for i in range(args.warmup + args.steps):
         dbest.fill(0xFF)
@@ -704,7 +776,7 @@ def main():
     P64 = C.POINTER(C.c_uint64)
     rr = np.full(n, R, dtype=np.uint64)
     kk = np.full(n, args.chain_steps, dtype=np.uint64)
-    stats = (SieveStats * 35)()
+    stats = (SieveStats * 37)()
     svals = None
 
     def one_step(timed: bool):

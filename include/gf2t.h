@@ -230,6 +230,22 @@ int gf2t_mulmod_noalloc(const uint64_t *da, uint64_t abits, const uint64_t *db,
                         uint64_t bbits, uint64_t r, uint64_t s, uint64_t *dout,
                         void *stream);
 
+/* n products of different sizes, each folded: dout[j] (nw words) =
+ * a_j * b_j mod x^r + x^s[j] + 1 (da, db, dout: host arrays of device
+ * pointers; abits, bbits, s: host arrays), with dwork of
+ * gf2t_mulmod_batch_workspace_bytes(n, abits, bbits) bytes.  Every pair
+ * keeps its own transform size, and the batch runs as one set of
+ * launches: the i-th pass of every pair's forward (inverse) transforms is
+ * one grouped launch, and the pointwise products, overlap-adds and folds
+ * are one launch each.  Synchronous with respect to the host only for
+ * its small descriptor upload.  No reference counterpart: the reference
+ * multiplies one pair at a time (gf2x_fft_mul_buf,
+ * src/gf2_cantor_engine.h:207-229). */
+size_t gf2t_mulmod_batch_workspace_bytes(int n, const uint64_t *abits, const uint64_t *bbits);
+int gf2t_mulmod_batch(int n, const uint64_t *const *da, const uint64_t *abits,
+                      const uint64_t *const *db, const uint64_t *bbits, uint64_t r,
+                      const uint64_t *s, uint64_t *const *dout, void *dwork, void *stream);
+
 /* Slab form of the interval accumulation: for i = 0..n-1,
  * out_i = a_i * (b_i + [b_add_x] x) mod x^r + x^s_i + 1, with a_i at
  * da + i*apitch, b_i at db + i*bpitch, out_i at dout + i*opitch (r-bit
@@ -325,14 +341,18 @@ typedef struct gf2t_sieve_stats {
 } gf2t_sieve_stats;
 
 /* Mark every s in [1, smax] divisible by an irreducible of degree d in
- * [dmin, dmax] (2 <= dmin <= dmax <= 34) into dbest (smax+1 uint64 on
+ * [dmin, dmax] (2 <= dmin <= dmax <= 36) into dbest (smax+1 uint64 on
  * the device, atomic min of (d<<48)|mask; initialise to ~0 or a prior
  * image).  Replaces run_gpu/sieve_degree_gpu (src/coarse_sieve.cu:852-
  * 918, :1119-1225).  stats (host, dmax+1 entries, may be NULL) gets one
- * record per degree.  Degrees 33..34 (coarse_sieve_wide, src/
- * gf2_wide_field.h) use 64-bit field elements and dense tables of about
- * 201 GB at d = 34; the tables of degrees <= 32 are freed first.
- * Synchronous. */
+ * record per degree.  Degrees 33..36 (coarse_sieve_wide, src/
+ * gf2_wide_field.h) use 64-bit field elements and tables of about 205 GB
+ * at d = 34; degrees 35 and 36 build and use the log table in 2 and 4
+ * value slices of that size (the role of the reference's low-memory
+ * buckets, src/coarse_sieve.cu:962-1117); the tables of degrees <= 32 are
+ * freed first.  Test knobs: gf2t_tune "sieve_wide_from" (25..33) runs
+ * degrees from there on the 64-bit path, "sieve_slices" (1..16) forces at
+ * least that many slices.  Synchronous. */
 int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax,
                uint64_t *dbest, gf2t_sieve_stats *stats);
 

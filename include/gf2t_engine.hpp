@@ -413,7 +413,7 @@ inline void run_gpu(u64 r, int depth, u64 smax, std::vector<u64> &best, int min_
     if (best.size() != smax + 1) best.assign(smax + 1, ~0ULL);
     DevWords d(smax + 1);
     d.upload(best.data(), smax + 1);
-    std::vector<gf2t_sieve_stats> st(35);
+    std::vector<gf2t_sieve_stats> st(37);
     check(gf2t_sieve(r, min_depth, depth, smax, d.p, st.data()), "gf2t_sieve");
     d.download(best.data(), smax + 1);
     if (stats) *stats = st;

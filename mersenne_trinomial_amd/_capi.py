@@ -84,6 +84,9 @@ _SIGS = {
                                    _vp, _vp]),
     "gf2t_fold": (C.c_int, [_vp, _u64, _vp, _u64, _u64, _vp]),
     "gf2t_mulmod_noalloc": (C.c_int, [_vp, _u64, _vp, _u64, _u64, _u64, _vp, _vp]),
+    "gf2t_mulmod_batch_workspace_bytes": (_sz, [C.c_int, _p64, _p64]),
+    "gf2t_mulmod_batch": (C.c_int, [C.c_int, C.POINTER(_vp), _p64, C.POINTER(_vp), _p64, _u64,
+                                    _p64, C.POINTER(_vp), _vp, _vp]),
     "gf2t_mulx_xor": (C.c_int, [_vp, _vp, _u64, _u64, _vp]),
     "gf2t_set_x": (C.c_int, [_vp, _sz, _vp]),
     "gf2t_set_one": (C.c_int, [_vp, _sz, _vp]),

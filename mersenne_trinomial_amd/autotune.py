@@ -169,7 +169,7 @@ def tune_block_m(r: int, reps: int = 3):
 
 
 def write(path: str, fft: dict, sq: dict, blocks: list, device: str) -> None:
-    lines = [f"# gf2t autotuner ({device}), {time.strftime('%Y-%m-%d %H:%M:%S')}",
+    lines = [f"# gf2t autotuner ({device.strip()}), {time.strftime('%Y-%m-%d %H:%M:%S')}",
              "# key value -- applied by gf2t_init; see mersenne_trinomial_amd/autotune.py"]
     for m, v in sorted(fft.items()):
         lines.append(f"fft_tile_{m} {v['chosen']}")

@@ -108,37 +108,19 @@ GF2T_DEV u64 packed_chunk(const u32 *__restrict__ src, u64 nchunks, u32 lastmask
     return v;
 }
 
-// blockIdx.y selects one transform of a batch: element y lives at
-// f + y*f_stride.  pack (first pass only) describes the input to pack.
-// The block has 2^(T-1) threads (at most FFT_THREADS) for its 2^T-element
-// tile; ntab byte tables of twiddles cover the twiddle indices t < 2^(8
-// ntab) this transform size uses.
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(u64 *__restrict__ f, const PassDesc pd,
-                                                          const PackArg pk, int pack, int ntab,
-                                                          u64 f_stride) {
-    extern __shared__ u64 lds[];
-    const unsigned y = blockIdx.y;
-    f += (u64)y * f_stride;
-    const u32 *psrc = nullptr;
-    u64 pn = 0;
-    u32 pmask = 0;
-    if (pack) {
-        if (pk.nlist) {
-            psrc = pk.lsrc[y];
-            pn = pk.lnch[y];
-            pmask = pk.lmask[y];
-        } else {
-            psrc = pk.src + (u64)y * pk.stride;
-            pn = pk.nchunks;
-            pmask = pk.lastmask;
-        }
-    }
+// One pass over the tile `blk` of the transform at f (2^m elements):
+// load the tile (packing the input chunks on the first pass), apply the
+// pass's ops in LDS, store.  The block's threads stride over the tile;
+// ntab byte tables of twiddles cover the twiddle indices t < 2^(8 ntab)
+// the transform uses.
+GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
+                            const u32 *__restrict__ psrc, u64 pn, u32 pmask, u32 xor0, int ntab,
+                            u64 blk, u64 *lds) {
     TwTables &tw = *reinterpret_cast<TwTables *>(lds);
     u64 *tile = lds + 4 * 256;
     const int T = pd.c + pd.w;
     const u64 tsize = 1ULL << T;
     const unsigned nthr = blockDim.x;
-    const u64 blk = blockIdx.x;
     const int nlo = pd.a - pd.c;
     const u64 rest_lo = blk & ((1ULL << nlo) - 1);
     const u64 rest_hi = blk >> nlo;
@@ -160,7 +142,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(u64 *__restrict__ f, c
         u64 v;
         if (pack) {
             v = packed_chunk(psrc, pn, pmask, g);
-            if (g == 0) v ^= pk.xor0;
+            if (g == 0) v ^= xor0;
         } else {
             v = f[g];
         }
@@ -205,6 +187,62 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(u64 *__restrict__ f, c
     }
     for (u64 li = threadIdx.x; li < tsize; li += nthr)
         f[global_index(pd, li, rest_lo, rest_hi)] = tile[li];
+}
+
+// blockIdx.y selects one transform of a batch: element y lives at
+// f + y*f_stride.  pack (first pass only) describes the input to pack.
+// The block has 2^(T-1) threads (at most FFT_THREADS) for its 2^T-element
+// tile.
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(u64 *__restrict__ f, const PassDesc pd,
+                                                          const PackArg pk, int pack, int ntab,
+                                                          u64 f_stride) {
+    extern __shared__ u64 lds[];
+    const unsigned y = blockIdx.y;
+    const u32 *psrc = nullptr;
+    u64 pn = 0;
+    u32 pmask = 0;
+    if (pack) {
+        if (pk.nlist) {
+            psrc = pk.lsrc[y];
+            pn = pk.lnch[y];
+            pmask = pk.lmask[y];
+        } else {
+            psrc = pk.src + (u64)y * pk.stride;
+            pn = pk.nchunks;
+            pmask = pk.lastmask;
+        }
+    }
+    fft_pass_tile(f + (u64)y * f_stride, pd, pack != 0, psrc, pn, pmask, pk.xor0, ntab,
+                  blockIdx.x, lds);
+}
+
+// A grouped launch: one pass each of several transforms of different
+// sizes (the i-th pass of every product in a batch that has one).  Entry
+// e owns blocks [block0, block0 + nblocks * ntrans) of the grid: ntrans
+// transforms n words apart, nblocks tiles each; a block finds its entry
+// by its index.
+struct GroupPass {
+    PassDesc pd;
+    u64 *f;
+    u64 f_stride;
+    const u32 *src[2];
+    u64 nch[2];
+    u32 mask[2];
+    int pack, ntab;
+    u32 block0, nblocks, ntrans, pad;
+};
+
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass_grouped(
+        const GroupPass *__restrict__ gp, int nent) {
+    extern __shared__ u64 lds[];
+    const u32 b = blockIdx.x;
+    int e = 0;
+    while (e + 1 < nent && gp[e + 1].block0 <= b) e++;
+    const GroupPass &G = gp[e];
+    const u32 local = b - G.block0;
+    const u32 y = local / G.nblocks, x = local - y * G.nblocks;
+    fft_pass_tile(G.f + (u64)y * G.f_stride, G.pd, G.pack != 0, G.pack ? G.src[y] : nullptr,
+                  G.pack ? G.nch[y] : 0, G.pack ? G.mask[y] : 0, 0, G.ntab, x, lds);
 }
 
 __global__ void k_pointwise(u64 *__restrict__ fa, const u64 *__restrict__ fb, size_t n) {
@@ -736,6 +774,243 @@ extern "C" int gf2t_mulmod_noalloc(const uint64_t *da, uint64_t abits, const uin
     int rc = mul_core(da, abits, db, bbits, prod, ptr, st);
     if (rc) return rc;
     return gf2t_fold(prod, abits + bbits - 1, dout, r, s, stream);
+}
+
+// ------------------------------------------------- variable-size batch
+// gf2t_mulmod_batch: products of different sizes, each folded mod
+// x^r + x^s_j + 1, in one set of launches: the i-th forward (inverse) pass
+// of every product that has one shares a grouped launch, and the
+// pointwise products, overlap-adds and folds of all pairs are one launch
+// each.  Each product keeps its own transform size and tile.
+namespace {
+struct PairSeg {       // one pair's part of a flat grouped launch over elements
+    u64 *fa;
+    const u64 *fb;
+    u64 *prod;
+    u64 *out;
+    u64 n, nch, ow, pw, L, s, start;   // start: the pair's first element in the launch
+};
+
+GF2T_DEV int find_seg(const PairSeg *__restrict__ sg, int np, u64 i) {
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sg[mid].start <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// phase 0: FA *= FB pointwise; 1: overlap-add FA into prod; 2: fold prod
+// into out.  total = sum of the pairs' element counts for the phase.
+__global__ void k_pairs_grouped(const PairSeg *__restrict__ sg, int np, u64 total, int phase,
+                                u64 r) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (; i < total; i += stride) {
+        const PairSeg &S = sg[find_seg(sg, np, i)];
+        const u64 k = i - S.start;
+        if (phase == 0) {
+            S.fa[k] = gf64_mul(S.fa[k], S.fb[k]);
+        } else if (phase == 1) {
+            const u64 c = 2 * k;
+            u64 v = 0;
+            if (c < S.nch) v ^= S.fa[c];
+            if (c + 1 < S.nch) v ^= S.fa[c + 1] << 32;
+            if (k && c - 1 < S.nch) v ^= S.fa[c - 1] >> 32;
+            S.prod[k] = v;
+        } else {
+            S.out[k] = fold_word(S.prod, S.pw, S.L, r, S.s, k);
+        }
+    }
+}
+
+struct BatchLayout {
+    std::vector<size_t> fa, prod;   // word offsets of each pair's transforms and product
+    std::vector<int> m;
+    size_t desc_off = 0, words = 0;
+};
+
+BatchLayout batch_layout(int n, const uint64_t *abits, const uint64_t *bbits, size_t ndesc_bytes) {
+    BatchLayout L;
+    size_t off = 0;
+    for (int j = 0; j < n; j++) {
+        u64 ca = (abits[j] + 31) >> 5, cb = (bbits[j] + 31) >> 5;
+        int m = (ca && cb) ? fft_m_for(ca + cb - 1) : 0;
+        L.m.push_back(m);
+        L.fa.push_back(off);
+        off += m ? 2 * ((size_t)1 << m) : 0;
+        L.prod.push_back(off);
+        off += (gf2t_mul_out_words(abits[j], bbits[j]) + 31) & ~(size_t)31;
+    }
+    L.desc_off = off;
+    L.words = off + (ndesc_bytes + 7) / 8 + 32;
+    return L;
+}
+
+// room for the descriptors whatever tiles the plans use: a pass holds at
+// least one op, so a transform has at most m(m+1)/2 passes each way
+size_t batch_desc_bytes(int n, const std::vector<int> &ms) {
+    size_t steps = 0;
+    for (int m : ms)
+        if (m) steps += 2 * (size_t)(m * (m + 1) / 2);
+    return steps * sizeof(GroupPass) + (size_t)n * 3 * sizeof(PairSeg) + 256;
+}
+}  // namespace
+
+extern "C" size_t gf2t_mulmod_batch_workspace_bytes(int n, const uint64_t *abits,
+                                                   const uint64_t *bbits) {
+    if (n <= 0 || !abits || !bbits) return 0;
+    BatchLayout L0 = batch_layout(n, abits, bbits, 0);
+    return batch_layout(n, abits, bbits, batch_desc_bytes(n, L0.m)).words * 8;
+}
+
+extern "C" int gf2t_mulmod_batch(int n, const uint64_t *const *da, const uint64_t *abits,
+                                 const uint64_t *const *db, const uint64_t *bbits, uint64_t r,
+                                 const uint64_t *s, uint64_t *const *dout, void *dwork,
+                                 void *stream) {
+    if (n < 0 || (n > 0 && (!da || !abits || !db || !bbits || !s || !dout || !dwork))) {
+        gf2t::set_error("gf2t_mulmod_batch: bad arguments");
+        return GF2T_EINVAL;
+    }
+    if (n == 0) return GF2T_OK;
+    for (int j = 0; j < n; j++) {
+        if (!da[j] || !db[j] || !dout[j] || !(r >= 3 && s[j] >= 1 && 2 * s[j] < r) ||
+            abits[j] + bbits[j] > 2 * r) {
+            gf2t::set_error("gf2t_mulmod_batch: pair %d: bad pointers, s, or a product over "
+                            "2r-1 bits", j);
+            return GF2T_EINVAL;
+        }
+        u64 ca = (abits[j] + 31) >> 5, cb = (bbits[j] + 31) >> 5;
+        if (ca && cb && fft_m_for(ca + cb - 1) > FFT_MAX_M) {
+            gf2t::set_error("gf2t_mulmod_batch: pair %d needs more than 2^%d points", j,
+                            FFT_MAX_M);
+            return GF2T_ERANGE;
+        }
+    }
+    int rc = ensure_basis();
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    BatchLayout L0 = batch_layout(n, abits, bbits, 0);
+    BatchLayout L = batch_layout(n, abits, bbits, batch_desc_bytes(n, L0.m));
+    u64 *W = (u64 *)dwork;
+    // the steps of the grouped launches: forward passes, then inverse
+    std::vector<const gf2t_fft_plan *> plan(n, nullptr);
+    size_t maxf = 0, maxi = 0;
+    for (int j = 0; j < n; j++)
+        if (L.m[j]) {
+            plan[j] = cached_plan(L.m[j]);
+            maxf = std::max(maxf, plan[j]->fwd.size());
+            maxi = std::max(maxi, plan[j]->inv.size());
+        }
+    std::vector<GroupPass> desc;
+    struct Step { size_t first; int count; u32 blocks; int T; };
+    std::vector<Step> steps;
+    for (int dir = 0; dir < 2; dir++) {
+        const size_t ns = dir ? maxi : maxf;
+        for (size_t i = 0; i < ns; i++) {
+            Step S{desc.size(), 0, 0, 0};
+            for (int j = 0; j < n; j++) {
+                if (!plan[j]) continue;
+                const std::vector<PassDesc> &ps = dir ? plan[j]->inv : plan[j]->fwd;
+                if (i >= ps.size()) continue;
+                GroupPass g{};
+                g.pd = ps[i];
+                const size_t nn = (size_t)1 << L.m[j];
+                g.f = W + L.fa[j];
+                g.f_stride = nn;
+                g.ntrans = dir ? 1 : 2;
+                g.pack = (!dir && i == 0) ? 1 : 0;
+                g.src[0] = (const u32 *)da[j];
+                g.src[1] = (const u32 *)db[j];
+                g.nch[0] = (abits[j] + 31) >> 5;
+                g.nch[1] = (bbits[j] + 31) >> 5;
+                g.mask[0] = (abits[j] & 31) ? (u32)((1ULL << (abits[j] & 31)) - 1) : 0xFFFFFFFFu;
+                g.mask[1] = (bbits[j] & 31) ? (u32)((1ULL << (bbits[j] & 31)) - 1) : 0xFFFFFFFFu;
+                g.ntab = std::max(1, std::min(4, (L.m[j] - 1 + 7) / 8));
+                const int T = g.pd.c + g.pd.w;
+                g.nblocks = (u32)(1ULL << (g.pd.m - T));
+                g.block0 = S.blocks;
+                S.blocks += g.nblocks * g.ntrans;
+                S.T = std::max(S.T, T);
+                S.count++;
+                desc.push_back(g);
+            }
+            steps.push_back(S);
+        }
+    }
+    std::vector<PairSeg> seg[3];
+    u64 total[3] = {0, 0, 0};
+    int nz = 0;
+    for (int j = 0; j < n; j++) {
+        const size_t nw = words_of(r);
+        if (!L.m[j]) {   // a zero operand: the residue is 0
+            rc = zero_words(dout[j], nw, st);
+            if (rc) return rc;
+            nz++;
+            continue;
+        }
+        const u64 nn = 1ULL << L.m[j], ca = (abits[j] + 31) >> 5, cb = (bbits[j] + 31) >> 5;
+        const u64 ow = gf2t_mul_out_words(abits[j], bbits[j]);
+        PairSeg p{};
+        p.fa = W + L.fa[j];
+        p.fb = p.fa + nn;
+        p.prod = W + L.prod[j];
+        p.out = dout[j];
+        p.n = nn;
+        p.nch = ca + cb - 1;
+        p.ow = ow;
+        p.pw = ow;
+        p.L = abits[j] + bbits[j] - 1;
+        p.s = s[j];
+        const u64 cnt[3] = {nn, ow, (u64)nw};
+        for (int ph = 0; ph < 3; ph++) {
+            p.start = total[ph];
+            total[ph] += cnt[ph];
+            seg[ph].push_back(p);
+        }
+    }
+    const int np = n - nz;
+    if (np == 0) return GF2T_OK;
+    // descriptors: the pass entries, then the three segment tables
+    std::vector<unsigned char> host(desc.size() * sizeof(GroupPass) + 3 * np * sizeof(PairSeg));
+    std::memcpy(host.data(), desc.data(), desc.size() * sizeof(GroupPass));
+    for (int ph = 0; ph < 3; ph++)
+        std::memcpy(host.data() + desc.size() * sizeof(GroupPass) + ph * np * sizeof(PairSeg),
+                    seg[ph].data(), np * sizeof(PairSeg));
+    unsigned char *dd = (unsigned char *)(W + L.desc_off);
+    dd = (unsigned char *)(((uintptr_t)dd + 255) & ~(uintptr_t)255);
+    GF2T_HIP(hipMemcpyAsync(dd, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    GF2T_HIP(hipStreamSynchronize(st));   // `host` is a pageable local buffer
+    const GroupPass *dgp = (const GroupPass *)dd;
+    const PairSeg *dseg = (const PairSeg *)(dd + desc.size() * sizeof(GroupPass));
+    static std::once_flag lds_once;
+    static hipError_t lds_err = hipSuccess;
+    std::call_once(lds_once, [] {
+        lds_err = hipFuncSetAttribute((const void *)k_fft_pass_grouped,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)((4 * 256 + ((size_t)1 << FFT_TILE_BITS)) * sizeof(u64)));
+    });
+    GF2T_HIP(lds_err);
+    auto grouped_phase = [&](int ph) {
+        const u64 tot = total[ph];
+        unsigned blocks = (unsigned)std::min<u64>((tot + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_pairs_grouped, dim3(blocks), dim3(256), 0, st, dseg + ph * np, np,
+                           tot, ph, (u64)r);
+    };
+    for (size_t k = 0; k < steps.size(); k++) {
+        if (k == maxf) grouped_phase(0);   // between the forward and inverse passes
+        const Step &S = steps[k];
+        const size_t lds = (4 * 256 + ((size_t)1 << S.T)) * sizeof(u64);
+        const unsigned threads =
+            (unsigned)std::max<u64>(64, std::min<u64>(FFT_THREADS, 1ULL << (S.T - 1)));
+        hipLaunchKernelGGL(k_fft_pass_grouped, dim3(S.blocks), dim3(threads), lds, st,
+                           dgp + S.first, S.count);
+    }
+    if (steps.size() == maxf) grouped_phase(0);
+    grouped_phase(1);
+    grouped_phase(2);
+    GF2T_CHECK_LAUNCH("gf2t_mulmod_batch");
+    return GF2T_OK;
 }
 
 // ------------------------------------------------------------------ slabs

@@ -601,16 +601,25 @@ struct DevBuf {
     }
 };
 
-// ===================================================== degrees 33..34
+// ===================================================== degrees 33..36
 // Elements of GF(2^d) are u64.  The log table holds d-bit exponents, so
-// it is stored as the low 32 bits (L_lo, u32) plus the top two bits
-// packed sixteen to a word (L_hi).  It is built by the same radix
+// it is stored as the low 32 bits (L_lo, u32) plus the top four bits
+// packed eight to a word (L_hi).  It is built by the same radix
 // partition as above, except that the level-2 split runs one level-1
 // bucket at a time, to a bucket-sized scratch region and back (two
 // counting-sort passes), so the whole build needs 8 bytes per element
-// plus the table (201 GB at d = 34, inside 288 GB).  Entries are
+// plus the table (about 205 GB at d = 34, inside 288 GB).  Entries are
 // (g^i mod 2^(d-8)) << 36 | i.
-constexpr int WIDE_MAX_D = 34;
+// Above d = 34 the table is built and used in value slices: slice k holds
+// the logs of the field elements whose top byte lies in [k nb, (k+1) nb),
+// nb = 256 / slices.  Each slice's partition pass generates every power
+// but keeps only its own buckets, and the Zech pass of the slice resolves
+// only the classes whose lookup value falls in it; the marking queue
+// collects all slices' classes.  So the memory of d = 35 (two slices) and
+// d = 36 (four) is that of d = 34, and the extra cost is one generation
+// pass over the powers per slice.
+constexpr int WIDE_MAX_D = 36;
+constexpr int W_DENSE_D = 34;       // largest degree with the whole table at once
 constexpr int W_SB = 15;            // values per placed slice: 2^15
 constexpr int W_E = 8192;           // entries per block in passes 1 and 2
 constexpr int W_T = 256;
@@ -635,9 +644,13 @@ __device__ void block_scan(const unsigned *cnt, unsigned *off, unsigned nb, unsi
     __syncthreads();
 }
 
+// Pass 1 of the table build for the level-1 buckets [b0, b0 + nb): every
+// power g^i is generated, and those whose top byte is in range are
+// appended to bucket (top byte - b0) of out (2^(d-8) entries each).
 __global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *__restrict__ out,
                                                  unsigned *__restrict__ cursor,
-                                                 unsigned *__restrict__ bad) {
+                                                 unsigned *__restrict__ bad, unsigned b0,
+                                                 unsigned nb) {
     __shared__ unsigned cnt[256], off[256], gbase[256], part[W_T];
     __shared__ unsigned long long stage[W_E];
     __shared__ unsigned char sbkt[W_E];   // bucket of each staged entry
@@ -653,7 +666,8 @@ __global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         rank[k] = 0;
-        if (my0 + k < P.M) rank[k] = atomicAdd(&cnt[(unsigned)(a >> shift)], 1u);
+        const unsigned b = (unsigned)(a >> shift);
+        if (my0 + k < P.M && b - b0 < nb) rank[k] = atomicAdd(&cnt[b], 1u);
         a <<= 1;
         if (a >> P.d) a ^= P.q;
     }
@@ -663,8 +677,8 @@ __global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *
     a = a0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        if (my0 + k < P.M) {
-            const unsigned b = (unsigned)(a >> shift);
+        const unsigned b = (unsigned)(a >> shift);
+        if (my0 + k < P.M && b - b0 < nb) {
             const unsigned at = off[b] + rank[k];
             stage[at] = ((unsigned long long)(a & (cap - 1)) << W_IB) | (my0 + k);
             sbkt[at] = (unsigned char)b;
@@ -673,12 +687,12 @@ __global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *
         if (a >> P.d) a ^= P.q;
     }
     __syncthreads();
-    const u64 nvalid = (i0 + W_E <= P.M) ? W_E : P.M - i0;
+    const u64 nvalid = (u64)off[255] + cnt[255];   // entries staged in range
     for (u64 k = threadIdx.x; k < nvalid; k += W_T) {
         const unsigned b = sbkt[k];
         u64 pos = (u64)gbase[b] + (k - off[b]);
         if (pos >= cap) { atomicOr(bad, 4u); continue; }
-        out[(u64)b * cap + pos] = stage[k];
+        out[(u64)(b - b0) * cap + pos] = stage[k];
     }
 }
 
@@ -737,41 +751,47 @@ __global__ __launch_bounds__(WS_T) void k_w_split(const unsigned long long *__re
     }
 }
 
-// one block per slice z of 2^W_SB values: L_lo and the packed top bits
+// one block per run z of 2^W_SB values of the slice: L_lo and the
+// packed top bits.  zero_first: the slice starts at value 0, which has no
+// logarithm (its run holds 2^W_SB - 1 entries; L[0] is left 0)
 __global__ __launch_bounds__(1024) void k_w_place(const unsigned long long *__restrict__ in,
-                                                  u32 *__restrict__ Llo, u32 *__restrict__ Lhi) {
+                                                  u32 *__restrict__ Llo, u32 *__restrict__ Lhi,
+                                                  int zero_first) {
     extern __shared__ u32 wtile[];
-    u32 *lo = wtile, *hi = wtile + (1u << W_SB);          // hi: 2 bits per value
+    u32 *lo = wtile, *hi = wtile + (1u << W_SB);          // hi: 4 bits per value
     const u64 cap = (u64)1 << W_SB;
     const u64 z = blockIdx.x;
-    const u64 count = z ? cap : cap - 1;
+    const bool has_zero = zero_first && z == 0;
+    const u64 count = has_zero ? cap - 1 : cap;
     const unsigned long long *src = in + z * cap;
-    for (u64 k = threadIdx.x; k < cap / 16; k += blockDim.x) hi[k] = 0;
-    if (z == 0 && threadIdx.x == 0) lo[0] = 0;
+    for (u64 k = threadIdx.x; k < cap / 8; k += blockDim.x) hi[k] = 0;
+    if (has_zero && threadIdx.x == 0) lo[0] = 0;
     __syncthreads();
     for (u64 k = threadIdx.x; k < count; k += blockDim.x) {
         unsigned long long e = src[k];
         u32 v = (u32)(e >> W_IB) & (u32)(cap - 1);
         u64 i = e & (((u64)1 << W_IB) - 1);
         lo[v] = (u32)i;
-        atomicOr(&hi[v >> 4], (u32)((i >> 32) & 3) << (2 * (v & 15)));
+        atomicOr(&hi[v >> 3], (u32)((i >> 32) & 15) << (4 * (v & 7)));
     }
     __syncthreads();
     for (u64 k = threadIdx.x; k < cap; k += blockDim.x) Llo[z * cap + k] = lo[k];
-    for (u64 k = threadIdx.x; k < cap / 16; k += blockDim.x) Lhi[z * (cap / 16) + k] = hi[k];
+    for (u64 k = threadIdx.x; k < cap / 8; k += blockDim.x) Lhi[z * (cap / 8) + k] = hi[k];
 }
 
 struct MarkRecW {
     u64 j, n, sf;
 };
 
-// as k_zech: ZC_PER classes per thread, lookups first, one reservation per block
+// as k_zech: ZC_PER classes per thread, lookups first, one reservation
+// per block; only the classes whose lookup value y = beta^r + 1 lies in
+// the table slice [ybase, ybase + yspan) are resolved
 __global__ __launch_bounds__(ZC_T) void k_zech_w(WParams P, const u64 *__restrict__ surv,
                                                  u64 nsurv, const u32 *__restrict__ Llo,
                                                  const u32 *__restrict__ Lhi,
                                                  MarkRecW *__restrict__ queue,
                                                  unsigned long long *__restrict__ nqueue,
-                                                 u64 qcap) {
+                                                 u64 qcap, u64 ybase, u64 yspan) {
     __shared__ unsigned wtot[ZC_T / 64];
     __shared__ unsigned long long bbase;
     const u64 chunk = (u64)ZC_T * ZC_PER;
@@ -788,12 +808,13 @@ __global__ __launch_bounds__(ZC_T) void k_zech_w(WParams P, const u64 *__restric
         for (int q = 0; q < ZC_PER; q++) {
             const u64 u = mulmod64(j[q], P.rmod, P.M);
             live[q] = live[q] && u;                    // u = 0: beta^r = 1, never a root
-            y[q] = antilog_w(P, u ? u : 1) ^ 1u;
+            y[q] = (antilog_w(P, u ? u : 1) ^ 1u) - ybase;
+            live[q] = live[q] && y[q] < yspan;         // resolved by this slice's table
         }
 #pragma unroll
         for (int q = 0; q < ZC_PER; q++)
             v[q] = live[q] ? ((u64)Llo[y[q]] |
-                              ((u64)((Lhi[y[q] >> 4] >> (2 * (y[q] & 15))) & 3) << 32))
+                              ((u64)((Lhi[y[q] >> 3] >> (4 * (y[q] & 7))) & 15) << 32))
                            : 0;
         unsigned keep = 0;
         MarkRecW rec[ZC_PER];
@@ -1133,16 +1154,26 @@ int sieve_narrow(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     return status;
 }
 
-// degrees dmin..dmax in 33..34 (u64 field elements); g_sw_mu held
-// the tables of degrees up to dmax in 33..34
+// slices of the degree-d log table: the whole table up to W_DENSE_D, then
+// the fewest slices that keep a slice at 2^W_DENSE_D values; at least
+// g_min_slices (a test knob, gf2t_tune "sieve_slices")
+int g_min_slices = 1;
+int g_wide_from = 33;   // first degree of the 64-bit path (gf2t_tune "sieve_wide_from")
+int slices_for(int d) {
+    int n = d > W_DENSE_D ? 1 << (d - W_DENSE_D) : 1;
+    while (n < g_min_slices && n < 256) n <<= 1;
+    return n;
+}
+
+// the tables of the 64-bit path up to degree dmax (one slice's worth)
 int wide_mem(int dmax) {
     SieveWork &W = *g_sw;
     int rc;
-    const u64 N = (u64)1 << dmax;
+    const u64 N = (u64)1 << std::min(dmax, W_DENSE_D);          // values per slice, at most
     if ((rc = W.wP1.ensure(N * 8))) return rc;
-    if ((rc = W.wTmp.ensure((N >> W_B1) * 8))) return rc;
+    if ((rc = W.wTmp.ensure((((u64)1 << dmax) >> W_B1) * 8))) return rc;   // one level-1 bucket
     if ((rc = W.wLlo.ensure(N * 4))) return rc;
-    if ((rc = W.wLhi.ensure(N / 4))) return rc;
+    if ((rc = W.wLhi.ensure(N / 2))) return rc;
     if ((rc = W.wTab.ensure(2 * ((size_t)1 << W_AB) * 8))) return rc;
     if ((rc = W.wCur.ensure((256 + 2048) * 4))) return rc;
     if ((rc = W.bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
@@ -1157,7 +1188,7 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     if ((rc = wide_mem(dmax))) return rc;
     static std::once_flag lds_once;
     static hipError_t lds_err = hipSuccess;
-    const int place_lds = (int)((((size_t)1 << W_SB) + ((size_t)1 << W_SB) / 16) * sizeof(u32));
+    const int place_lds = (int)((((size_t)1 << W_SB) + ((size_t)1 << W_SB) / 8) * sizeof(u32));
     std::call_once(lds_once, [place_lds] {
         lds_err = hipFuncSetAttribute((const void *)k_w_place,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, place_lds);
@@ -1170,14 +1201,28 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
     u32 *Llo = (u32 *)W.wLlo.p, *Lhi = (u32 *)W.wLhi.p;
     u64 *A0 = (u64 *)W.wTab.p, *A1 = A0 + ((size_t)1 << W_AB);
     Heavy *heavy = (Heavy *)W.bHeavy.p;
-    SieveEvents evs;
-    hipEvent_t(&ev)[4] = evs.e;
+    constexpr int MAX_SL = 16;
+    hipEvent_t ev[4 + 2 * MAX_SL];
     for (auto &e : ev) GF2T_HIP(hipEventCreate(&e));
+    struct Destroy {
+        hipEvent_t *e;
+        int n;
+        ~Destroy() {
+            for (int i = 0; i < n; i++) hipEventDestroy(e[i]);
+        }
+    } destroy{ev, 4 + 2 * MAX_SL};
     int status = GF2T_OK;
     for (int d = dmin; d <= dmax && status == GF2T_OK; d++) {
         const u64 M = ((u64)1 << d) - 1;
         const WideHost &H = wide_host(d);
         const u64 q = H.q, rmod = r % M;
+        const int nsl = slices_for(d);
+        if (d - W_B1 - W_SB < 2 || nsl > MAX_SL || 256 % nsl) {
+            gf2t::set_error("gf2t_sieve: d=%d: the 64-bit path needs d >= 25 and at most %d "
+                            "slices", d, MAX_SL);
+            status = GF2T_EINVAL;
+            break;
+        }
         if (!q || gcd_host(rmod ? rmod : M, M) != 1) {
             gf2t::set_error("gf2t_sieve: d=%d: gcd(r, 2^d-1) != 1, unsupported r", d);
             status = GF2T_EINVAL;
@@ -1198,42 +1243,9 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         u64 *surv = (u64 *)W.wSurv.p;
         MarkRecW *queue = (MarkRecW *)W.wQueue.p;
 
+        // the classes (Lyndon words) first: every slice resolves its share
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
         GF2T_HIP(hipEventRecord(ev[0], 0));
-        // log table: partition pass 1, level-2 split per bucket, placement
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur1, (u64)256);
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
-        hipLaunchKernelGGL(k_w_part1, dim3((unsigned)((M + W_E - 1) / W_E)), dim3(W_T), 0, 0, P,
-                           P1, cur1, cnt32 + 1);
-        GF2T_CHECK_LAUNCH("k_w_part1");
-        // level 2 per level-1 bucket: bucket -> scratch by the high half of
-        // the level-2 bits, scratch -> bucket by the low half
-        const int B2 = d - W_B1 - W_SB, B2a = (B2 + 1) / 2, B2b = B2 - B2a;
-        const u64 cap1 = (u64)1 << (d - W_B1), capA = cap1 >> B2a;
-        for (unsigned b1 = 0; b1 < 256; b1++) {
-            const u64 count = b1 ? cap1 : cap1 - 1;   // value 0 never occurs
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur2, (u64)(1u << B2a));
-            hipLaunchKernelGGL(k_w_split, dim3((unsigned)((count + W_E - 1) / W_E), 1), dim3(WS_T),
-                               0, 0, P1 + (u64)b1 * cap1, cap1, count, W_SB + B2b, B2a, tmp, capA,
-                               cur2, cnt32 + 1);
-            hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((1u << B2) + 255) / 256)), dim3(256), 0,
-                               0, cur2, (u64)(1u << B2));
-            hipLaunchKernelGGL(k_w_split, dim3((unsigned)((capA + W_E - 1) / W_E), 1u << B2a),
-                               dim3(WS_T), 0, 0, tmp, capA, b1 ? capA : capA - 1, W_SB, B2b,
-                               P1 + (u64)b1 * cap1, (u64)1 << W_SB, cur2, cnt32 + 1);
-        }
-        GF2T_CHECK_LAUNCH("k_w_split");
-        hipLaunchKernelGGL(k_w_place, dim3((unsigned)((u64)1 << (d - W_SB))), dim3(1024),
-                           (size_t)place_lds, 0, P1, Llo, Lhi);
-        GF2T_CHECK_LAUNCH("k_w_place");
-        unsigned chk[2];
-        GF2T_HIP(hipMemcpy(chk, cnt32, 8, hipMemcpyDeviceToHost));
-        if (chk[1]) {
-            gf2t::set_error("gf2t_sieve: d=%d: log table partition overflow", d);
-            status = GF2T_EINTERNAL;
-            break;
-        }
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, 0, cnt64, cnt32);
-        GF2T_HIP(hipEventRecord(ev[1], 0));
         hipLaunchKernelGGL(k_lyndon_t<u64>, dim3(lyndon_blocks(d)), dim3(LY_T), 0, 0, d, M, surv,
                            cnt64, (u64)(ncls + 64));
         GF2T_CHECK_LAUNCH("k_lyndon_w");
@@ -1246,11 +1258,54 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             status = GF2T_EINTERNAL;
             break;
         }
+        GF2T_HIP(hipEventRecord(ev[1], 0));
+        const int nb = 256 / nsl;
+        const u64 cap1 = (u64)1 << (d - W_B1);                 // values per level-1 bucket
+        const int B2 = d - W_B1 - W_SB, B2a = (B2 + 1) / 2, B2b = B2 - B2a;
+        const u64 capA = cap1 >> B2a;
         const u64 zb = std::min<u64>((nsurv + (u64)ZC_T * ZC_PER - 1) / ((u64)ZC_T * ZC_PER),
                                      1u << 16);
-        hipLaunchKernelGGL(k_zech_w, dim3((unsigned)zb), dim3(ZC_T), 0, 0, P, surv, nsurv, Llo, Lhi,
-                           queue, cnt64 + 1, qcap);
-        GF2T_CHECK_LAUNCH("k_zech_w");
+        for (int sl = 0; sl < nsl && status == GF2T_OK; sl++) {
+            const unsigned b0 = (unsigned)(sl * nb);
+            GF2T_HIP(hipEventRecord(ev[4 + 2 * sl], 0));
+            // log table slice: partition pass 1 (this slice's buckets),
+            // level-2 split per bucket, placement
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur1, (u64)256);
+            hipLaunchKernelGGL(k_w_part1, dim3((unsigned)((M + W_E - 1) / W_E)), dim3(W_T), 0, 0,
+                               P, P1, cur1, cnt32 + 1, b0, (unsigned)nb);
+            GF2T_CHECK_LAUNCH("k_w_part1");
+            // level 2 per level-1 bucket: bucket -> scratch by the high half
+            // of the level-2 bits, scratch -> bucket by the low half
+            for (unsigned b1 = b0; b1 < b0 + (unsigned)nb; b1++) {
+                const u64 count = b1 ? cap1 : cap1 - 1;   // value 0 never occurs
+                unsigned long long *bk = P1 + (u64)(b1 - b0) * cap1;
+                hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, 0, cur2, (u64)(1u << B2a));
+                hipLaunchKernelGGL(k_w_split, dim3((unsigned)((count + W_E - 1) / W_E), 1),
+                                   dim3(WS_T), 0, 0, bk, cap1, count, W_SB + B2b, B2a, tmp, capA,
+                                   cur2, cnt32 + 1);
+                hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((1u << B2) + 255) / 256)),
+                                   dim3(256), 0, 0, cur2, (u64)(1u << B2));
+                hipLaunchKernelGGL(k_w_split, dim3((unsigned)((capA + W_E - 1) / W_E), 1u << B2a),
+                                   dim3(WS_T), 0, 0, tmp, capA, b1 ? capA : capA - 1, W_SB, B2b,
+                                   bk, (u64)1 << W_SB, cur2, cnt32 + 1);
+            }
+            GF2T_CHECK_LAUNCH("k_w_split");
+            hipLaunchKernelGGL(k_w_place, dim3((unsigned)(((u64)nb * cap1) >> W_SB)), dim3(1024),
+                               (size_t)place_lds, 0, P1, Llo, Lhi, sl == 0 ? 1 : 0);
+            GF2T_CHECK_LAUNCH("k_w_place");
+            GF2T_HIP(hipEventRecord(ev[5 + 2 * sl], 0));
+            unsigned chk[2];
+            GF2T_HIP(hipMemcpy(chk, cnt32, 8, hipMemcpyDeviceToHost));
+            if (chk[1]) {
+                gf2t::set_error("gf2t_sieve: d=%d: log table partition overflow", d);
+                status = GF2T_EINTERNAL;
+                break;
+            }
+            hipLaunchKernelGGL(k_zech_w, dim3((unsigned)zb), dim3(ZC_T), 0, 0, P, surv, nsurv, Llo,
+                               Lhi, queue, cnt64 + 1, qcap, (u64)b0 * cap1, (u64)nb * cap1);
+            GF2T_CHECK_LAUNCH("k_zech_w");
+        }
+        if (status != GF2T_OK) break;
         GF2T_HIP(hipMemcpy(hs, cnt64, 16, hipMemcpyDeviceToHost));
         if (hs[1] > qcap) {
             gf2t::set_error("gf2t_sieve: d=%d: marking queue overflow (%llu)", d,
@@ -1290,14 +1345,45 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
             S.classes = nsurv;
             S.marking = nq;
             S.heavy = nheavy;
-            GF2T_HIP(hipEventElapsedTime(&S.ms_table, ev[0], ev[1]));
-            GF2T_HIP(hipEventElapsedTime(&S.ms_classes, ev[1], ev[2]));
+            // table: the slices' builds; classes: the rest up to marking
+            float tab = 0, all = 0, x;
+            for (int sl = 0; sl < nsl; sl++) {
+                GF2T_HIP(hipEventElapsedTime(&x, ev[4 + 2 * sl], ev[5 + 2 * sl]));
+                tab += x;
+            }
+            GF2T_HIP(hipEventElapsedTime(&all, ev[0], ev[2]));
+            S.ms_table = tab;
+            S.ms_classes = all - tab;
             GF2T_HIP(hipEventElapsedTime(&S.ms_heavy, ev[2], ev[3]));
         }
     }
     return status;
 }
 }  // namespace
+
+// gf2t_tune keys of the sieve (test knobs): "sieve_wide_from" = first
+// degree of the 64-bit path (25..33, default 33), "sieve_slices" = least
+// number of log-table slices (1..16, a power of two)
+int gf2t_sieve_tune(const char *key, int64_t v) {
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    if (!strcmp(key, "sieve_wide_from") && v >= 25 && v <= 33) {
+        g_wide_from = (int)v;
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "sieve_slices") && v >= 1 && v <= 16 && !(v & (v - 1))) {
+        g_min_slices = (int)v;
+        return GF2T_OK;
+    }
+    gf2t::set_error("gf2t_tune: %s needs 25..33 (sieve_wide_from) or a power of two 1..16 "
+                    "(sieve_slices)", key);
+    return GF2T_EINVAL;
+}
+int gf2t_sieve_tune_get(const char *key, int64_t *v) {
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    if (!strcmp(key, "sieve_wide_from")) { *v = g_wide_from; return GF2T_OK; }
+    if (!strcmp(key, "sieve_slices")) { *v = g_min_slices; return GF2T_OK; }
+    return GF2T_EINVAL;
+}
 
 extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
                           gf2t_sieve_stats *stats) {
@@ -1312,15 +1398,16 @@ extern "C" int gf2t_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_
     std::lock_guard<std::mutex> lk(g_sw_mu);
     if (!g_sw) g_sw = new SieveWork();
     int status = GF2T_OK;
+    const int wf = g_wide_from;
     if (dmax >= 33) {
         // the degree-<=32 arrays live inside the wide tables from here on
         for (DevBuf *b : {&g_sw->bL, &g_sw->bP1, &g_sw->bP2, &g_sw->bSurv, &g_sw->bQueue})
             b->release();
         status = wide_mem(dmax);
     }
-    if (status == GF2T_OK && dmin <= 32)
-        status = sieve_narrow(r, dmin, std::min(dmax, 32), smax, dbest, stats);
-    if (status == GF2T_OK && dmax >= 33)
-        status = sieve_wide(r, std::max(dmin, 33), dmax, smax, dbest, stats);
+    if (status == GF2T_OK && dmin < wf)
+        status = sieve_narrow(r, dmin, std::min(dmax, wf - 1), smax, dbest, stats);
+    if (status == GF2T_OK && dmax >= wf)
+        status = sieve_wide(r, std::max(dmin, wf), dmax, smax, dbest, stats);
     return status;
 }

@@ -448,6 +448,8 @@ bool square_fuse_enabled() { return sq_fuse(); }
 
 int gf2t_fft_set_tile_bits(int m, int64_t T);   // fft.hip
 int gf2t_fft_tile_bits(int m);
+int gf2t_sieve_tune(const char *key, int64_t v);   // sieve.hip
+int gf2t_sieve_tune_get(const char *key, int64_t *v);
 
 extern "C" int gf2t_tune(const char *key, int64_t value) {
     if (!key) return GF2T_EINVAL;
@@ -456,6 +458,8 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
         long m = strtol(key + 9, &end, 10);
         if (*end == 0) return gf2t_fft_set_tile_bits((int)m, value);
     }
+    if (!strcmp(key, "sieve_wide_from") || !strcmp(key, "sieve_slices"))
+        return gf2t_sieve_tune(key, value);
     if (!strncmp(key, "block_m_", 8) && key[8]) {
         char *end = nullptr;
         long k = strtol(key + 8, &end, 10);
@@ -487,6 +491,8 @@ extern "C" int gf2t_tune_get(const char *key, int64_t *value) {
         *value = (int64_t)gf2t_host::device_product_bits();
         return GF2T_OK;
     }
+    if (!strcmp(key, "sieve_wide_from") || !strcmp(key, "sieve_slices"))
+        return gf2t_sieve_tune_get(key, value);
     if (!strncmp(key, "fft_tile_", 9) && key[9]) {
         char *end = nullptr;
         long m = strtol(key + 9, &end, 10);

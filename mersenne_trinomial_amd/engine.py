@@ -274,7 +274,7 @@ def run_gpu(r: int, depth: int, smax: int, best: np.ndarray | None = None,
         if best.size != smax + 1:
             raise ValueError("best must have smax+1 entries")
     dbest = DeviceBuffer.from_array(best)
-    stats = (SieveStats * 35)()
+    stats = (SieveStats * 37)()
     rc = lib.gf2t_sieve(r, int(min_depth), int(depth), int(smax), dbest.ptr, stats)
     check(rc, "gf2t_sieve")
     out = dbest.download(smax + 1)

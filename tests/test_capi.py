@@ -44,7 +44,7 @@ def test_argument_errors_return_status_not_exit():
     assert lib.gf2t_square_fold(8, 16, 31, 0, None) == -1       # s = 0
     assert lib.gf2t_fold(8, 100, 16, 31, 3, None) == -1         # src_bits > 2r-1
     assert lib.gf2t_fft_plan_create(0, C.byref(C.c_void_p())) == -1
-    assert lib.gf2t_sieve(101, 2, 35, 10, 8, None) == -1
+    assert lib.gf2t_sieve(101, 2, 37, 10, 8, None) == -1
 
 
 def test_engine_basis_verify():

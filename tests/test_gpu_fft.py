@@ -131,6 +131,6 @@ def test_every_tile_size_bit_identical(eng, m):
             assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f), T
             assert np.array_equal(eng.mul(a, bits, b, 29), pwant), T
     finally:
-        assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 0) == 0
+        assert lib.gf2t_tune(f"fft_tile_{m}".encode(), old.value) == 0
     assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 5) != 0
     assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 14) != 0

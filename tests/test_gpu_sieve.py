@@ -136,3 +136,57 @@ def test_wide_degrees_33_34_full_range_match_reference_log(eng):
             assert O.divides_trinomial_small(R136, int(s), mask, d)
     # the d<=32 part is untouched
     assert np.array_equal(best[base != O.NONE], base[base != O.NONE])
+
+
+@pytest.mark.parametrize("slices", [1, 2, 4, 8])
+def test_sliced_log_table_equals_dense(eng, slices):
+    """The 64-bit path with the log table built and used in value slices
+    (how degrees 35 and 36 run) gives the same marks as the dense 32-bit
+    path: degrees 25..27 at r = 136279841 forced through it with 1..8
+    slices."""
+    from mersenne_trinomial_amd._capi import lib
+    smax = 1 << 21
+    want, _ = eng.run_gpu(R136, 27, smax)
+    try:
+        assert lib.gf2t_tune(b"sieve_wide_from", 25) == 0
+        assert lib.gf2t_tune(b"sieve_slices", slices) == 0
+        print(f"sliced sieve: {slices} slices", flush=True)
+        got, st = eng.run_gpu(R136, 27, smax)
+    finally:
+        assert lib.gf2t_tune(b"sieve_wide_from", 33) == 0
+        assert lib.gf2t_tune(b"sieve_slices", 1) == 0
+    assert np.array_equal(got, want)
+    assert all(s.classes == O.count_irreducible(s.d) for s in st)
+    assert lib.gf2t_tune(b"sieve_slices", 3) != 0
+
+
+def test_wide_degrees_35_36_full_range_match_reference_log(eng):
+    """Degrees 35 and 36 over every s <= 68139920 at r=136279841 (the log
+    table in 2 and 4 slices), continuing from the d<=34 image: the
+    reference's low-memory runs leave 3,415,883 survivors at d<=35 (the
+    line count of its out35.survivors.txt) and 3,322,198 at d<=36, with
+    93,685 new degree-36 certificates, 1,908,866,960 degree-36 classes
+    and 2 heavy progressions (src/coarse_sieve.log:103-175).  Sampled new
+    marks are genuine irreducible factors of their trinomials."""
+    smax = R136 // 2
+    base, _ = eng.run_gpu(R136, 34, smax)
+    print("degrees 35..36", flush=True)
+    best, st = eng.run_gpu(R136, 36, smax, best=base, min_depth=35)
+    deg = best[1:] >> np.uint64(48)
+    marked = best[1:] != O.NONE
+    assert int(np.count_nonzero(~(marked & (deg <= 35)))) == 3415883
+    assert int(np.count_nonzero(~marked)) == 3322198
+    assert int(np.count_nonzero(marked & (deg == 36))) == 93685
+    got = {s.d: (s.classes, s.heavy) for s in st}
+    assert got[36] == (1908866960, 2)
+    assert got[35][0] == O.count_irreducible(35)
+    print({s.d: (round(s.ms_table, 1), round(s.ms_classes, 1)) for s in st}, flush=True)
+    rng = np.random.default_rng(9)
+    for d in (35, 36):
+        idx = np.nonzero(marked & (deg == d))[0] + 1
+        for s in rng.choice(idx, size=40, replace=False):
+            k = int(best[s])
+            mask = k & 0xFFFFFFFFFFFF
+            assert O.is_irreducible_small(mask, d)
+            assert O.divides_trinomial_small(R136, int(s), mask, d)
+    assert np.array_equal(best[base != O.NONE], base[base != O.NONE])

@@ -214,12 +214,15 @@ def test_results_file_long_lines(tmp_path):
     res.write_text(f"474 16001 p{hexd}\n505 u\n")
     p = subprocess.run([_ckpt_check(tmp_path), "done", str(res)], capture_output=True, text=True)
     assert p.stdout.split() == ["done", "474", "505"], p.stdout[:200]
-    # the command-line drivers report the same count before touching the GPU
+    # the command-line drivers report the same count before they start
+    # scanning (each on its own copy of the file: with a GPU present the
+    # run then goes on and appends the line of s = 3)
     (tmp_path / "surv.txt").write_text("3\n474\n505\n")
+    (tmp_path / "w2.results.txt").write_text(res.read_text())
     p = run(R, tmp_path / "surv.txt", "--skip", 8, "--out", tmp_path / "w", check=False)
     assert "2 already completed" in p.stderr, p.stderr
     p = subprocess.run([os.path.join(BIN, "trinomial_scan"), str(R), str(tmp_path / "surv.txt"),
-                        "8", "100", str(tmp_path / "w")], capture_output=True, text=True,
+                        "8", "100", str(tmp_path / "w2")], capture_output=True, text=True,
                        timeout=120)
     assert "3 survivors (2 already done)" in p.stderr, p.stderr
 
