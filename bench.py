@@ -99,7 +99,7 @@ def parse():
                     help="gcd mode: gcds run at once after the single one")
     ap.add_argument("--mode",
                     choices=["full", "squaring", "fft", "scan", "pipeline", "serve", "sieve",
-                             "c3", "gcd"],
+                             "c3", "gcd", "fft24"],
                     default="full",
                     help="full: the default bench line; squaring: a plain copy of known size "
                          "then only the batched squaring chain (for rocprofv3 counter runs); "
@@ -472,6 +472,29 @@ def mode_fft(args):
           flush=True)
 
 
+def mode_fft24(args):
+    """Only 2^24-point transforms (forward then inverse, --steps times each)
+    on one buffer, for rocprofv3 counter runs of k_fft_pass at the size of
+    the full 136279841-bit product."""
+    from mersenne_trinomial_amd import device
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+    device.init(0)
+    plan = C.c_void_p()
+    check(lib.gf2t_fft_plan_create(24, C.byref(plan)))
+    f = DeviceBuffer.words(1 << 24)
+    f.fill(0x3C)
+    for _ in range(args.steps):
+        check(lib.gf2t_fft_fwd(plan, f.ptr, None))
+        check(lib.gf2t_fft_inv(plan, f.ptr, None))
+    device.synchronize()
+    fp, ip = C.c_int(), C.c_int()
+    check(lib.gf2t_fft_plan_passes(plan, C.byref(fp), C.byref(ip)))
+    lib.gf2t_fft_plan_destroy(plan)
+    print(json.dumps({"mode": "fft24", "transforms": 2 * args.steps, "fwd_passes": fp.value,
+                      "inv_passes": ip.value}), flush=True)
+
+
 def mode_scan(args):
     """Deep scan of the given s values (one cohort) once per block size,
     with the scan's own phase times (device, gcd, resolution) on stderr."""
@@ -755,6 +778,8 @@ def main():
         return mode_sieve(args)
     if args.mode == "gcd":
         return mode_gcd(args)
+    if args.mode == "fft24":
+        return mode_fft24(args)
     if args.mode == "c3":
         from mersenne_trinomial_amd import device
         device.init(0)

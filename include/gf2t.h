@@ -129,7 +129,7 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  *              deep scan) go to the device when both operands have at
  *              least this many bits; 0 = host Karatsuba only
  * Environment defaults: GF2T_SQ_FUSE (1), GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0),
- * GF2T_SQ_NT (1), GF2T_GCD_DEV_BITS (2^18, set by gf2t_init; 0 before it).
+ * GF2T_SQ_NT (1), GF2T_GCD_DEV_BITS (2^16, set by gf2t_init; 0 before it).
  * GF2T_NO_VPCLMUL=1 keeps the host gcds' schoolbook products on 128-bit
  * PCLMULQDQ where the CPU also has AVX-512 VPCLMULQDQ (read once). */
 int gf2t_tune(const char *key, int64_t value);

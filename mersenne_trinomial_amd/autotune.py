@@ -14,6 +14,8 @@ its block size the same way, src/tsfactor.cu:1986-1989 and auto_m
   two-operand products back to back on one stream.
 - the batched squaring knobs sq_fuse / sq_nt / sq_order / sq_ppt, timed
   on the deep scan's residue shape at the given r.
+- gcd_dev_bits: the product size from which the half-gcd uses the device,
+  timed on concurrent gcds.
 - block_m_<k>: the deep scan's degrees per interval product for
   2^k <= r < 2^(k+1), from the measured per-degree cost
   m * t_sq + t_mul / m of the blocked scheme (t_sq and t_mul: a
@@ -168,7 +170,52 @@ def tune_block_m(r: int, reps: int = 3):
             "chosen": best, "ms_per_degree": {m: round(v, 4) for m, v in cost.items()}}
 
 
-def write(path: str, fft: dict, sq: dict, blocks: list, device: str) -> None:
+def tune_gcd(r: int, conc: int, thresholds):
+    """gcd_dev_bits: the size from which the half-gcd's products go to the
+    device, timed on `conc` concurrent gcds of random r-bit residues
+    against x^r + x^3 + 1 (the deep scan runs its interval gcds
+    concurrently)."""
+    import threading
+    nw = (r + 63) // 64
+    t = np.zeros((r + 64) // 64, dtype=np.uint64)
+    for e in (0, 3, r):
+        t[e // 64] |= np.uint64(1 << (e % 64))
+    rng = np.random.default_rng(3)
+    As = []
+    for _ in range(conc):
+        a = rng.integers(0, 2**64, nw, dtype=np.uint64)
+        if r % 64:
+            a[-1] &= np.uint64((1 << (r % 64)) - 1)
+        As.append(a)
+
+    def run_all():
+        def one(a):
+            g = np.zeros(4, dtype=np.uint64)
+            lib.gf2t_gcd_host(t.ctypes.data_as(P64), t.size, a.ctypes.data_as(P64), a.size,
+                              g.ctypes.data_as(P64), 4)
+        ths = [threading.Thread(target=one, args=(a,)) for a in As]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        return time.perf_counter() - t0
+
+    default = tune_get("gcd_dev_bits")
+    times = {}
+    for b in thresholds:
+        tune_set("gcd_dev_bits", b)
+        run_all()                                   # plans and buffers for this size mix
+        times[b] = run_all()
+    best = min(times, key=times.get)
+    tune_set("gcd_dev_bits", best)
+    print(f"gcd_dev_bits: {best} ({times[best]:.3f} s for {conc} gcds at r={r}; default "
+          f"{default} {times.get(default, float('nan')):.3f} s)", file=sys.stderr, flush=True)
+    return {"r": r, "concurrent": conc, "default": default, "chosen": best,
+            "s": {b: round(v, 3) for b, v in times.items()}}
+
+
+def write(path: str, fft: dict, sq: dict, blocks: list, device: str, gcd: dict | None) -> None:
     lines = [f"# gf2t autotuner ({device.strip()}), {time.strftime('%Y-%m-%d %H:%M:%S')}",
              "# key value -- applied by gf2t_init; see mersenne_trinomial_amd/autotune.py"]
     for m, v in sorted(fft.items()):
@@ -177,6 +224,8 @@ def write(path: str, fft: dict, sq: dict, blocks: list, device: str) -> None:
         lines.append(f"{key} {v['chosen']}")
     for b in blocks:
         lines.append(f"block_m_{b['k']} {b['chosen']}")
+    if gcd:
+        lines.append(f"gcd_dev_bits {gcd['chosen']}")
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -190,6 +239,8 @@ def main(argv=None):
     ap.add_argument("--sq-r", type=int, default=136279841)
     ap.add_argument("--sq-residues", type=int, default=64)
     ap.add_argument("--block-r", type=str, default="132049,756839,3021377,136279841")
+    ap.add_argument("--gcd-r", type=int, default=(1 << 24) + 1)
+    ap.add_argument("--gcd-conc", type=int, default=8, help="0 skips the gcd threshold")
     args = ap.parse_args(argv)
     os.environ["GF2T_TUNE_FILE"] = ""          # measure from the built-in defaults
     init(0)
@@ -200,10 +251,12 @@ def main(argv=None):
     fft = tune_fft(ms, range(6, 14), reps=3 if args.quick else 5)
     sq = tune_squaring(args.sq_r, args.sq_residues, 8 if args.quick else 16)
     blocks = [tune_block_m(int(r)) for r in args.block_r.split(",") if r]
-    write(args.out, fft, sq, blocks, dev)
-    print(json.dumps({"mode": "autotune", "device": dev, "file": args.out,
+    gcd = tune_gcd(args.gcd_r, args.gcd_conc,
+                   [1 << k for k in range(14, 19)]) if args.gcd_conc > 0 else None
+    write(args.out, fft, sq, blocks, dev, gcd)
+    print(json.dumps({"mode": "autotune", "device": dev.strip(), "file": args.out,
                       "seconds": round(time.perf_counter() - t0, 1), "fft_tile": fft,
-                      "squaring": sq, "block_m": blocks}), flush=True)
+                      "squaring": sq, "block_m": blocks, "gcd": gcd}), flush=True)
 
 
 if __name__ == "__main__":

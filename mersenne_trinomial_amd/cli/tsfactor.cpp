@@ -63,7 +63,7 @@ static void usage() {
             "  --gcd-threads N host threads for the interval gcds\n"
             "  --device N      GPU index\n"
             "  --gpu-gcd-min-bits N   gcd products with both operands >= N bits\n"
-            "                  run on the GPU (default 2^18)\n"
+            "                  run on the GPU (default 2^16)\n"
             "  --legacy-sq     one squaring per launch\n"
             "  --batch B       s values per device cohort (default: by memory)\n"
             "  --ckpt-mins N   checkpoint cadence (default 15); --no-ckpt: none\n"

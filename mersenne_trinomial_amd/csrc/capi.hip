@@ -89,6 +89,10 @@ extern "C" int gf2t_init(int device) {
         return GF2T_EINVAL;
     }
     GF2T_HIP(hipSetDevice(device));
+    // GF2T_SYNC=block: host threads waiting for the device sleep instead of
+    // spinning (for many gcd threads sharing the host's cores)
+    if (const char *e = getenv("GF2T_SYNC"))
+        if (!strcmp(e, "block")) (void)hipSetDeviceFlags(hipDeviceScheduleBlockingSync);
     GF2T_HIP(hipFree(nullptr));   // create the context now
     int rc = gf2t_noalloc_setup();
     if (rc) return rc;
@@ -96,7 +100,7 @@ extern "C" int gf2t_init(int device) {
     // with a device present, large gcd products go to the device
     if (!gf2t_host::device_product_bits()) {
         const char *e = getenv("GF2T_GCD_DEV_BITS");
-        gf2t_host::set_device_product_bits(e ? strtoull(e, nullptr, 10) : (1ULL << 18));
+        gf2t_host::set_device_product_bits(e ? strtoull(e, nullptr, 10) : (1ULL << 16));
     }
     return GF2T_OK;
 }

@@ -51,6 +51,7 @@ struct PassDesc {
     int m;        // transform size exponent
     int c, a, w;  // local bits: [0,c) and [a, a+w); tile = 2^(c+w)
     int nops;
+    int nbf;      // butterfly ops among them (0: no twiddle tables needed)
     uint8_t kind[FFT_MAX_OPS];
     uint8_t lbit[FFT_MAX_OPS];   // local bit of the op (low bit of a Taylor pair)
     uint8_t gbit[FFT_MAX_OPS];   // global bit (butterfly depth L)
@@ -93,10 +94,11 @@ GF2T_DEV u64 twiddle(const TwTables &tw, u64 t) {
     return w;
 }
 
-GF2T_DEV u64 global_index(const PassDesc &pd, u64 li, u64 rest_lo, u64 rest_hi) {
-    u64 cm = (1ULL << pd.c) - 1, wm = (1ULL << pd.w) - 1;
-    return (li & cm) | (rest_lo << pd.c) | (((li >> pd.c) & wm) << pd.a) |
-           (rest_hi << (pd.a + pd.w));
+// the global index of tile element li (li < 2^(c+w)): its low c bits stay,
+// its bits [c, c+w) go to [a, a+w), and gbase holds the tile's own bits
+GF2T_DEV u64 global_index(const PassDesc &pd, u32 li, u64 gbase) {
+    const u32 cm = (1u << pd.c) - 1;
+    return gbase | (u64)(li & cm) | ((u64)(li >> pd.c) << pd.a);
 }
 
 // pack: element k of the transform input = 32-bit chunk k of the bit
@@ -119,13 +121,15 @@ GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
     TwTables &tw = *reinterpret_cast<TwTables *>(lds);
     u64 *tile = lds + 4 * 256;
     const int T = pd.c + pd.w;
-    const u64 tsize = 1ULL << T;
+    const u32 tsize = 1u << T;
     const unsigned nthr = blockDim.x;
     const int nlo = pd.a - pd.c;
-    const u64 rest_lo = blk & ((1ULL << nlo) - 1);
-    const u64 rest_hi = blk >> nlo;
+    // the tile's fixed index bits: blk's low nlo bits at [c, a), the rest
+    // from a + w up
+    const u64 gbase = ((blk & ((1ULL << nlo) - 1)) << pd.c) | ((blk >> nlo) << (pd.a + pd.w));
 
     // twiddle byte tables: T_k[v] = sum_{bits j of v} b[8k + j + 1]
+    if (pd.nbf == 0) ntab = 0;
     for (int e = threadIdx.x; e < 256 * ntab; e += nthr) {
         int k = e >> 8, v = e & 255;
         u64 acc = 0;
@@ -137,8 +141,8 @@ GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
         tw.t[k][v] = acc;
     }
     // load the tile (low c bits contiguous in memory)
-    for (u64 li = threadIdx.x; li < tsize; li += nthr) {
-        u64 g = global_index(pd, li, rest_lo, rest_hi);
+    for (u32 li = threadIdx.x; li < tsize; li += nthr) {
+        const u64 g = global_index(pd, li, gbase);
         u64 v;
         if (pack) {
             v = packed_chunk(psrc, pn, pmask, g);
@@ -153,11 +157,11 @@ GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
     for (int o = 0; o < pd.nops; o++) {
         const int kind = pd.kind[o];
         const int l = pd.lbit[o];
-        const u64 lo_mask = (1ULL << l) - 1;
+        const u32 lo_mask = (1u << l) - 1;
         if (kind == OP_TF || kind == OP_TI) {
-            const u64 st = 1ULL << l;
-            for (u64 qd = threadIdx.x; qd < (tsize >> 2); qd += nthr) {
-                u64 base = (qd & lo_mask) | ((qd >> l) << (l + 2));
+            const u32 st = 1u << l;
+            for (u32 qd = threadIdx.x; qd < (tsize >> 2); qd += nthr) {
+                const u32 base = (qd & lo_mask) | ((qd >> l) << (l + 2));
                 u64 x01 = tile[base + st], x10 = tile[base + 2 * st], x11 = tile[base + 3 * st];
                 if (kind == OP_TF) { x10 ^= x11; x01 ^= x10; }
                 else { x01 ^= x10; x10 ^= x11; }
@@ -166,10 +170,10 @@ GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
             }
         } else {
             const int L = pd.gbit[o];
-            const u64 st = 1ULL << l;
-            for (u64 pr = threadIdx.x; pr < (tsize >> 1); pr += nthr) {
-                u64 i1 = (pr & lo_mask) | ((pr >> l) << (l + 1));
-                u64 g1 = global_index(pd, i1, rest_lo, rest_hi);
+            const u32 st = 1u << l;
+            for (u32 pr = threadIdx.x; pr < (tsize >> 1); pr += nthr) {
+                const u32 i1 = (pr & lo_mask) | ((pr >> l) << (l + 1));
+                const u64 g1 = global_index(pd, i1, gbase);
                 u64 w = twiddle(tw, g1 >> (L + 1));
                 u64 e1 = tile[i1], e2 = tile[i1 + st];
                 if (kind == OP_BF) {
@@ -185,8 +189,8 @@ GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
         }
         __syncthreads();
     }
-    for (u64 li = threadIdx.x; li < tsize; li += nthr)
-        f[global_index(pd, li, rest_lo, rest_hi)] = tile[li];
+    for (u32 li = threadIdx.x; li < tsize; li += nthr)
+        f[global_index(pd, li, gbase)] = tile[li];
 }
 
 // blockIdx.y selects one transform of a batch: element y lives at
@@ -340,7 +344,9 @@ std::vector<PassDesc> schedule(int m, bool forward, int tile_bits) {
         pd.a = bestW.a;
         pd.w = bestW.w;
         pd.nops = (int)best_take.size();
+        pd.nbf = 0;
         for (int i = 0; i < pd.nops; i++) {
+            pd.nbf += best_take[i].kind >= OP_BF;
             pd.kind[i] = best_take[i].kind;
             pd.lbit[i] = (uint8_t)bestW.loc(best_take[i].bit);
             pd.gbit[i] = best_take[i].bit;

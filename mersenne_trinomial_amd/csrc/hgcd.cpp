@@ -21,8 +21,12 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <future>
+#include <mutex>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -34,31 +38,63 @@ namespace {
 
 std::atomic<u64> g_dev_bits{0};   // 0: host products only
 
-// Helper threads for independent products inside one gcd: a process-wide
-// budget of (host threads - 1), taken without blocking; a product that
-// finds the budget empty runs on the calling thread.
-std::atomic<int> g_helpers{-1};
-bool take_helper() {
-    int cur = g_helpers.load();
-    if (cur < 0) {
-        int n = host_threads() - 1;
-        g_helpers.compare_exchange_strong(cur, n > 0 ? n : 0);
-        cur = g_helpers.load();
-    }
-    while (cur > 0)
-        if (g_helpers.compare_exchange_weak(cur, cur - 1)) return true;
-    return false;
+// Helper threads for independent products inside one gcd: a pool of
+// persistent workers (started on demand, at most host threads - 1).  A
+// task goes to the pool only while the threads computing (callers inside
+// gcd_fast plus helpers with a task) leave a hardware thread free; else it
+// runs on the calling thread.  So one gcd alone spreads over the host,
+// while many concurrent gcds (one per thread) run without helpers and
+// without oversubscribing the cores.  Every unfinished task counts in
+// `busy` and there are at least `busy` workers, so a task waiting on a
+// nested task always has a worker for it.
+struct HelperPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::packaged_task<void()>> q;
+    int workers = 0;
+    std::atomic<int> busy{0};
+};
+HelperPool &helper_pool() {
+    static HelperPool *p = new HelperPool;   // never destroyed: workers outlive main()
+    return *p;
 }
-void give_helper() { g_helpers.fetch_add(1); }
+std::atomic<int> g_gcd_callers{0};   // threads inside gcd_fast
 
-// run f on a helper thread when one is free, else now
+void helper_loop(HelperPool *P) {
+    for (;;) {
+        std::packaged_task<void()> task;
+        {
+            std::unique_lock<std::mutex> lk(P->mu);
+            P->cv.wait(lk, [&] { return !P->q.empty(); });
+            task = std::move(P->q.front());
+            P->q.pop_front();
+        }
+        task();
+        P->busy.fetch_sub(1);
+    }
+}
+
+// run f on a helper when the budget allows, else now
 template <class F>
 std::future<void> spawn(F &&f) {
-    if (take_helper())
-        return std::async(std::launch::async, [f = std::forward<F>(f)]() mutable {
-            struct Give { ~Give() { give_helper(); } } g;
-            f();
-        });
+    HelperPool &P = helper_pool();
+    const int cap = host_threads();
+    int b = P.busy.load();
+    while (g_gcd_callers.load(std::memory_order_relaxed) + b < cap) {
+        if (!P.busy.compare_exchange_weak(b, b + 1)) continue;
+        std::packaged_task<void()> task(std::forward<F>(f));
+        std::future<void> fut = task.get_future();
+        {
+            std::lock_guard<std::mutex> lk(P.mu);
+            P.q.push_back(std::move(task));
+            if (P.workers < P.busy.load()) {
+                std::thread(helper_loop, &P).detach();
+                P.workers++;
+            }
+        }
+        P.cv.notify_one();
+        return fut;
+    }
     f();
     std::promise<void> p;
     p.set_value();
@@ -108,40 +144,55 @@ void school128(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
     out[nk] ^= carry;
 }
 
-// The same with 512-bit VPCLMULQDQ (four 64x64 products per instruction),
-// for nb <= 64: b is reversed once (rb[t] = b[nb-1-t]), so column k is the
-// lane-aligned sum of a[i0 + u] rb[t0 + u], u < len, eight words per step
-// (masked loads zero the tail).
+// out[0, na+nb) ^= a * b with 512-bit VPCLMULQDQ, by output blocks of
+// eight words: for block k and each word b[j] that reaches it, the eight
+// words a[k-j .. k-j+8) (from a copy of a with eight zero words on each
+// side) are multiplied by b[j] in two instructions, the even words' 128-bit
+// products landing on output words (k+2l, k+2l+1) and the odd words' one
+// word higher; the two accumulators are combined once per block, the odd
+// one shifted up by a word with its top word carried into the next block.
 __attribute__((target("avx512f,vpclmulqdq")))
-void school512(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
-    u64 rb[64];
-    for (size_t t = 0; t < nb; t++) rb[t] = b[nb - 1 - t];
-    const size_t nk = na + nb - 1;
-    u64 carry = 0;
-    for (size_t k = 0; k < nk; k++) {
-        const size_t i0 = k >= nb ? k - nb + 1 : 0;
-        const size_t i1 = k < na ? k : na - 1;
-        const size_t len = i1 - i0 + 1, t0 = nb - 1 - (k - i0);
-        const u64 *pa = a + i0, *pr = rb + t0;
-        __m512i s = _mm512_setzero_si512();
-        size_t u = 0;
-        for (; u + 8 <= len; u += 8) {
-            const __m512i A = _mm512_loadu_si512(pa + u), R = _mm512_loadu_si512(pr + u);
-            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x00));
-            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x11));
+void rows512(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
+    thread_local std::vector<u64> pad;   // grows to the largest na seen
+    if (pad.size() < na + 24) pad.resize(na + 24);
+    u64 *const ap = pad.data();
+    std::memset(ap, 0, 8 * 8);
+    std::memcpy(ap + 8, a, na * 8);
+    std::memset(ap + 8 + na, 0, 16 * 8);
+    const size_t no = na + nb;          // output words
+    __m512i prevO = _mm512_setzero_si512();
+    for (size_t k = 0; k < no; k += 8) {
+        const size_t jlo = k + 1 > na ? k + 1 - na : 0;
+        const size_t jhi = std::min(nb, k + 8);          // exclusive
+        __m512i E0 = _mm512_setzero_si512(), O0 = _mm512_setzero_si512();
+        __m512i E1 = _mm512_setzero_si512(), O1 = _mm512_setzero_si512();
+        size_t j = jlo;
+        for (; j + 2 <= jhi; j += 2) {
+            const __m512i A0 = _mm512_loadu_si512(ap + 8 + k - j);
+            const __m512i B0 = _mm512_set1_epi64((long long)b[j]);
+            const __m512i A1 = _mm512_loadu_si512(ap + 7 + k - j);
+            const __m512i B1 = _mm512_set1_epi64((long long)b[j + 1]);
+            E0 = _mm512_xor_si512(E0, _mm512_clmulepi64_epi128(A0, B0, 0x00));
+            O0 = _mm512_xor_si512(O0, _mm512_clmulepi64_epi128(A0, B0, 0x01));
+            E1 = _mm512_xor_si512(E1, _mm512_clmulepi64_epi128(A1, B1, 0x00));
+            O1 = _mm512_xor_si512(O1, _mm512_clmulepi64_epi128(A1, B1, 0x01));
         }
-        if (u < len) {
-            const __mmask8 m = (__mmask8)((1u << (len - u)) - 1);
-            const __m512i A = _mm512_maskz_loadu_epi64(m, pa + u), R = _mm512_maskz_loadu_epi64(m, pr + u);
-            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x00));
-            s = _mm512_xor_si512(s, _mm512_clmulepi64_epi128(A, R, 0x11));
+        if (j < jhi) {
+            const __m512i A0 = _mm512_loadu_si512(ap + 8 + k - j);
+            const __m512i B0 = _mm512_set1_epi64((long long)b[j]);
+            E0 = _mm512_xor_si512(E0, _mm512_clmulepi64_epi128(A0, B0, 0x00));
+            O0 = _mm512_xor_si512(O0, _mm512_clmulepi64_epi128(A0, B0, 0x01));
         }
-        const __m256i h = _mm256_xor_si256(_mm512_castsi512_si256(s), _mm512_extracti64x4_epi64(s, 1));
-        const __m128i q = _mm_xor_si128(_mm256_castsi256_si128(h), _mm256_extracti128_si256(h, 1));
-        out[k] ^= (u64)_mm_cvtsi128_si64(q) ^ carry;
-        carry = (u64)_mm_extract_epi64(q, 1);
+        const __m512i E = _mm512_xor_si512(E0, E1), O = _mm512_xor_si512(O0, O1);
+        const __m512i r = _mm512_xor_si512(E, _mm512_alignr_epi64(O, prevO, 7));
+        prevO = O;
+        if (k + 8 <= no) {
+            _mm512_storeu_si512(out + k, _mm512_xor_si512(_mm512_loadu_si512(out + k), r));
+        } else {
+            const __mmask8 m = (__mmask8)((1u << (no - k)) - 1);
+            _mm512_mask_storeu_epi64(out + k, m, _mm512_xor_si512(_mm512_maskz_loadu_epi64(m, out + k), r));
+        }
     }
-    out[nk] ^= carry;
 }
 
 // The 512-bit form where the CPU has it (AVX-512F and VPCLMULQDQ), unless
@@ -162,10 +213,48 @@ void school(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out) {
         std::swap(a, b);
         std::swap(na, nb);
     }
-    if (nb <= 64 && use512())
-        school512(a, na, b, nb, out);
+    if (use512())
+        rows512(a, na, b, nb, out);
     else
         school128(a, na, b, nb, out);
+}
+
+// Karatsuba's recombination: with z0 = a0 b0 in out[0, 2h), z2 = a1 b1 in
+// out[2h, 2h+2l) and z1 = (a0+a1)(b0+b1), out[h, 3h) ^= z1 + z0 + z2.
+// Step i reads the four words i, h+i, 2h+i, 3h+i of out and z1[i],
+// z1[h+i], and writes out[h+i] and out[2h+i], which no later step reads.
+__attribute__((target("avx512f")))
+void kara_mix512(u64 *out, const u64 *z1, size_t h, size_t l) {
+    const size_t nd = 2 * l - h;                  // steps with a z2[h+i] word
+    size_t i = 0;
+    for (; i + 8 <= h; i += 8) {
+        const __m512i A = _mm512_loadu_si512(out + i), B = _mm512_loadu_si512(out + h + i);
+        const __m512i Cv = _mm512_loadu_si512(out + 2 * h + i);
+        __m512i D = _mm512_setzero_si512();
+        if (i + 8 <= nd)
+            D = _mm512_loadu_si512(out + 3 * h + i);
+        else if (i < nd)
+            D = _mm512_maskz_loadu_epi64((__mmask8)((1u << (nd - i)) - 1), out + 3 * h + i);
+        const __m512i lo = _mm512_xor_si512(_mm512_xor_si512(_mm512_loadu_si512(z1 + i), A), Cv);
+        const __m512i hi = _mm512_xor_si512(_mm512_xor_si512(_mm512_loadu_si512(z1 + h + i), B), D);
+        _mm512_storeu_si512(out + h + i, _mm512_xor_si512(B, lo));
+        _mm512_storeu_si512(out + 2 * h + i, _mm512_xor_si512(Cv, hi));
+    }
+    for (; i < h; i++) {
+        const u64 A = out[i], B = out[h + i], Cv = out[2 * h + i], D = i < nd ? out[3 * h + i] : 0;
+        out[h + i] = B ^ z1[i] ^ A ^ Cv;
+        out[2 * h + i] = Cv ^ z1[h + i] ^ B ^ D;
+    }
+}
+
+void kara_mix(u64 *out, const u64 *z1, size_t h, size_t l) {
+    if (use512()) return kara_mix512(out, z1, h, l);
+    const size_t nd = 2 * l - h;
+    for (size_t i = 0; i < h; i++) {
+        const u64 A = out[i], B = out[h + i], Cv = out[2 * h + i], D = i < nd ? out[3 * h + i] : 0;
+        out[h + i] = B ^ z1[i] ^ A ^ Cv;
+        out[2 * h + i] = Cv ^ z1[h + i] ^ B ^ D;
+    }
 }
 
 // out[0, 2n) = a[0,n) * b[0,n); ws holds at least 4n words per level
@@ -177,16 +266,20 @@ void kara(const u64 *a, const u64 *b, size_t n, u64 *out, u64 *ws) {
     }
     size_t h = (n + 1) / 2, l = n - h;
     u64 *sa = ws, *sb = ws + h, *z1 = ws + 2 * h, *next = ws + 4 * h;
-    for (size_t i = 0; i < h; i++) {
-        sa[i] = a[i] ^ (i < l ? a[h + i] : 0);
-        sb[i] = b[i] ^ (i < l ? b[h + i] : 0);
+    for (size_t i = 0; i < l; i++) {
+        sa[i] = a[i] ^ a[h + i];
+        sb[i] = b[i] ^ b[h + i];
+    }
+    if (l < h) {
+        sa[h - 1] = a[h - 1];
+        sb[h - 1] = b[h - 1];
     }
     if (n >= 2 * PAR_WORDS) {
         // the three half products are independent; each helper gets its
-        // own workspace
-        std::vector<u64> ws0(8 * h + 64), ws2(8 * h + 64);
-        auto f0 = spawn([&] { kara(a, b, h, out, ws0.data()); });
-        auto f2 = spawn([&] { kara(a + h, b + h, l, out + 2 * h, ws2.data()); });
+        // own workspace (left uninitialized)
+        std::unique_ptr<u64[]> w0(new u64[8 * h + 64]), w2(new u64[8 * h + 64]);
+        auto f0 = spawn([&] { kara(a, b, h, out, w0.get()); });
+        auto f2 = spawn([&] { kara(a + h, b + h, l, out + 2 * h, w2.get()); });
         kara(sa, sb, h, z1, next);
         f0.get();
         f2.get();
@@ -195,9 +288,7 @@ void kara(const u64 *a, const u64 *b, size_t n, u64 *out, u64 *ws) {
         kara(a + h, b + h, l, out + 2 * h, next); // z2 -> out[2h, 2n)
         kara(sa, sb, h, z1, next);                // z1 (2h words)
     }
-    for (size_t i = 0; i < 2 * h; i++) z1[i] ^= out[i];
-    for (size_t i = 0; i < 2 * l; i++) z1[i] ^= out[2 * h + i];
-    for (size_t i = 0; i < 2 * h; i++) out[h + i] ^= z1[i];
+    kara_mix(out, z1, h, l);
 }
 
 size_t words(const Poly &a) {
@@ -221,13 +312,15 @@ Poly host_mul(const u64 *a, size_t na, const u64 *b, size_t nb) {
         trim(out);
         return out;
     }
-    // a in blocks of nb words, each block times b by Karatsuba
-    std::vector<u64> ws(8 * nb + 64), blk(nb), part(2 * nb);
+    // a in blocks of nb words, each block times b by Karatsuba (scratch
+    // left uninitialized: every word is written before it is read)
+    std::unique_ptr<u64[]> scratch(new u64[8 * nb + 64 + 3 * nb]);
+    u64 *const wsp = scratch.get(), *const blk = wsp + 8 * nb + 64, *const part = blk + nb;
     for (size_t off = 0; off < na; off += nb) {
         size_t len = std::min(nb, na - off);
-        std::memcpy(blk.data(), a + off, len * 8);
-        if (len < nb) std::memset(blk.data() + len, 0, (nb - len) * 8);
-        kara(blk.data(), b, nb, part.data(), ws.data());
+        std::memcpy(blk, a + off, len * 8);
+        if (len < nb) std::memset(blk + len, 0, (nb - len) * 8);
+        kara(blk, b, nb, part, wsp);
         size_t lim = std::min(2 * nb, out.size() - off);
         for (size_t i = 0; i < lim; i++) out[off + i] ^= part[i];
     }
@@ -618,6 +711,10 @@ void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d, bool want_M) 
 }
 
 Poly gcd_fast(const Poly &a0, const Poly &b0) {
+    struct Caller {   // this thread computes: helpers are budgeted around it
+        Caller() { g_gcd_callers.fetch_add(1); }
+        ~Caller() { g_gcd_callers.fetch_sub(1); }
+    } caller;
     Poly a = a0, b = b0;
     trim(a);
     trim(b);

@@ -238,11 +238,18 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         t_dev += t1 - t0;
         // speculation (src/tsfactor.cu:1279-1331): the next interval goes
         // to the device for every slot while the host takes this one's
-        // gcds; the slots that finish here simply drop its results
+        // gcds; the slots that finish here simply drop its results.  An s
+        // clear through degree k is still clear through kb with
+        // probability about k / kb, so when that is below a quarter (the
+        // first interval after the sieve depth) most of the speculative
+        // work would be dropped, and it would slow the gcds' own device
+        // products: the next interval is then queued after the gcds, for
+        // the slots that continue.
         const u64 ka = k + 1, Ln = std::min(2 * L, Lmax);
         const bool more = kb < rhigh;
+        const bool spec = more && 4 * k >= kb;
         u64 kb_next = 0;
-        if (more) SCAN_CHECK(enqueue(kb, Ln, n, kb_next));
+        if (spec) SCAN_CHECK(enqueue(kb, Ln, n, kb_next));
         std::vector<Poly> g(n);
         out.n_gcd += (uint64_t)n;
         parallel_for((size_t)n, threads, [&](size_t i) {
@@ -273,15 +280,16 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                     done[i] = 1;
                 }
         // compact the survivors to the front of the slabs: their advanced
-        // Frobenius state and, when the next interval is queued, its
-        // products (the copies queue behind that interval's launches)
+        // Frobenius state and, when the next interval was queued
+        // speculatively, its products (the copies queue behind that
+        // interval's launches)
         int nl = 0;
         for (int i = 0; i < n; i++) {
             if (done[i]) continue;
             if (nl != i) {
                 SCAN_CHECK(gf2t_memcpy_d2d(cur + (size_t)nl * tab, cur + (size_t)i * tab,
                                            tab * 8, st));
-                if (more)
+                if (spec)
                     SCAN_CHECK(gf2t_memcpy_d2d(ACC.w() + (size_t)nl * pitch,
                                                ACC.w() + (size_t)i * pitch, pitch * 8, st));
                 live[nl] = live[i];
@@ -297,6 +305,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
                     (unsigned long long)k, (unsigned long long)kb, n, hit.size(), nl, m, t_dev,
                     t_gcd, t_res);
         n = nl;
+        if (more && !spec && n > 0) SCAN_CHECK(enqueue(kb, Ln, n, kb_next));
         k = kb;
         L = Ln;
         kb = kb_next;

@@ -1,5 +1,5 @@
-// Host carry-less products of hgcd.cpp (column schoolbook, 512-bit or
-// 128-bit, below the Karatsuba base, Karatsuba above it, block splitting
+// Host carry-less products of hgcd.cpp (512-bit output-block or 128-bit
+// column schoolbook below the Karatsuba base, Karatsuba above it, block splitting
 // for unbalanced operands, parallel half products from 1024 words)
 // against a bit-serial shift-and-xor product.  Device products are off (gcd_dev_bits 0), so
 // the device entry points below are never reached; they abort if called.
@@ -57,7 +57,8 @@ int main() {
         {24, 24},   {25, 25},  {31, 33},   {32, 32},   {33, 33},   {47, 49},  {48, 48},
         {49, 50},   {64, 65},  {1, 300},   {300, 1},   {5, 200},   {23, 1000}, {31, 1000},
         {32, 1000}, {33, 1000}, {96, 97},  {130, 129}, {333, 77},  {512, 512}, {700, 900},
-        {1100, 1100}};
+        {1100, 1100}, {8, 8},   {9, 7},    {15, 17},   {16, 16},   {17, 9},   {40, 8},
+        {8, 40},    {1025, 1025}, {2048, 2047}};
     int bad = 0, n = 0;
     for (auto &s : sizes)
         for (int zero_every : {0, 3}) {

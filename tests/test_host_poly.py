@@ -142,10 +142,11 @@ def test_cert_formats(tmp_path):
 
 
 def test_host_products_vs_bitserial(tmp_path):
-    """The gcd's host products (column schoolbook in its 512-bit and 128-bit
-    forms, Karatsuba, unbalanced blocks, parallel halves) against a
-    bit-serial product, with sizes around every switch point, zero words
-    and all-ones words."""
+    """The gcd's host products (the 512-bit output-block schoolbook and the
+    128-bit column one, Karatsuba with its fused recombination, unbalanced
+    blocks, parallel halves) against a bit-serial product, with sizes around
+    every switch point and eight-word block edge, zero words and all-ones
+    words."""
     import os
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -155,7 +156,7 @@ def test_host_products_vs_bitserial(tmp_path):
                            "-I" + os.path.join(root, "include"), "-I" + csrc, "-o", str(exe),
                            os.path.join(root, "tests", "host", "host_mul_check.cpp"),
                            os.path.join(csrc, "hgcd.cpp"), os.path.join(csrc, "host_poly.cpp")])
-    # the 512-bit schoolbook where the CPU has VPCLMULQDQ, and the 128-bit one
+    # the 512-bit schoolbook where the CPU has AVX-512 VPCLMULQDQ, and the 128-bit one
     for env in ({}, {"GF2T_NO_VPCLMUL": "1"}):
         p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120,
                            env={**os.environ, **env})
