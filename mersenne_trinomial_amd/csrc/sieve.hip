@@ -702,7 +702,7 @@ __global__ __launch_bounds__(W_T) void k_w_part1(WParams P, unsigned long long *
 // (src_cap entries apart; the first holds first_count) is partitioned by
 // the nbits-bit digit at bit `shift` of the stored value into
 // destination buckets src * 2^nbits + digit of dst_cap entries each.
-constexpr int WS_MAXB = 64;   // digits per pass
+constexpr int WS_MAXB = 128;  // digits per pass (2^7: degree 36 splits 13 bits as 7 + 6)
 constexpr int WS_T = 512;     // threads per block
 __global__ __launch_bounds__(WS_T) void k_w_split(const unsigned long long *__restrict__ in,
                                                  u64 src_cap, u64 first_count, int shift,
@@ -1175,7 +1175,10 @@ int wide_mem(int dmax) {
     if ((rc = W.wLlo.ensure(N * 4))) return rc;
     if ((rc = W.wLhi.ensure(N / 2))) return rc;
     if ((rc = W.wTab.ensure(2 * ((size_t)1 << W_AB) * 8))) return rc;
-    if ((rc = W.wCur.ensure((256 + 2048) * 4))) return rc;
+    // cursors: 256 level-1 buckets, then one per level-2 bucket of a
+    // level-1 bucket (2^(d - 8 - 15) at the largest d)
+    const size_t n2 = (size_t)1 << std::max(0, dmax - W_B1 - W_SB);
+    if ((rc = W.wCur.ensure((256 + n2) * 4))) return rc;
     if ((rc = W.bHeavy.ensure((size_t)HEAVY_CAP * sizeof(Heavy)))) return rc;
     if ((rc = W.bCnt.ensure(64))) return rc;
     return GF2T_OK;
@@ -1262,6 +1265,12 @@ int sieve_wide(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *dbest,
         const int nb = 256 / nsl;
         const u64 cap1 = (u64)1 << (d - W_B1);                 // values per level-1 bucket
         const int B2 = d - W_B1 - W_SB, B2a = (B2 + 1) / 2, B2b = B2 - B2a;
+        if ((1 << B2a) > WS_MAXB || W.wCur.bytes < (256 + ((size_t)1 << B2)) * 4) {
+            gf2t::set_error("gf2t_sieve: d=%d: level-2 split of %d bits beyond the cursor or "
+                            "digit tables", d, B2);
+            status = GF2T_EINTERNAL;
+            break;
+        }
         const u64 capA = cap1 >> B2a;
         const u64 zb = std::min<u64>((nsurv + (u64)ZC_T * ZC_PER - 1) / ((u64)ZC_T * ZC_PER),
                                      1u << 16);
