@@ -610,6 +610,101 @@ int zero_words(u64 *p, size_t n, hipStream_t st) {
     return GF2T_OK;
 }
 
+// ---------------------------------------------------- wrapped products
+// A product whose N = ca + cb - 1 chunk coefficients lie just above a
+// power of two, 2^mp < N <= 2^mp + 2^mp/8, runs on 2^mp-point transforms
+// instead of 2^(mp+1).  The 2^mp points are the span of the first mp
+// Cantor basis elements, whose vanishing polynomial is
+//     V(X) = sum over k with bits(k) inside bits(mp) of X^(2^k)
+// (V_{i+1} = V_i^2 + V_i, since V_i(b_i) = 1 for the Cantor basis; the
+// binomial coefficients mod 2 follow Lucas).  So pointwise products and
+// the inverse give C mod V.  The e = N - 2^mp top coefficients
+// C_high = C div X^(2^mp) come from the product of the top e chunks of
+// each operand (pairs i + j >= 2^mp only involve those), a 2^ms-point
+// product.  V's next term below X^(2^mp) is at most X^(2^(mp-1)), and
+// deg C_high < e <= 2^(mp-1), so C div V = C_high and
+//     C = (C mod V) + C_high * (V - X^(2^mp)) + C_high * X^(2^mp):
+// C_high is XORed into the low coefficients at each offset 2^k, k a
+// proper bit-subset of mp, and forms the coefficients from 2^mp up.
+struct Wrap {
+    bool on = false;
+    int mp = 0, ms = 0;
+    u64 e = 0;
+};
+struct WrapArg {
+    u64 n;          // 2^mp
+    u64 e;          // top coefficients
+    int noff;
+    u32 off[15];    // 2^k for the proper bit-subsets k of mp
+};
+Wrap wrap_for(u64 ca, u64 cb) {
+    Wrap w;
+    const u64 N = ca + cb - 1;
+    const int m = fft_m_for(N);
+    if (m < 12 || m - 1 > FFT_MAX_M) return w;
+    const int mp = m - 1;
+    const u64 n = (u64)1 << mp;
+    if (N <= n) return w;
+    const u64 e = N - n;
+    if (e > n / 8 || ca > n || cb > n || e >= ca || e >= cb) return w;
+    w.on = true;
+    w.mp = mp;
+    w.e = e;
+    w.ms = fft_m_for(2 * e - 1);
+    return w;
+}
+WrapArg wrap_arg(const Wrap &w) {
+    WrapArg a{};
+    a.n = (u64)1 << w.mp;
+    a.e = w.e;
+    for (int k = 0; k < w.mp; k++)
+        if ((k & w.mp) == k) a.off[a.noff++] = 1u << k;
+    return a;
+}
+
+// chunk coefficient k < n + e of the wrapped product: R = C mod V (2^mp
+// coefficients), H = C_high
+GF2T_DEV u64 wrap_chunk(const u64 *__restrict__ R, const u64 *__restrict__ H, const WrapArg &w,
+                        u64 k) {
+    if (k >= w.n) return H[k - w.n];
+    u64 v = R[k];
+    for (int j = 0; j < w.noff; j++) {
+        const u64 o = w.off[j];
+        if (k >= o && k - o < w.e) v ^= H[k - o];
+    }
+    return v;
+}
+
+// overlap-add of a wrapped product (gather form, as k_overlap_add);
+// blockIdx.y selects a batch element: R + y*r_stride, H + y*h_stride,
+// out + y*o_stride
+__global__ void k_overlap_add_wrap(const u64 *__restrict__ R, u64 r_stride,
+                                   const u64 *__restrict__ H, u64 h_stride, const WrapArg w,
+                                   size_t nchunks, u64 *__restrict__ out, size_t owords,
+                                   u64 o_stride) {
+    size_t wd = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (wd >= owords) return;
+    R += (u64)blockIdx.y * r_stride;
+    H += (u64)blockIdx.y * h_stride;
+    out += (u64)blockIdx.y * o_stride;
+    size_t k = 2 * wd;
+    u64 v = 0;
+    if (k < nchunks) v ^= wrap_chunk(R, H, w, k);
+    if (k + 1 < nchunks) v ^= wrap_chunk(R, H, w, k + 1) << 32;
+    if (wd && k - 1 < nchunks) v ^= wrap_chunk(R, H, w, k - 1) >> 32;
+    out[wd] = v;
+}
+
+// pack source of chunks [first, first + count) of an operand of nch chunks
+// whose last chunk is masked to lastmask
+PackArg pack_range(const u32 *src, u64 first, u64 count, u64 nch, u32 lastmask) {
+    PackArg p{};
+    p.src = src + first;
+    p.nchunks = count;
+    p.lastmask = first + count == nch ? lastmask : 0xFFFFFFFFu;
+    return p;
+}
+
 int mul_core(const u64 *da, u64 abits, const u64 *db, u64 bbits, u64 *dprod, void *dwork,
              hipStream_t st) {
     u64 ca = (abits + 31) >> 5, cb = (bbits + 31) >> 5;
@@ -622,6 +717,44 @@ int mul_core(const u64 *da, u64 abits, const u64 *db, u64 bbits, u64 *dprod, voi
     }
     int rc = ensure_basis();
     if (rc) return rc;
+    const Wrap W = wrap_for(ca, cb);
+    if (W.on) {
+        // both operands at 2^mp points, their top e chunks at 2^ms points
+        // (2^mp + 2^ms <= 2^m words each, inside the workspace of an
+        // unwrapped product)
+        const gf2t_fft_plan *pl = cached_plan(W.mp), *ps = cached_plan(W.ms);
+        const size_t n = (size_t)1 << W.mp, ns = (size_t)1 << W.ms;
+        u64 *FA = (u64 *)dwork, *FB = FA + n, *TA = FB + n, *TB = TA + ns;
+        const PackArg pa = pack_one(da, abits), pb = pack_one(db, bbits);
+        PackArg two{};
+        two.nlist = 2;
+        two.lsrc[0] = pa.src; two.lnch[0] = pa.nchunks; two.lmask[0] = pa.lastmask;
+        two.lsrc[1] = pb.src; two.lnch[1] = pb.nchunks; two.lmask[1] = pb.lastmask;
+        rc = run_passes(pl->fwd, FA, st, &two, 2, n);
+        if (rc) return rc;
+        const PackArg ta = pack_range(pa.src, ca - W.e, W.e, ca, pa.lastmask);
+        const PackArg tb = pack_range(pb.src, cb - W.e, W.e, cb, pb.lastmask);
+        PackArg top{};
+        top.nlist = 2;
+        top.lsrc[0] = ta.src; top.lnch[0] = ta.nchunks; top.lmask[0] = ta.lastmask;
+        top.lsrc[1] = tb.src; top.lnch[1] = tb.nchunks; top.lmask[1] = tb.lastmask;
+        rc = run_passes(ps->fwd, TA, st, &top, 2, ns);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pointwise, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)),
+                           dim3(256), 0, st, FA, FB, n);
+        hipLaunchKernelGGL(k_pointwise, dim3((unsigned)std::min<size_t>((ns + 255) / 256, 8192)),
+                           dim3(256), 0, st, TA, TB, ns);
+        rc = run_passes(pl->inv, FA, st, nullptr);
+        if (rc) return rc;
+        rc = run_passes(ps->inv, TA, st, nullptr);
+        if (rc) return rc;
+        // C_high[u] = coefficient e - 1 + u of the top product
+        hipLaunchKernelGGL(k_overlap_add_wrap, dim3((unsigned)((ow + 255) / 256)), dim3(256), 0,
+                           st, FA, (u64)0, TA + (W.e - 1), (u64)0, wrap_arg(W),
+                           (size_t)(ca + cb - 1), dprod, ow, (u64)0);
+        GF2T_CHECK_LAUNCH("k_overlap_add_wrap");
+        return GF2T_OK;
+    }
     const gf2t_fft_plan *plan = cached_plan(m);
     size_t n = (size_t)1 << m;
     u64 *FA = (u64 *)dwork, *FB = FA + n;
@@ -1064,6 +1197,55 @@ extern "C" int gf2t_mulmod_slab(const uint64_t *da, size_t apitch, const uint64_
     u64 *FA = (u64 *)dwork, *FB = FA + N * n, *prod = FB + N * n;
     size_t ppitch = 2 * nw + 2;
     u32 lm = (r & 31) ? (u32)((1ULL << (r & 31)) - 1) : 0xFFFFFFFFu;
+    const Wrap W = wrap_for(ca, ca);
+    if (W.on) {
+        // the wrapped product (see wrap_for): 2^mp-point transforms of both
+        // operands and 2^ms-point transforms of their top e chunks, laid
+        // out inside the 2 N n words an unwrapped product uses
+        const gf2t_fft_plan *pl = cached_plan(W.mp), *ps = cached_plan(W.ms);
+        const size_t np = (size_t)1 << W.mp, ns = (size_t)1 << W.ms;
+        u64 *GA = (u64 *)dwork, *GB = GA + np * n, *TA = GB + np * n, *TB = TA + ns * n;
+        PackArg pa{};
+        pa.src = (const u32 *)da;
+        pa.stride = 2 * apitch;
+        pa.nchunks = ca;
+        pa.lastmask = lm;
+        rc = run_passes(pl->fwd, GA, st, &pa, (unsigned)n, np);
+        if (rc) return rc;
+        PackArg pb = pa;
+        pb.src = (const u32 *)db;
+        pb.stride = 2 * bpitch;
+        pb.xor0 = b_add_x ? 2u : 0u;
+        rc = run_passes(pl->fwd, GB, st, &pb, (unsigned)n, np);
+        if (rc) return rc;
+        // top chunks [ca - e, ca); chunk 0 (where b_add_x adds x) is not among them
+        PackArg ta = pa, tb = pb;
+        ta.src += ca - W.e;
+        tb.src += ca - W.e;
+        ta.nchunks = tb.nchunks = W.e;
+        tb.xor0 = 0;
+        rc = run_passes(ps->fwd, TA, st, &ta, (unsigned)n, ns);
+        if (rc) return rc;
+        rc = run_passes(ps->fwd, TB, st, &tb, (unsigned)n, ns);
+        if (rc) return rc;
+        size_t tot = np * (size_t)n, tots = ns * (size_t)n;
+        hipLaunchKernelGGL(k_pointwise, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 65536)),
+                           dim3(256), 0, st, GA, GB, tot);
+        hipLaunchKernelGGL(k_pointwise, dim3((unsigned)std::min<size_t>((tots + 255) / 256, 65536)),
+                           dim3(256), 0, st, TA, TB, tots);
+        rc = run_passes(pl->inv, GA, st, nullptr, (unsigned)n, np);
+        if (rc) return rc;
+        rc = run_passes(ps->inv, TA, st, nullptr, (unsigned)n, ns);
+        if (rc) return rc;
+        size_t ow = 2 * nw;
+        hipLaunchKernelGGL(k_overlap_add_wrap, dim3((unsigned)((ow + 255) / 256), n), dim3(256), 0,
+                           st, GA, (u64)np, TA + (W.e - 1), (u64)ns, wrap_arg(W),
+                           (size_t)(2 * ca - 1), prod, ow, (u64)ppitch);
+        hipLaunchKernelGGL(k_fold_slab, dim3((unsigned)((nw + 255) / 256), n), dim3(256), 0, st,
+                           prod, ow, (u64)ppitch, 2 * r - 1, dout, (u64)opitch, nw, r, ds);
+        GF2T_CHECK_LAUNCH("gf2t_mulmod_slab");
+        return GF2T_OK;
+    }
     PackArg pa{};
     pa.src = (const u32 *)da;
     pa.stride = 2 * apitch;

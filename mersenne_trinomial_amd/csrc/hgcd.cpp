@@ -20,6 +20,8 @@
 #include <smmintrin.h>
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <condition_variable>
 #include <cstring>
@@ -37,6 +39,37 @@ namespace gf2t_host {
 namespace {
 
 std::atomic<u64> g_dev_bits{0};   // 0: host products only
+
+// Per-thread phase times of the gcd this thread runs, printed to stderr
+// at its end when GF2T_GCD_STATS is set (for gcds of at least 2^20 bits):
+// calls into the device product service and the time the thread waited
+// in them, and the word-level base case.  The rest of a gcd's time is
+// host products, divisions and bookkeeping.
+struct GcdStats {
+    long long dev_calls = 0, dev_ns = 0, base_calls = 0, base_ns = 0;
+};
+thread_local GcdStats t_stats;
+bool gcd_stats_on() {
+    static const bool on = [] {
+        const char *e = getenv("GF2T_GCD_STATS");
+        return e && *e && *e != '0';
+    }();
+    return on;
+}
+inline long long now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+// times one call for t_stats when statistics are on
+struct PhaseTimer {
+    long long *calls, *ns, t0;
+    PhaseTimer(long long &c, long long &n) : calls(&c), ns(&n), t0(gcd_stats_on() ? now_ns() : 0) {}
+    ~PhaseTimer() {
+        if (!t0) return;
+        ++*calls;
+        *ns += now_ns() - t0;
+    }
+};
 
 // Helper threads for independent products inside one gcd: a pool of
 // persistent workers (started on demand, at most host threads - 1).  A
@@ -393,6 +426,7 @@ void apply(const HMat &M, const Poly &x, const Poly &y, Poly &ox, Poly &oy) {
         size_t ow = gf2t_mul_out_words((u64)dm + 1, (u64)dv + 1);
         ox.assign(ow, 0);
         oy.assign(ow, 0);
+        PhaseTimer pt(t_stats.dev_calls, t_stats.dev_ns);
         check_rc(gf2t_mat2_host(hm, mb, x.data(), (u64)(deg(x) + 1), y.data(),
                                 (u64)(deg(y) + 1), ox.data(), oy.data(), ow),
                  "gf2t_mat2_host");
@@ -448,7 +482,10 @@ void applyN(const HMat &M, int n, const Poly *const *x, const Poly *const *y, Po
             ho[2 * i] = ox[i]->data();
             ho[2 * i + 1] = oy[i]->data();
         }
-        check_rc(gf2t_mat2n_host(hm, mb, n, hv, vb, ho, ow), "gf2t_mat2n_host");
+        {
+            PhaseTimer pt(t_stats.dev_calls, t_stats.dev_ns);
+            check_rc(gf2t_mat2n_host(hm, mb, n, hv, vb, ho, ow), "gf2t_mat2n_host");
+        }
         for (int i = 0; i < n; i++) {
             trim(*ox[i]);
             trim(*oy[i]);
@@ -639,6 +676,7 @@ Poly mul_fast(const Poly &a, const Poly &b) {
     int64_t da = deg(a), db = deg(b);
     if (lim && da + 1 >= (int64_t)lim && db + 1 >= (int64_t)lim) {
         Poly out(gf2t_mul_out_words((u64)da + 1, (u64)db + 1), 0);
+        PhaseTimer pt(t_stats.dev_calls, t_stats.dev_ns);
         check_rc(gf2t_mul_host(a.data(), (u64)da + 1, b.data(), (u64)db + 1, out.data()),
                  "gf2t_mul_host");
         trim(out);
@@ -659,6 +697,7 @@ void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d, bool want_M) 
         return;
     }
     if (n < HGCD_BASE) {
+        PhaseTimer pt(t_stats.base_calls, t_stats.base_ns);
         hgcd_base(a, b, m, M, c, d);
         return;
     }
@@ -715,6 +754,18 @@ Poly gcd_fast(const Poly &a0, const Poly &b0) {
         Caller() { g_gcd_callers.fetch_add(1); }
         ~Caller() { g_gcd_callers.fetch_sub(1); }
     } caller;
+    struct Report {   // GF2T_GCD_STATS: one line per large gcd
+        long long t0;
+        int64_t bits;
+        Report(int64_t b) : t0(gcd_stats_on() ? now_ns() : 0), bits(b) { t_stats = GcdStats{}; }
+        ~Report() {
+            if (!t0 || bits < (1 << 20)) return;
+            const GcdStats &S = t_stats;
+            fprintf(stderr, "gcd stats: %lld bits, %.3f s; device product calls %lld, %.3f s; "
+                    "base case %lld calls, %.3f s\n", (long long)bits, (now_ns() - t0) * 1e-9,
+                    S.dev_calls, S.dev_ns * 1e-9, S.base_calls, S.base_ns * 1e-9);
+        }
+    } report(std::max(deg(a0), deg(b0)) + 1);
     Poly a = a0, b = b0;
     trim(a);
     trim(b);

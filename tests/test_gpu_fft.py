@@ -46,6 +46,65 @@ def test_product_vs_oracle(eng, abits, bbits):
     assert np.array_equal(p, want)
 
 
+def _bits_for_chunks(rng, c):
+    """a bit length whose 32-bit chunk count is c (last chunk partly used)"""
+    return 32 * (c - 1) + int(rng.integers(1, 33))
+
+
+# (2^mp, e, ca): products of N = ca + cb - 1 = 2^mp + e chunk coefficients,
+# which run on 2^mp-point transforms plus a product of the top e chunks
+# (wrapped products): e = 1, e = 2^mp / 8 (the largest wrapped), balanced
+# and unbalanced operands, and e one past the limit (an ordinary product)
+WRAPPED = [(11, 1, 1025), (12, 512, 2304), (13, 2, 8000), (13, 3, 6000), (16, 3000, 34268),
+           (17, 16384, 73728), (15, 4097, 18432), (14, 700, 16384)]
+
+
+@pytest.mark.parametrize("mp,e,ca", WRAPPED)
+def test_wrapped_product_vs_oracle(eng, mp, e, ca):
+    rng = np.random.default_rng(mp * 1000 + e)
+    cb = (1 << mp) + e + 1 - ca
+    abits, bbits = _bits_for_chunks(rng, ca), _bits_for_chunks(rng, cb)
+    a, b = rand_bits(rng, abits), rand_bits(rng, bbits)
+    print(f"2^{mp} + {e} coefficients: {abits} x {bbits} bits", flush=True)
+    assert np.array_equal(eng.mul(a, abits, b, bbits), O.fft_mul(a, abits, b, bbits))
+    assert np.array_equal(eng.mul(b, bbits, a, abits), O.fft_mul(a, abits, b, bbits))
+
+
+@pytest.mark.parametrize("r", [530017, 589000, 600011])
+def test_wrapped_slab_mulmod_vs_oracle(eng, r):
+    """gf2t_mulmod_slab at r where the product's 2 ceil(r/32) - 1 chunk
+    coefficients lie just above 2^15 (530017: 359 over, 589000: 4045 over)
+    and, for comparison, past the wrapped range (600011), with F + x as the
+    second operand on every other call."""
+    import ctypes as C
+    from mersenne_trinomial_amd._capi import check, lib
+    from mersenne_trinomial_amd.device import DeviceBuffer
+    n, nw = 3, O.words(r)
+    pitch = (nw + 31) & ~31
+    rng = np.random.default_rng(r)
+    svals = np.array([1, 12345, (r - 1) // 2], np.uint64)
+    a = [rand_bits(rng, r) for _ in range(n)]
+    b = [rand_bits(rng, r) for _ in range(n)]
+
+    def slab(vs):
+        h = np.zeros(n * pitch, np.uint64)
+        for i, v in enumerate(vs):
+            h[i * pitch:i * pitch + nw] = v
+        return DeviceBuffer.from_array(h)
+    A, B, S = slab(a), slab(b), DeviceBuffer.from_array(svals)
+    out = DeviceBuffer.words(n * pitch)
+    work = DeviceBuffer(lib.gf2t_mulmod_slab_workspace_bytes(r, n))
+    for add_x in (0, 1):
+        check(lib.gf2t_mulmod_slab(A.ptr, pitch, B.ptr, pitch, add_x, r, S.ptr, out.ptr, pitch,
+                                   n, work.ptr, None))
+        got = out.download().reshape(n, pitch)
+        for i in range(n):
+            bb = b[i].copy()
+            bb[0] ^= np.uint64(2 * add_x)
+            want = O.mulmod(a[i], r, bb, r, r, int(svals[i]))
+            assert np.array_equal(got[i, :nw], want), (add_x, i)
+
+
 @pytest.mark.parametrize("r,s", [(4423, 1200), (132049, 1), (132049, 66024), (756839, 377)])
 def test_mulmod_vs_oracle(eng, r, s):
     rng = np.random.default_rng(r ^ s)
