@@ -89,12 +89,18 @@ def parse():
     ap.add_argument("--c3", type=int, default=1,
                     help="run the deep-scan workload (256 seeded s, sieve then deep scan) once "
                          "after the timed steps; its rate is the line's value")
+    ap.add_argument("--c4", type=int, default=1,
+                    help="run the Cantor-product batch (64 seeded pairs of 2^16..2^27 bits, each "
+                         "folded) after the timed steps and report it in a c4 object")
     ap.add_argument("--c3-n", type=int, default=256)
     ap.add_argument("--c3-seed", type=int, default=20250101)
     ap.add_argument("--c3-maxd", type=int, default=4096)
     ap.add_argument("--c3-interval", type=int, default=512)
     ap.add_argument("--c3-plain", type=int, default=1,
                     help="also run the same s one at a time (the plain per-s form)")
+    ap.add_argument("--scan-stagger", type=int, default=None,
+                    help="parts a deep-scan cohort is staggered into (gf2t_tune scan_stagger: "
+                         "0 = automatic, -1 = never)")
     ap.add_argument("--gcd-conc", type=int, default=8,
                     help="gcd mode: gcds run at once after the single one")
     ap.add_argument("--mode",
@@ -379,15 +385,23 @@ def c4_pairs(seed: int):
 
 
 def mode_fft(args):
+    """The Cantor-product batch alone (c4_measure), one JSON line."""
+    from mersenne_trinomial_amd import device
+    device.init(0)
+    print(json.dumps({"mode": "fft", **c4_measure(args)}), flush=True)
+
+
+def c4_measure(args):
     """The Cantor-product batch (64 seeded pairs of 2^16..2^27 bits, each
     product folded mod x^136279841 + x^s + 1): one pair at a time
-    (gf2t_mulmod) and as one variable-size batch (gf2t_mulmod_batch); then
-    full 136279841 x 136279841-bit products (2^24 points) with the forward
-    and inverse transforms timed apart, for the transform pass's rate."""
+    (gf2t_mulmod) and as one variable-size batch (gf2t_mulmod_batch), then
+    2^24-point transforms with the forward and inverse timed apart, for
+    the transform pass's rate, and full 136279841 x 136279841-bit modular
+    products (wrapped: 2^23-point transforms plus a product of the top
+    chunks)."""
     from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import check, lib
     from mersenne_trinomial_amd.device import DeviceBuffer, Event
-    device.init(0)
     nw = (R + 63) // 64
     pairs = c4_pairs(args.seed)
     n = len(pairs)
@@ -455,21 +469,37 @@ def mode_fft(args):
     device.synchronize()
     fwd_ms, inv_ms = e[0].elapsed_ms(e[1]) / reps, e[1].elapsed_ms(e[2]) / reps
     lib.gf2t_fft_plan_destroy(plan)
+    del f
     pass_bytes = 16 * (1 << 24)
     fwd_pass_ms = fwd_ms / fp.value
     mults = 24 * (1 << 23)            # GF(2^64) butterfly products per transform
-    print(json.dumps({"mode": "fft", "products": n, "operand_bits": bits,
-                      "one_by_one_ms": ms_one, "one_by_one_products_per_s": n / (ms_one * 1e-3),
-                      "batch_ms": ms_batch, "batch_products_per_s": n / (ms_batch * 1e-3),
-                      "batch_speedup": ms_one / ms_batch,
-                      "transform_2p24": {"fwd_ms": fwd_ms, "inv_ms": inv_ms,
-                                         "fwd_passes": fp.value, "inv_passes": ip.value,
-                                         "pass_ms": fwd_pass_ms,
-                                         "pass_GB_per_s": pass_bytes / (fwd_pass_ms * 1e-3) / 1e9,
-                                         "hbm_frac_per_pass": pass_bytes / (fwd_pass_ms * 1e-3)
-                                         / 1e9 / HBM_PEAK_GBS,
-                                         "gf64_mul_per_s": mults / (fwd_ms * 1e-3)}}),
-          flush=True)
+    # full-size modular products (the deep scan's interval product)
+    fa, fb, fo = pairs_full = (DeviceBuffer.words(nw), DeviceBuffer.words(nw),
+                               DeviceBuffer.words(nw))
+    rng = np.random.default_rng(args.seed)
+    for x in (fa, fb):
+        v = rng.integers(0, 2**64, nw, dtype=np.uint64)
+        v[-1] &= np.uint64((1 << (R % 64)) - 1)
+        x.upload(v)
+    fwork = DeviceBuffer(lib.gf2t_mul_workspace_bytes(R, R) + 8 * (2 * nw + 8))
+
+    def full():
+        check(lib.gf2t_mulmod(fa.ptr, R, fb.ptr, R, R, 474, fo.ptr, fwork.ptr, None),
+              "gf2t_mulmod")
+    ms_full = timed(full)
+    del pairs_full, fa, fb, fo, fwork
+    return {"products": n, "operand_bits": bits,
+            "one_by_one_ms": ms_one, "one_by_one_products_per_s": n / (ms_one * 1e-3),
+            "batch_ms": ms_batch, "batch_products_per_s": n / (ms_batch * 1e-3),
+            "batch_speedup": ms_one / ms_batch,
+            "full_mulmod_ms": ms_full,
+            "transform_2p24": {"fwd_ms": fwd_ms, "inv_ms": inv_ms,
+                               "fwd_passes": fp.value, "inv_passes": ip.value,
+                               "pass_ms": fwd_pass_ms,
+                               "pass_GB_per_s": pass_bytes / (fwd_pass_ms * 1e-3) / 1e9,
+                               "hbm_frac_per_pass": pass_bytes / (fwd_pass_ms * 1e-3)
+                               / 1e9 / HBM_PEAK_GBS,
+                               "gf64_mul_per_s": mults / (fwd_ms * 1e-3)}}
 
 
 def mode_fft24(args):
@@ -743,6 +773,8 @@ def apply_tuning(args):
         check(lib.gf2t_tune(b"sq_nt", args.sq_nt))
     if args.sq_fuse >= 0:
         check(lib.gf2t_tune(b"sq_fuse", args.sq_fuse))
+    if args.scan_stagger is not None:
+        check(lib.gf2t_tune(b"scan_stagger", args.scan_stagger))
 
 
 def chain_launches(steps: int, fuse: bool):
@@ -863,6 +895,10 @@ def main():
     sieve_rate = certified / (float(np.mean(sieve_ms)) * 1e-3)
     c3 = c3_measure(args, dbest) if args.c3 else None
     value = c3["s_per_s"] if c3 else sieve_rate
+    c4 = None
+    if args.c4:
+        print("c4: 64 seeded Cantor products, folded", file=sys.stderr, flush=True)
+        c4 = c4_measure(args)
 
     kernel, launches, per = chain_launches(args.chain_steps, args.sq_fuse != 0)
     launch_ms = float(np.mean(chain_ms)) / (args.chain_steps / per)
@@ -901,6 +937,7 @@ def main():
             "chain_steps": args.chain_steps, "seed": args.seed,
         },
         "c3": c3,
+        "c4": c4,
         "roofline": {
             "bound": "hbm",
             "kernel": kernel,

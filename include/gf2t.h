@@ -128,6 +128,11 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  *   "gcd_dev_bits" products inside the host gcds (gf2t_gcd_host, the
  *              deep scan) go to the device when both operands have at
  *              least this many bits; 0 = host Karatsuba only
+ *   "scan_stagger" gf2t_scan_run splits each cohort into this many parts
+ *              that start their device work one after another, so the
+ *              gcds of one part overlap the device work of the next
+ *              (not with a progress callback); 0 = up to 4 parts from
+ *              r = 2^22 on, -1 = never
  * Environment defaults: GF2T_SQ_FUSE (1), GF2T_SQ_PPT (2), GF2T_SQ_ORDER (0),
  * GF2T_SQ_NT (1), GF2T_GCD_DEV_BITS (2^16, set by gf2t_init; 0 before it).
  * GF2T_NO_VPCLMUL=1 keeps the host gcds' schoolbook products on 128-bit

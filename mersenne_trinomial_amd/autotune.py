@@ -15,7 +15,8 @@ its block size the same way, src/tsfactor.cu:1986-1989 and auto_m
 - the batched squaring knobs sq_fuse / sq_nt / sq_order / sq_ppt, timed
   on the deep scan's residue shape at the given r.
 - gcd_dev_bits: the product size from which the half-gcd uses the device,
-  timed on concurrent gcds.
+  timed on concurrent full-size gcds (the deep scan's interval gcds at
+  r = 136279841).
 - block_m_<k>: the deep scan's degrees per interval product for
   2^k <= r < 2^(k+1), from the measured per-degree cost
   m * t_sq + t_mul / m of the blocked scheme (t_sq and t_mul: a
@@ -153,8 +154,14 @@ def tune_block_m(r: int, reps: int = 3):
         check(lib.gf2t_set_x(b.word_ptr(i * pitch), nw, None))
     s = DeviceBuffer.from_array(np.array([(r - 1) // 3 - i for i in range(n)], dtype=np.uint64))
     work = DeviceBuffer(lib.gf2t_mulmod_slab_workspace_bytes(r, n))
-    t_sq = _time_ms(lambda: check(lib.gf2t_square_fold_slab_g(a.ptr, o.ptr, pitch, r, s.ptr, n, 1,
-                                                               None)), 8 * reps) / n
+    # per squaring, in the launches the deep scan uses for this r: two
+    # squarings per launch from 2^18 bits (when sq_fuse is on), else one
+    if r >= (1 << 18) and tune_get("sq_fuse"):
+        t_sq = _time_ms(lambda: check(lib.gf2t_square2_fold_slab_g(a.ptr, o.ptr, pitch, r, s.ptr,
+                                                                    n, 1, None)), 8 * reps) / (2 * n)
+    else:
+        t_sq = _time_ms(lambda: check(lib.gf2t_square_fold_slab_g(a.ptr, o.ptr, pitch, r, s.ptr,
+                                                                   n, 1, None)), 8 * reps) / n
     t_mul = _time_ms(lambda: check(lib.gf2t_mulmod_slab(a.ptr, pitch, b.ptr, pitch, 0, r, s.ptr,
                                                         o.ptr, pitch, n, work.ptr, None)),
                      reps) / n
@@ -239,8 +246,10 @@ def main(argv=None):
     ap.add_argument("--sq-r", type=int, default=136279841)
     ap.add_argument("--sq-residues", type=int, default=64)
     ap.add_argument("--block-r", type=str, default="132049,756839,3021377,136279841")
-    ap.add_argument("--gcd-r", type=int, default=(1 << 24) + 1)
-    ap.add_argument("--gcd-conc", type=int, default=8, help="0 skips the gcd threshold")
+    ap.add_argument("--gcd-r", type=int, default=136279841)
+    ap.add_argument("--gcd-conc", type=int, default=15,
+                    help="gcds at once when timing the gcd threshold (the deep scan's survivors "
+                         "of 256 uniformly drawn s at r = 136279841 number about 15); 0 skips it")
     args = ap.parse_args(argv)
     os.environ["GF2T_TUNE_FILE"] = ""          # measure from the built-in defaults
     init(0)
@@ -252,7 +261,7 @@ def main(argv=None):
     sq = tune_squaring(args.sq_r, args.sq_residues, 8 if args.quick else 16)
     blocks = [tune_block_m(int(r)) for r in args.block_r.split(",") if r]
     gcd = tune_gcd(args.gcd_r, args.gcd_conc,
-                   [1 << k for k in range(14, 19)]) if args.gcd_conc > 0 else None
+                   [1 << k for k in range(15, 19)]) if args.gcd_conc > 0 else None
     write(args.out, fft, sq, blocks, dev, gcd)
     print(json.dumps({"mode": "autotune", "device": dev.strip(), "file": args.out,
                       "seconds": round(time.perf_counter() - t0, 1), "fft_tile": fft,

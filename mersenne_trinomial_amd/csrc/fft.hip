@@ -460,6 +460,16 @@ int tile_bits_for(int m) {
     return T ? T : default_tile_bits(m);
 }
 
+// Threads of a pass over a 2^T-element tile: one per `ept` elements
+// (gf2t_tune "fft_ept": 2, 4, 8 or 16; default 2), between 64 and
+// FFT_THREADS.  Fewer threads per tile leave room for more blocks of
+// other launches on a CU (concurrent products of small transforms).
+std::atomic<int> g_fft_ept{2};
+unsigned threads_for_tile(int T) {
+    const u64 ept = (u64)g_fft_ept.load(std::memory_order_relaxed);
+    return (unsigned)std::max<u64>(64, std::min<u64>(FFT_THREADS, (1ULL << (T > 0 ? T : 0)) / ept));
+}
+
 const gf2t_fft_plan *cached_plan_T(int m, int T) {
     T = std::max(FFT_MIN_TILE_BITS, std::min(T, FFT_TILE_BITS));
     std::lock_guard<std::mutex> lk(g_plan_mu);
@@ -495,7 +505,7 @@ int run_passes(const std::vector<PassDesc> &ps, u64 *df, hipStream_t st, const P
         const int T = pd.c + pd.w;
         const int ntab = std::max(1, std::min(4, (pd.m - 1 + 7) / 8));
         const unsigned blocks = (unsigned)(1ULL << (pd.m - T));
-        const unsigned threads = (unsigned)std::max<u64>(64, std::min<u64>(FFT_THREADS, 1ULL << (T > 0 ? T - 1 : 0)));
+        const unsigned threads = threads_for_tile(T);
         const size_t lds = (4 * 256 + ((size_t)1 << T)) * sizeof(u64);
         const bool pack = i == 0 && pk;
         hipLaunchKernelGGL(k_fft_pass, dim3(blocks, nbatch), dim3(threads), lds, st, df, pd,
@@ -535,6 +545,15 @@ int gf2t_fft_set_tile_bits(int m, int64_t T) {
     return GF2T_OK;
 }
 int gf2t_fft_tile_bits(int m) { return (m >= 1 && m <= FFT_MAX_M) ? tile_bits_for(m) : 0; }
+int gf2t_fft_set_ept(int64_t ept) {
+    if (ept != 2 && ept != 4 && ept != 8 && ept != 16) {
+        gf2t::set_error("gf2t_tune: fft_ept needs 2, 4, 8 or 16");
+        return GF2T_EINVAL;
+    }
+    g_fft_ept.store((int)ept);
+    return GF2T_OK;
+}
+int gf2t_fft_ept() { return g_fft_ept.load(); }
 
 extern "C" int gf2t_cantor_basis(uint64_t *hbasis, int *len) {
     build_basis_host();
@@ -1141,7 +1160,7 @@ extern "C" int gf2t_mulmod_batch(int n, const uint64_t *const *da, const uint64_
         const Step &S = steps[k];
         const size_t lds = (4 * 256 + ((size_t)1 << S.T)) * sizeof(u64);
         const unsigned threads =
-            (unsigned)std::max<u64>(64, std::min<u64>(FFT_THREADS, 1ULL << (S.T - 1)));
+            threads_for_tile(S.T);
         hipLaunchKernelGGL(k_fft_pass_grouped, dim3(S.blocks), dim3(threads), lds, st,
                            dgp + S.first, S.count);
     }

@@ -89,6 +89,11 @@ int host_threads();
 int tuned_block_m(u64 r);
 bool set_tuned_block_m(int k, int m);
 
+// Deep-scan cohorts split into this many staggered parts (gf2t_tune
+// "scan_stagger"; 0 = automatic, -1 = never)
+void set_scan_stagger(int parts);
+int scan_stagger();
+
 // want_M = false: the caller needs only (c, d), and the node's final
 // matrix product M = S R2 is skipped (M is then unspecified)
 void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d, bool want_M = true);

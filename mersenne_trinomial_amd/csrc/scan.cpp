@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -61,6 +62,31 @@ bool scan_graphs_enabled() {
     return on;
 }
 
+// Staggered sub-cohorts: one cohort's s values split into parts that run
+// on their own threads and streams, part j starting its device work when
+// part j-1 has the products of its first interval.  The host then takes
+// the gcds of one part while the device runs the next, instead of every
+// gcd waiting for the whole cohort's device work.  `mu` guards the
+// results' shared phase counters.
+struct Stagger {
+    std::mutex mu;
+    std::condition_variable cv;
+    int next = 0;   // the part whose turn it is to start
+    void wait(int turn) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return next >= turn; });
+    }
+    void pass(int turn) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (next == turn) next = turn + 1;
+        }
+        cv.notify_all();
+    }
+};
+
+std::atomic<int> g_scan_stagger{0};   // parts per cohort; 0 = automatic
+
 #define SCAN_CHECK(call)                                                       \
     do {                                                                       \
         int rc__ = (call);                                                     \
@@ -78,10 +104,24 @@ struct Live {
 // block_m > 1: the blocked scheme of ddf.hip (tsfactor's h-table,
 // src/tsfactor.cu:1242-1341), one product per m degrees.  Intervals end
 // with a host gcd(ACC, T_s) per residue.
+// sg (may be null): this cohort is part `turn` of a staggered cohort
 int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
-               std::vector<Live> live, gf2t_scan_results &out, bool &stopped) {
+               std::vector<Live> live, gf2t_scan_results &out, bool &stopped,
+               Stagger *sg = nullptr, int turn = 0) {
     std::vector<gf2t_scan_result> &rows = out.rows;
     std::vector<std::string> &hexes = out.hex;
+    // the next part's turn comes once this one has its first interval's
+    // products, or when it ends (an error, nothing to scan)
+    struct TurnGuard {
+        Stagger *sg;
+        int turn;
+        bool passed = false;
+        void pass() {
+            if (sg && !passed) sg->pass(turn);
+            passed = true;
+        }
+        ~TurnGuard() { pass(); }
+    } turn_guard{sg, turn};
     if (P.k0 >= rhigh) {   // nothing left to scan: the bound is at or below the sieve depth
         for (const Live &L : live)
             rows[L.idx].kind = limited ? GF2T_SCAN_GAVE_UP : GF2T_SCAN_PRIMITIVE;
@@ -114,12 +154,20 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     for (int i = 0; i < B; i++) svals[i] = live[i].s;
     SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, st));
     uint64_t *cur = T0.w(), *oth = T1.w();
+    if (sg) sg->wait(turn);   // staggered: the previous part has its first products
     double t_dev = 0, t_gcd = 0, t_res = 0, t0 = now_s();
+    uint64_t n_gcd = 0;
     struct AddTimes {   // the cohort's phase times go into the results on every exit
         gf2t_scan_results &o;
         double &a, &b, &c;
-        ~AddTimes() { o.t_dev += a; o.t_gcd += b; o.t_res += c; }
-    } add_times{out, t_dev, t_gcd, t_res};
+        uint64_t &g;
+        Stagger *sg;
+        ~AddTimes() {
+            std::unique_lock<std::mutex> lk;
+            if (sg) lk = std::unique_lock<std::mutex>(sg->mu);
+            o.t_dev += a; o.t_gcd += b; o.t_res += c; o.n_gcd += g;
+        }
+    } add_times{out, t_dev, t_gcd, t_res, n_gcd, sg};
     if (m > 1) {
         // h-table for the first block, degrees k0+1 .. k0+m
         SCAN_CHECK(gf2t_ddf_init_slab(cur, oth, pitch, m, r, SV.w(), P.k0 + 1, B, st));
@@ -161,7 +209,9 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             for (auto &x : v) gf2t_graph_destroy(x.g);
         }
     } graph_guard{graphs};
-    const bool use_graphs = scan_graphs_enabled();
+    // a staggered part issues its launches directly: its sibling parts'
+    // gcd threads allocate and synchronize while it would be capturing
+    const bool use_graphs = scan_graphs_enabled() && !sg;
     bool warmed = false;
     auto run_block = [&](int nslot) -> int {
         if (!use_graphs || !warmed) {   // the first block runs eagerly (plans, constants)
@@ -201,8 +251,9 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     // interval) must finish before the buffers above are freed; declared
     // after them, this runs first.
     struct SyncOnExit {
-        ~SyncOnExit() { gf2t_device_sync(); }
-    } sync_on_exit;
+        void *st;
+        ~SyncOnExit() { gf2t_stream_sync(st); }
+    } sync_on_exit{st};
     int n = B;
     u64 k = P.k0, L = P.L0 ? P.L0 : 64;
     const u64 Lmax = P.Lmax ? P.Lmax : 4096;
@@ -236,6 +287,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         SCAN_CHECK(gf2t_memcpy_d2h(acc_host.data(), ACC.p, acc_host.size() * 8, st));
         double t1 = now_s();
         t_dev += t1 - t0;
+        turn_guard.pass();
         // speculation (src/tsfactor.cu:1279-1331): the next interval goes
         // to the device for every slot while the host takes this one's
         // gcds; the slots that finish here simply drop its results.  An s
@@ -251,7 +303,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         u64 kb_next = 0;
         if (spec) SCAN_CHECK(enqueue(kb, Ln, n, kb_next));
         std::vector<Poly> g(n);
-        out.n_gcd += (uint64_t)n;
+        n_gcd += (uint64_t)n;
         parallel_for((size_t)n, threads, [&](size_t i) {
             Poly a = from_words(acc_host.data() + i * pitch, nw);
             g[i] = gcd_fast(trinomial(r, live[i].s), a);
@@ -322,7 +374,58 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     return GF2T_OK;
 }
 
+// Parts a cohort of n s values is staggered into: the "scan_stagger"
+// knob when set; else, for residues of 2^22 bits and more (where one
+// interval's device work per s is long against the start of a gcd), up
+// to four; never with a progress callback (its calls and stop flag
+// belong to one cohort at a time).
+int stagger_parts(u64 r, size_t n, const gf2t_scan_params &P) {
+    if (P.progress || n < 2) return 1;
+    const int v = g_scan_stagger.load(std::memory_order_relaxed);
+    if (v > 0) return (int)std::min<size_t>((size_t)v, n);
+    if (v < 0 || r < ((u64)1 << 22)) return 1;
+    return (int)std::min<size_t>(4, n);
+}
+
+// The s values of one cohort in K contiguous parts, each run by its own
+// thread as a cohort of its own, part j starting its device work when
+// part j-1 has its first interval's products (Stagger).
+int run_staggered(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
+                  const std::vector<Live> &live, int K, gf2t_scan_results &out) {
+    Stagger sg;
+    std::vector<int> rcs((size_t)K, GF2T_OK);
+    std::vector<std::string> errs((size_t)K);
+    std::vector<std::thread> th;
+    const size_t n = live.size();
+    for (int j = 0; j < K; j++) {
+        const size_t a = n * (size_t)j / (size_t)K, b = n * (size_t)(j + 1) / (size_t)K;
+        std::vector<Live> part(live.begin() + a, live.begin() + b);
+        th.emplace_back([&, j, part]() mutable {
+            bool stopped = false;
+            try {
+                rcs[j] = run_cohort(r, P, rhigh, limited, std::move(part), out, stopped, &sg, j);
+                if (rcs[j]) errs[j] = gf2t_last_error();
+            } catch (const std::exception &e) {
+                rcs[j] = GF2T_EHIP;
+                errs[j] = e.what();
+            }
+        });
+    }
+    for (auto &t : th) t.join();
+    for (int j = 0; j < K; j++)
+        if (rcs[j]) {
+            gf2t::set_error("gf2t_scan_run: %s", errs[j].c_str());
+            return rcs[j];
+        }
+    return GF2T_OK;
+}
+
 }  // namespace
+
+namespace gf2t_host {
+void set_scan_stagger(int parts) { g_scan_stagger.store(parts); }
+int scan_stagger() { return g_scan_stagger.load(); }
+}  // namespace gf2t_host
 
 extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
                              const gf2t_scan_params *params, gf2t_scan_results **out) {
@@ -386,11 +489,16 @@ extern "C" int gf2t_scan_run(uint64_t r, const uint64_t *svals, size_t n,
                 for (const Live &L : part) res->rows[L.idx].kind = GF2T_SCAN_STOPPED;
                 continue;
             }
-            try {
-                rc = run_cohort(r, P, G.rhigh, G.limited, part, *res, stopped);
-            } catch (const std::exception &e) {
-                gf2t::set_error("gf2t_scan_run: %s", e.what());
-                rc = GF2T_EHIP;
+            const int K = stagger_parts(r, part.size(), P);
+            if (K > 1) {
+                rc = run_staggered(r, P, G.rhigh, G.limited, part, K, *res);
+            } else {
+                try {
+                    rc = run_cohort(r, P, G.rhigh, G.limited, part, *res, stopped);
+                } catch (const std::exception &e) {
+                    gf2t::set_error("gf2t_scan_run: %s", e.what());
+                    rc = GF2T_EHIP;
+                }
             }
             if (rc) { delete res; return rc; }
         }

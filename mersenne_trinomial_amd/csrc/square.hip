@@ -448,6 +448,8 @@ bool square_fuse_enabled() { return sq_fuse(); }
 
 int gf2t_fft_set_tile_bits(int m, int64_t T);   // fft.hip
 int gf2t_fft_tile_bits(int m);
+int gf2t_fft_set_ept(int64_t ept);                // fft.hip
+int gf2t_fft_ept();
 int gf2t_sieve_tune(const char *key, int64_t v);   // sieve.hip
 int gf2t_sieve_tune_get(const char *key, int64_t *v);
 
@@ -475,6 +477,15 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
         gf2t_host::set_device_product_bits(value > 0 ? (uint64_t)value : 0);
         return GF2T_OK;
     }
+    if (!strcmp(key, "fft_ept")) return gf2t_fft_set_ept(value);
+    if (!strcmp(key, "scan_stagger")) {
+        if (value < -1 || value > 64) {
+            gf2t::set_error("gf2t_tune: scan_stagger needs -1 <= value <= 64");
+            return GF2T_EINVAL;
+        }
+        gf2t_host::set_scan_stagger((int)value);
+        return GF2T_OK;
+    }
     gf2t::set_error("gf2t_tune: unknown key '%s'", key);
     return GF2T_EINVAL;
 }
@@ -489,6 +500,14 @@ extern "C" int gf2t_tune_get(const char *key, int64_t *value) {
     if (!strcmp(key, "sq_fuse")) { *value = sq_fuse() ? 1 : 0; return GF2T_OK; }
     if (!strcmp(key, "gcd_dev_bits")) {
         *value = (int64_t)gf2t_host::device_product_bits();
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "fft_ept")) {
+        *value = gf2t_fft_ept();
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "scan_stagger")) {
+        *value = gf2t_host::scan_stagger();
         return GF2T_OK;
     }
     if (!strcmp(key, "sieve_wide_from") || !strcmp(key, "sieve_slices"))

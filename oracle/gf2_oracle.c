@@ -168,18 +168,27 @@ static uint64_t inv_u(uint64_t a, uint64_t m) {
     return (uint64_t)t;
 }
 
-static void amin64(uint64_t *p, uint64_t v) {
+static inline void amin64(uint64_t *p, uint64_t v) __attribute__((unused));
+static inline void amin64(uint64_t *p, uint64_t v) {
     uint64_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
     while (v < cur &&
            !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED,
                                         __ATOMIC_RELAXED)) {}
 }
 
+/* one marking progression: s = sf, sf + n, ... (<= smax) gets key */
+struct prog { uint64_t sf, n, key; };
+
 struct sieve_job {
-    int d; uint64_t M, q, rmod, smax, j0, j1;
+    int d; uint64_t M, q, rmod, smax, j0, j1;   /* j0: first block, j1: block stride */
     const uint32_t *A, *L;
     uint64_t *best;
     uint64_t classes;
+    struct prog *pg;          /* progressions found by this job */
+    size_t npg, cap;
+    uint64_t lo, hi;          /* marking phase: this job's s range [lo, hi) */
+    struct sieve_job *all;    /* every job's progressions (marking phase) */
+    int njobs;
 };
 
 static void *sieve_worker(void *arg) {
@@ -187,7 +196,11 @@ static void *sieve_worker(void *arg) {
     const int d = J->d;
     const uint64_t M = J->M, q = J->q;
     uint32_t P[40];
-    for (uint64_t j = J->j0; j < J->j1; j++) {
+    /* blocks of 4096 candidates dealt round-robin: the classes that
+     * survive (Lyndon words) all lie below 2^(d-1), so contiguous ranges
+     * would leave the upper threads idle */
+    for (uint64_t b0 = J->j0; b0 < J->M; b0 += J->j1)
+    for (uint64_t j = b0 ? b0 : 1; j < b0 + 4096 && j < J->M; j++) {
         /* keep j iff it is the strict minimum of its rotation orbit
          * {j*2^i mod M}: that excludes proper-subfield elements (orbit
          * shorter than d) and non-minimal conjugates alike
@@ -226,8 +239,49 @@ static void *sieve_worker(void *arg) {
             if (P[k] > 1) return (void *)1;       /* not in GF(2): bug */
             mask |= (uint64_t)P[k] << k;
         }
-        uint64_t key = ((uint64_t)d << 48) | mask;
-        for (uint64_t s = sf; s <= J->smax; s += n) amin64(&J->best[s], key);
+        if (J->npg == J->cap) {
+            J->cap = J->cap ? 2 * J->cap : 1024;
+            struct prog *np = (struct prog *)realloc(J->pg, J->cap * sizeof *np);
+            if (!np) return (void *)1;
+            J->pg = np;
+        }
+        J->pg[J->npg++] = (struct prog){sf, n, ((uint64_t)d << 48) | mask};
+    }
+    return 0;
+}
+
+/* marking phase: every progression of every job, restricted to this
+ * job's s range, so no two threads write the same best[s] */
+static void *mark_worker(void *arg) {
+    struct sieve_job *J = (struct sieve_job *)arg;
+    for (int k = 0; k < J->njobs; k++) {
+        const struct sieve_job *O = &J->all[k];
+        for (size_t p = 0; p < O->npg; p++) {
+            const struct prog g = O->pg[p];
+            uint64_t s = g.sf;
+            if (s < J->lo) s += (J->lo - s + g.n - 1) / g.n * g.n;
+            for (; s < J->hi; s += g.n)
+                if (g.key < J->best[s]) J->best[s] = g.key;
+        }
+    }
+    return 0;
+}
+
+/* Antilog and log tables of GF(2^d) = GF(2)[x]/(q) for i in [i0, i1):
+ * A[i] = x^i, L[x^i] = i (the table build of src/sieve_ref.cpp, split
+ * into ranges so threads can share it; each range starts at x^i0). */
+struct table_job {
+    int d; uint64_t q, i0, i1;
+    uint32_t *A, *L;
+};
+
+static void *table_worker(void *arg) {
+    struct table_job *T = (struct table_job *)arg;
+    uint32_t t = fpow(2u, T->i0, T->q, T->d);
+    for (uint64_t i = T->i0; i < T->i1; i++) {
+        T->A[i] = t;
+        T->L[t] = (uint32_t)i;
+        t = fmulx(t, T->q, T->d);
     }
     return 0;
 }
@@ -245,20 +299,30 @@ int gf2o_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *best,
         uint32_t *A = (uint32_t *)malloc(M * sizeof(uint32_t));
         uint32_t *L = (uint32_t *)malloc((M + 1) * sizeof(uint32_t));
         if (!A || !L) { free(A); free(L); return -4; }
-        uint32_t t = 1;
-        for (uint64_t i = 0; i < M; i++) { A[i] = t; t = fmulx(t, q, d); }
-        if (t != 1) { free(A); free(L); return -5; }
-        for (uint64_t i = 0; i < M; i++) L[A[i]] = (uint32_t)i;
         int nt = nthreads;
         if ((uint64_t)nt > M) nt = 1;
+        {
+            struct table_job *tj = (struct table_job *)calloc(nt, sizeof *tj);
+            pthread_t *tt = (pthread_t *)calloc(nt, sizeof *tt);
+            for (int k = 0; k < nt; k++) {
+                tj[k] = (struct table_job){d, q, M * k / nt, M * (k + 1) / nt, A, L};
+                if (nt == 1) table_worker(&tj[k]);
+                else pthread_create(&tt[k], 0, table_worker, &tj[k]);
+            }
+            if (nt > 1)
+                for (int k = 0; k < nt; k++) pthread_join(tt[k], 0);
+            free(tj);
+            free(tt);
+        }
+        if (fmulx(A[M - 1], q, d) != 1) { free(A); free(L); return -5; }
         struct sieve_job *jobs = (struct sieve_job *)calloc(nt, sizeof *jobs);
         pthread_t *th = (pthread_t *)calloc(nt, sizeof *th);
         int bad = 0;
         for (int k = 0; k < nt; k++) {
             struct sieve_job *J = &jobs[k];
             J->d = d; J->M = M; J->q = q; J->rmod = rmod; J->smax = smax;
-            J->j0 = 1 + (M - 1) * k / nt;
-            J->j1 = 1 + (M - 1) * (k + 1) / nt;
+            J->j0 = 4096 * (uint64_t)k;        /* first block of this thread */
+            J->j1 = 4096 * (uint64_t)nt;       /* stride between its blocks */
             J->A = A; J->L = L; J->best = best;
             if (nt == 1) { bad |= sieve_worker(J) != 0; }
             else pthread_create(&th[k], 0, sieve_worker, J);
@@ -269,6 +333,20 @@ int gf2o_sieve(uint64_t r, int dmin, int dmax, uint64_t smax, uint64_t *best,
             cls += jobs[k].classes;
         }
         if (classes) classes[d] = cls;
+        if (!bad) {   /* mark: the s range split among the threads */
+            for (int k = 0; k < nt; k++) {
+                struct sieve_job *J = &jobs[k];
+                J->lo = 1 + smax * (uint64_t)k / nt;
+                J->hi = 1 + smax * (uint64_t)(k + 1) / nt;
+                J->all = jobs;
+                J->njobs = nt;
+                if (nt == 1) mark_worker(J);
+                else pthread_create(&th[k], 0, mark_worker, J);
+            }
+            if (nt > 1)
+                for (int k = 0; k < nt; k++) pthread_join(th[k], 0);
+        }
+        for (int k = 0; k < nt; k++) free(jobs[k].pg);
         free(jobs); free(th); free(A); free(L);
         if (bad) return -6;
     }

@@ -113,7 +113,8 @@ def test_mulmod_vs_oracle(eng, r, s):
 
 
 def test_full_scale_product_residues(eng):
-    # one 136279841 x 136279841-bit product (2^24 points), checked modulo
+    # one 136279841 x 136279841-bit product (a wrapped product: 2^23-point
+    # transforms plus a 2^18-point product of the top chunks), checked modulo
     # three degree-61 irreducibles (the reference's bigtest seeds,
     # src/cantor_ref.cpp:159-224), then folded mod x^r + x^474 + 1 and
     # compared with the oracle's linear fold
@@ -169,8 +170,9 @@ def test_trin_gpu_interval_product(eng):
 @pytest.mark.parametrize("m", [6, 9, 12, 14, 15, 17])
 def test_every_tile_size_bit_identical(eng, m):
     """The transform is the same whatever tile the passes use: every tile
-    size from 2^6 to 2^13 elements (gf2t_tune "fft_tile_<m>"), forward
-    against the oracle and back, and a product through the same plans."""
+    size from 2^6 to 2^13 elements (gf2t_tune "fft_tile_<m>"), with 2, 8
+    and 16 elements per thread (gf2t_tune "fft_ept"), forward against the
+    oracle and back, and a product through the same plans."""
     from mersenne_trinomial_amd._capi import lib
     import ctypes as C
     rng = np.random.default_rng(100 + m)
@@ -179,17 +181,22 @@ def test_every_tile_size_bit_identical(eng, m):
     bits = (1 << (m + 4)) - 77         # a product that needs 2^m points
     a, b = rand_bits(rng, bits), rand_bits(rng, 29)
     pwant = O.fft_mul(a, bits, b, 29)
-    old = C.c_int64()
+    old, old_ept = C.c_int64(), C.c_int64()
     assert lib.gf2t_tune_get(f"fft_tile_{m}".encode(), C.byref(old)) == 0
+    assert lib.gf2t_tune_get(b"fft_ept", C.byref(old_ept)) == 0
     try:
-        for T in range(6, 14):
-            print(f"m={m} tile 2^{T}", flush=True)
-            assert lib.gf2t_tune(f"fft_tile_{m}".encode(), T) == 0
-            F = eng.fft_fwd(f, m)
-            assert np.array_equal(F, want), T
-            assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f), T
-            assert np.array_equal(eng.mul(a, bits, b, 29), pwant), T
+        for ept in (2, 8, 16):
+            assert lib.gf2t_tune(b"fft_ept", ept) == 0
+            for T in range(6, 14):
+                print(f"m={m} tile 2^{T}, {ept} elements per thread", flush=True)
+                assert lib.gf2t_tune(f"fft_tile_{m}".encode(), T) == 0
+                F = eng.fft_fwd(f, m)
+                assert np.array_equal(F, want), (T, ept)
+                assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f), (T, ept)
+                assert np.array_equal(eng.mul(a, bits, b, 29), pwant), (T, ept)
     finally:
         assert lib.gf2t_tune(f"fft_tile_{m}".encode(), old.value) == 0
+        assert lib.gf2t_tune(b"fft_ept", old_ept.value) == 0
     assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 5) != 0
+    assert lib.gf2t_tune(b"fft_ept", 3) != 0
     assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 14) != 0

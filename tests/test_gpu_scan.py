@@ -41,6 +41,29 @@ def test_scan_r4423_vs_oracle(eng, L0, Lmax):
     assert lines == want
 
 
+@pytest.mark.parametrize("parts", [2, 3, 7])
+def test_scan_staggered_parts_vs_oracle(eng, parts):
+    """A cohort split into staggered parts (gf2t_tune "scan_stagger": each
+    part on its own thread and stream, starting its device work once the
+    previous part has its first interval's products) gives the oracle's
+    lines, with uneven part sizes and parts that finish at different
+    intervals."""
+    import ctypes as C
+    from mersenne_trinomial_amd._capi import lib
+    r, k0, maxd = 4423, 5, 200
+    want = oracle_lines(r, range(2, 351), k0, maxd)
+    svals = sorted(want)
+    old = C.c_int64()
+    assert lib.gf2t_tune_get(b"scan_stagger", C.byref(old)) == 0
+    try:
+        assert lib.gf2t_tune(b"scan_stagger", parts) == 0
+        got = eng.scan(r, svals, k0=k0, maxd=maxd, L0=16, Lmax=64)
+    finally:
+        assert lib.gf2t_tune(b"scan_stagger", old.value) == 0
+    assert {o.s: o.line() for o in got} == want
+    assert lib.gf2t_tune(b"scan_stagger", 65) != 0
+
+
 def test_scan_primitive_and_swan(eng):
     # x^127 + x^1 + 1 is primitive (127 is a Mersenne exponent, T(x)
     # irreducible): the scan reaches the bound with no factor
