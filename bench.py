@@ -130,10 +130,10 @@ def cpu_model() -> str:
 def cpu_baseline(certified: int, chain_s: int, c3: dict | None):
     """The oracle (the CPU restatement) on the same inputs as the sieve leg:
     r = 136279841, every s <= 68139920, degrees 2..32.  On all the job's
-    cores: degrees 2..24 in one run, 25 and 26 one degree each, and 27..32
-    extrapolated from degree 26 (per-degree work is dominated by the 2^d
-    table entries, so each degree costs twice the one before).  On one
-    core: 2..20, then 21 and 22, and 23..32 extrapolated from 22.  Plus
+    cores: degrees 2..26 in one run, then 27, and 28..32 extrapolated from
+    degree 27 (per-degree work is dominated by the 2^d
+    classes and table entries, so each degree costs twice the one before).
+    On one core: 2..22, then 23 and 24, and 25..32 extrapolated from 24.  Plus
     two full-size squaring+fold steps on one core, and from those a lower
     bound on the CPU time of the deep-scan workload (its squarings only)."""
     import oracle as O
@@ -157,8 +157,8 @@ def cpu_baseline(certified: int, chain_s: int, c3: dict | None):
                        "extrapolated_per_degree_s": {d: round(v, 2) for d, v in extra.items()},
                        "total_s": round(total, 2)}
 
-    t_n, info_n = sieve_time(cores, 24, [25, 26])
-    t_1, info_1 = sieve_time(1, 20, [21, 22])
+    t_n, info_n = sieve_time(cores, 26, [27])
+    t_1, info_1 = sieve_time(1, 22, [23, 24])
     x = O.x_residue(R)
     t0 = time.perf_counter()
     O.square_fold_steps(x, R, chain_s, 2)
@@ -169,9 +169,9 @@ def cpu_baseline(certified: int, chain_s: int, c3: dict | None):
         "cores": cores,
         "kind": "port",
         "sample": (f"oracle coarse sieve at r={R} over all s<=68139920, degrees 2..32 (the "
-                   f"sieve leg's inputs; {certified} certified): degrees 2..26 measured on "
-                   f"{cores} threads, 27..32 extrapolated (x2 per degree) = {t_n:.1f} s; "
-                   f"one core: {t_1:.1f} s (2..22 measured)"),
+                   f"sieve leg's inputs; {certified} certified): degrees 2..27 measured on "
+                   f"{cores} threads, 28..32 extrapolated (x2 per degree) = {t_n:.1f} s; "
+                   f"one core: {t_1:.1f} s (2..24 measured)"),
         "nproc": os.cpu_count(),
         "affinity_cpus": len(os.sched_getaffinity(0)),
         "cpu_model": cpu_model(),
@@ -191,7 +191,7 @@ def cpu_baseline(certified: int, chain_s: int, c3: dict | None):
         out["value"] = c3["n_s"] / (t_n + sq)
         out["sample"] = (f"deep-scan workload ({c3['n_s']} s) on the oracle, {cores} threads: the "
                          f"coarse sieve of all s<=68139920 to degree 32 ({t_n:.1f} s: degrees "
-                         f"2..26 measured, 27..32 extrapolated x2 per degree) + the "
+                         f"2..27 measured, 28..32 extrapolated x2 per degree) + the "
                          f"{c3['survivors']} survivors' first-interval squarings "
                          f"({c3['survivors']} x 512 x 32 x {1e3 * t_sq:.1f} ms / {cores} = "
                          f"{sq:.1f} s); interval products and gcds omitted, so value is an "
