@@ -128,6 +128,13 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  *   "gcd_dev_bits" products inside the host gcds (gf2t_gcd_host, the
  *              deep scan) go to the device when both operands have at
  *              least this many bits; 0 = host Karatsuba only
+ *   "fft_ept"  elements of a transform tile per thread (2, 4, 8, 16;
+ *              default 2): fewer threads per tile
+ *   "hm_zc_words" products of the host-buffer service (gf2t_mat2n_host,
+ *              gf2t_mat2_host) whose operands and results take at most
+ *              this many words together are read and written by the GPU
+ *              in pinned host memory directly, with no copy each way
+ *              (default 65536; 0 = always copy)
  *   "scan_stagger" gf2t_scan_run splits each cohort into this many parts
  *              that start their device work one after another, so the
  *              gcds of one part overlap the device work of the next

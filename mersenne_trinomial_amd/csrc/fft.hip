@@ -1306,6 +1306,9 @@ namespace {
 // buffer and go to the device in one transfer; results come back the
 // same way.
 constexpr int HM_CONTEXTS = 16;
+// products whose operands and results together take at most this many
+// words go through pinned host memory directly (gf2t_tune "hm_zc_words")
+std::atomic<long long> g_hm_zc_words{65536};
 struct HostMulCtx {
     hipStream_t st = nullptr;
     u64 *dbuf = nullptr;
@@ -1372,6 +1375,16 @@ int hm_reserve(HostMulCtx &c, size_t words, size_t host_words) {
 
 u32 last_mask(u64 bits) { return (bits & 31) ? (u32)((1ULL << (bits & 31)) - 1) : 0xFFFFFFFFu; }
 }  // namespace
+
+int gf2t_hm_set_zc_words(int64_t words) {
+    if (words < 0) {
+        gf2t::set_error("gf2t_tune: hm_zc_words must be >= 0");
+        return GF2T_EINVAL;
+    }
+    g_hm_zc_words.store(words);
+    return GF2T_OK;
+}
+int64_t gf2t_hm_zc_words() { return g_hm_zc_words.load(); }
 
 extern "C" int gf2t_mul_host(const uint64_t *ha, uint64_t abits, const uint64_t *hb,
                              uint64_t bbits, uint64_t *hout) {
@@ -1486,7 +1499,20 @@ extern "C" int gf2t_mat2n_host(const uint64_t *const hm[4], const uint64_t mbits
         pk.lmask[i] = last_mask(bits[i]);
         off += (in_w[i] + 3) & ~(size_t)3;
     }
-    if (off) GF2T_HIP(hipMemcpyAsync(dstage, hc.hbuf, off * 8, hipMemcpyHostToDevice, st));
+    // small products read their operands from the pinned staging buffer
+    // and write their results back into it (the GPU reaches pinned host
+    // memory directly), instead of a copy each way
+    const bool zc = off + nv * ow <= (size_t)g_hm_zc_words.load(std::memory_order_relaxed);
+    if (zc) {
+        size_t o2 = 0;
+        for (int i = 0; i < nt; i++) {
+            pk.lsrc[i] = (const u32 *)(hc.hbuf + o2);
+            o2 += (in_w[i] + 3) & ~(size_t)3;
+        }
+        dout = hc.hbuf;
+    } else if (off) {
+        GF2T_HIP(hipMemcpyAsync(dstage, hc.hbuf, off * 8, hipMemcpyHostToDevice, st));
+    }
     rc = run_passes(plan->fwd, F, st, &pk, (unsigned)nt, n);
     if (rc) return rc;
     unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
@@ -1497,7 +1523,7 @@ extern "C" int gf2t_mat2n_host(const uint64_t *const hm[4], const uint64_t mbits
     hipLaunchKernelGGL(k_overlap_add, dim3((unsigned)((ow + 255) / 256), nv), dim3(256), 0, st,
                        F + 4 * n, nch, dout, ow, (u64)n, (u64)ow);
     GF2T_CHECK_LAUNCH("gf2t_mat2n_host");
-    GF2T_HIP(hipMemcpyAsync(hc.hbuf, dout, nv * ow * 8, hipMemcpyDeviceToHost, st));
+    if (!zc) GF2T_HIP(hipMemcpyAsync(hc.hbuf, dout, nv * ow * 8, hipMemcpyDeviceToHost, st));
     GF2T_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < nv; i++) {
         std::memcpy(hout[i], hc.hbuf + (size_t)i * ow, ow * 8);

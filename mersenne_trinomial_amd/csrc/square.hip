@@ -450,6 +450,8 @@ int gf2t_fft_set_tile_bits(int m, int64_t T);   // fft.hip
 int gf2t_fft_tile_bits(int m);
 int gf2t_fft_set_ept(int64_t ept);                // fft.hip
 int gf2t_fft_ept();
+int gf2t_hm_set_zc_words(int64_t words);          // fft.hip
+int64_t gf2t_hm_zc_words();
 int gf2t_sieve_tune(const char *key, int64_t v);   // sieve.hip
 int gf2t_sieve_tune_get(const char *key, int64_t *v);
 
@@ -478,6 +480,7 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
         return GF2T_OK;
     }
     if (!strcmp(key, "fft_ept")) return gf2t_fft_set_ept(value);
+    if (!strcmp(key, "hm_zc_words")) return gf2t_hm_set_zc_words(value);
     if (!strcmp(key, "scan_stagger")) {
         if (value < -1 || value > 64) {
             gf2t::set_error("gf2t_tune: scan_stagger needs -1 <= value <= 64");
@@ -504,6 +507,10 @@ extern "C" int gf2t_tune_get(const char *key, int64_t *value) {
     }
     if (!strcmp(key, "fft_ept")) {
         *value = gf2t_fft_ept();
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "hm_zc_words")) {
+        *value = gf2t_hm_zc_words();
         return GF2T_OK;
     }
     if (!strcmp(key, "scan_stagger")) {

@@ -44,9 +44,21 @@ def test_mul_host_vs_oracle(lib, abits, bbits):
     assert np.array_equal(out, O.mul_comb(a, abits, b, bbits))
 
 
+@pytest.fixture(params=[0, 1 << 30], ids=["copies", "zero-copy"])
+def zc(lib, request):
+    """the product service with its operands and results copied through
+    device buffers (hm_zc_words 0) and read and written in pinned host
+    memory directly (hm_zc_words above every size here)"""
+    old = C.c_int64()
+    assert lib.gf2t_tune_get(b"hm_zc_words", C.byref(old)) == 0
+    assert lib.gf2t_tune(b"hm_zc_words", request.param) == 0
+    yield request.param
+    assert lib.gf2t_tune(b"hm_zc_words", old.value) == 0
+
+
 @pytest.mark.parametrize("npairs,mb,vb", [(1, 300, 7000), (2, 40000, 100000), (3, 70001, 260000),
                                           (4, 64, 64)])
-def test_mat2n_host_vs_products(lib, npairs, mb, vb):
+def test_mat2n_host_vs_products(lib, zc, npairs, mb, vb):
     """One matrix applied to npairs vector pairs of different lengths (one
     set of matrix transforms) equals the schoolbook products."""
     rng = np.random.default_rng(npairs * 1000 + mb)
@@ -78,7 +90,7 @@ def test_mat2n_host_vs_products(lib, npairs, mb, vb):
 
 
 @pytest.mark.parametrize("mb,vb", [(100, 5000), (30000, 90000), (1, 1)])
-def test_mat2_host_vs_products(lib, mb, vb):
+def test_mat2_host_vs_products(lib, zc, mb, vb):
     rng = np.random.default_rng(mb + vb)
     m = [rp(rng, mb - i if mb > 4 else mb) for i in range(4)]
     mbits = np.array([mb - i if mb > 4 else mb for i in range(4)], dtype=np.uint64)
@@ -108,7 +120,7 @@ def gcd_host(lib, a, b):
 
 
 @pytest.mark.parametrize("dev_bits", [0, 1 << 12])
-def test_gcd_with_device_products_vs_oracle(lib, dev_bits):
+def test_gcd_with_device_products_vs_oracle(lib, zc, dev_bits):
     assert lib.gf2t_tune(b"gcd_dev_bits", dev_bits) == 0
     try:
         rng = np.random.default_rng(dev_bits + 3)
