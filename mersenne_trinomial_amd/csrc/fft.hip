@@ -631,9 +631,10 @@ int zero_words(u64 *p, size_t n, hipStream_t st) {
 
 // ---------------------------------------------------- wrapped products
 // A product whose N = ca + cb - 1 chunk coefficients lie just above a
-// power of two, 2^mp < N <= 2^mp + 2^mp/8, runs on 2^mp-point transforms
-// instead of 2^(mp+1).  The 2^mp points are the span of the first mp
-// Cantor basis elements, whose vanishing polynomial is
+// power of two, 2^mp < N <= 2^mp + 2^mp/4, runs on 2^mp-point transforms
+// instead of 2^(mp+1), plus a product at most half that size.  The 2^mp
+// points are the span of the first mp Cantor basis elements, whose
+// vanishing polynomial is
 //     V(X) = sum over k with bits(k) inside bits(mp) of X^(2^k)
 // (V_{i+1} = V_i^2 + V_i, since V_i(b_i) = 1 for the Cantor basis; the
 // binomial coefficients mod 2 follow Lucas).  So pointwise products and
@@ -641,7 +642,7 @@ int zero_words(u64 *p, size_t n, hipStream_t st) {
 // C_high = C div X^(2^mp) come from the product of the top e chunks of
 // each operand (pairs i + j >= 2^mp only involve those), a 2^ms-point
 // product.  V's next term below X^(2^mp) is at most X^(2^(mp-1)), and
-// deg C_high < e <= 2^(mp-1), so C div V = C_high and
+// deg C_high < e <= 2^(mp-2), so C div V = C_high and
 //     C = (C mod V) + C_high * (V - X^(2^mp)) + C_high * X^(2^mp):
 // C_high is XORed into the low coefficients at each offset 2^k, k a
 // proper bit-subset of mp, and forms the coefficients from 2^mp up.
@@ -665,7 +666,7 @@ Wrap wrap_for(u64 ca, u64 cb) {
     const u64 n = (u64)1 << mp;
     if (N <= n) return w;
     const u64 e = N - n;
-    if (e > n / 8 || ca > n || cb > n || e >= ca || e >= cb) return w;
+    if (e > n / 4 || ca > n || cb > n || e >= ca || e >= cb) return w;
     w.on = true;
     w.mp = mp;
     w.e = e;

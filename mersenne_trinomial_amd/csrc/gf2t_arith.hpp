@@ -61,10 +61,43 @@ GF2T_HD u64 range_mask64(int lc, int hc) {
 }
 
 // ---------------------------------------------- carry-less 32x32 -> 64
-// Operands split into four bit classes (bits 4 apart); every partial
-// integer product a_i*b_j has at most 8 set bits per column of its
-// result class, so columns never carry into the next used position and
-// the class bits are exact carry-less sums.  (A three-class form over
+// Three-input XOR and bitwise select (m ? x : y per bit): one gfx950
+// instruction each on a 32-bit half (v_bitop3_b32, truth tables 0x96 and
+// 0xCA over inputs 0xF0, 0xCC, 0xAA); the same functions in plain C on
+// the host.
+GF2T_HD u32 xor3_32(u32 a, u32 b, u32 c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+GF2T_HD u32 sel_32(u32 m, u32 x, u32 y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(m, x, y, 0xCA);
+#else
+    return (m & x) | (~m & y);
+#endif
+}
+GF2T_HD u64 xor3(u64 a, u64 b, u64 c) {
+    return ((u64)xor3_32((u32)(a >> 32), (u32)(b >> 32), (u32)(c >> 32)) << 32) |
+           xor3_32((u32)a, (u32)b, (u32)c);
+}
+GF2T_HD u64 sel64(u64 m, u64 x, u64 y) {
+    return ((u64)sel_32((u32)(m >> 32), (u32)(x >> 32), (u32)(y >> 32)) << 32) |
+           sel_32((u32)m, (u32)x, (u32)y);
+}
+
+// Operands split into four bit classes (bits 4 apart): the integer
+// product a_i*b_j of class i of a and class j of b has its bits in class
+// i + j mod 4, with at most 8 terms per column, so a column's count fits
+// below the next class position and its low bit is the carry-less sum.
+// Two such products whose i + j differ by 4 have column counts that
+// peak 4 positions apart and sum to at most 15 (checked by brute force on
+// all-ones operands), so they are added in one multiply-add
+// (v_mad_u64_u32); the other products of a class are XORed in.  The
+// classes are then merged by bit selects.  40 VALU instructions: 8
+// masks, 16 multiplies, 10 XORs, 6 selects.  (A three-class form over
 // 16-bit halves with 24-bit multiplies, v_mul_u32_u24, was checked
 // bit-exact and made the transform passes 6% slower on gfx950.)
 GF2T_HD u64 clmul32(u32 a, u32 b) {
@@ -72,26 +105,30 @@ GF2T_HD u64 clmul32(u32 a, u32 b) {
               m3 = 0x88888888u;
     u64 a0 = a & m0, a1 = a & m1, a2 = a & m2, a3 = a & m3;
     u64 b0 = b & m0, b1 = b & m1, b2 = b & m2, b3 = b & m3;
-    u64 c0 = (a0 * b0) ^ (a1 * b3) ^ (a2 * b2) ^ (a3 * b1);
-    u64 c1 = (a0 * b1) ^ (a1 * b0) ^ (a2 * b3) ^ (a3 * b2);
-    u64 c2 = (a0 * b2) ^ (a1 * b1) ^ (a2 * b0) ^ (a3 * b3);
-    u64 c3 = (a0 * b3) ^ (a1 * b2) ^ (a2 * b1) ^ (a3 * b0);
-    return (c0 & 0x1111111111111111ULL) ^ (c1 & 0x2222222222222222ULL) ^
-           (c2 & 0x4444444444444444ULL) ^ (c3 & 0x8888888888888888ULL);
+    u64 c0 = xor3(a0 * b0 + a1 * b3, a2 * b2, a3 * b1);     // i + j: 0 + 4, 4, 4
+    u64 c1 = (a0 * b1 + a2 * b3) ^ (a1 * b0 + a3 * b2);     // 1 + 5, 1 + 5
+    u64 c2 = xor3(a0 * b2 + a3 * b3, a1 * b1, a2 * b0);     // 2 + 6, 2, 2
+    u64 c3 = xor3(a0 * b3, a1 * b2, a2 * b1) ^ (a3 * b0);   // 3, 3, 3, 3
+    return sel64(0x1111111111111111ULL, c0,
+                 sel64(0x2222222222222222ULL, c1, sel64(0x4444444444444444ULL, c2, c3)));
 }
 
 // GF(2^64) = GF(2)[x]/(x^64 + x^4 + x^3 + x + 1) multiply: Karatsuba
-// over clmul32, then fold the high word twice (x^64 = x^4+x^3+x+1).
+// over clmul32, then fold the high word twice (x^64 = x^4+x^3+x+1): the
+// first fold's overflow s is at most 4 bits, so the second stays in the
+// low word.
 GF2T_HD u64 gf64_mul(u64 a, u64 b) {
     u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
     u64 lo = clmul32(a0, b0);
     u64 hi = clmul32(a1, b1);
-    u64 mid = clmul32(a0 ^ a1, b0 ^ b1) ^ lo ^ hi;
+    u64 mid = xor3(clmul32(a0 ^ a1, b0 ^ b1), lo, hi);
     lo ^= mid << 32;
     hi ^= mid >> 32;
-    u64 r = lo ^ hi ^ (hi << 1) ^ (hi << 3) ^ (hi << 4);
-    u64 s = (hi >> 63) ^ (hi >> 61) ^ (hi >> 60);
-    return r ^ s ^ (s << 1) ^ (s << 3) ^ (s << 4);
+    u64 r = xor3(xor3(lo, hi, hi << 1), hi << 3, hi << 4);
+    u32 h = (u32)(hi >> 32);
+    u32 sv = xor3_32(h >> 31, h >> 29, h >> 28);
+    u32 rl = xor3_32((u32)r, xor3_32(sv, sv << 1, sv << 3), sv << 4);
+    return ((r >> 32) << 32) | rl;
 }
 
 // ------------------------------------------------------- GF(2^d), d <= 32
@@ -109,7 +146,7 @@ GF2T_HD u32 fsqr(u32 a, u64 q, int d) { return fred(spread32(a), q, d); }
 GF2T_HD void clmul64(u64 a, u64 b, u64 &lo, u64 &hi) {
     u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
     u64 l = clmul32(a0, b0), h = clmul32(a1, b1);
-    u64 mid = clmul32(a0 ^ a1, b0 ^ b1) ^ l ^ h;
+    u64 mid = xor3(clmul32(a0 ^ a1, b0 ^ b1), l, h);
     lo = l ^ (mid << 32);
     hi = h ^ (mid >> 32);
 }

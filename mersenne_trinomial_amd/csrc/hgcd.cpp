@@ -504,29 +504,39 @@ inline u64 clmul_lo(u64 a, u64 b) {
     return lo;
 }
 
-// bits [p, p + 64) of a (zero beyond its words)
-inline u64 window64(const Poly &a, u64 p) {
-    size_t w = (size_t)(p >> 6);
-    unsigned off = (unsigned)(p & 63);
-    u64 lo = w < a.size() ? a[w] >> off : 0;
-    u64 hi = (off && w + 1 < a.size()) ? a[w + 1] << (64 - off) : 0;
-    return lo | hi;
+typedef unsigned __int128 u128;
+
+inline int deg128(u128 v) {
+    const u64 hi = (u64)(v >> 64);
+    return hi ? 127 - __builtin_clzll(hi) : deg64((u64)v);
+}
+
+// bits [p, p + 128) of a (zero beyond its words)
+inline u128 window128(const Poly &a, u64 p) {
+    const size_t w = (size_t)(p >> 6), n = a.size();
+    const unsigned off = (unsigned)(p & 63);
+    const u64 w0 = w < n ? a[w] : 0, w1 = w + 1 < n ? a[w + 1] : 0, w2 = w + 2 < n ? a[w + 2] : 0;
+    if (!off) return ((u128)w1 << 64) | w0;
+    return ((u128)((w1 >> off) | (w2 << (64 - off))) << 64) | ((w0 >> off) | (w1 << (64 - off)));
 }
 
 struct W2 {
     u64 m00, m01, m10, m11;
 };
 
-// Remainder steps of single words (deg c <= 63, deg c > deg d) while
-// deg d >= stop; returns the step matrix (entries stay below 64 bits
-// because each has degree deg(c0) - deg(current c) <= 63 - stop).
-W2 hgcd_word(u64 c, u64 d, int stop, int &steps) {
+// Remainder steps of a pair of at most 128-bit polynomials (deg c > deg d)
+// while deg d >= stop; returns the step matrix.  Each entry has degree
+// deg(c0) - deg(current c) <= 127 - stop, and each quotient at most
+// deg(c) - stop, so both fit in one word when stop >= 64 (or deg c0 <=
+// 63 + stop).
+W2 hgcd_word(u128 c, u128 d, int stop, int &steps) {
     W2 W{1, 0, 0, 1};
     steps = 0;
-    while (d && deg64(d) >= stop) {
-        const int dd = deg64(d);
-        u64 q = 0, r = c;
-        for (int dr = deg64(r); dr >= dd; dr = deg64(r)) {
+    while (d && deg128(d) >= stop) {
+        const int dd = deg128(d);
+        u64 q = 0;
+        u128 r = c;
+        for (int dr = deg128(r); dr >= dd; dr = deg128(r)) {
             q |= 1ULL << (dr - dd);
             r ^= d << (dr - dd);
         }
@@ -568,13 +578,13 @@ void lin2(u64 w0, const Poly &x, u64 w1, const Poly &y, Poly &out) {
     trim(out);
 }
 
-// half-gcd for small inputs (deg a < HGCD_BASE): rounds of single-word
-// half-gcds on the top 64 coefficients.  A window's quotients are those
-// of the full pair while the window remainders keep at least half the
-// window's degree (the same rule as the recursion), so each round applies
-// a correct prefix of the remainder sequence, about 32 degrees long.  A
-// round that cannot step (a quotient longer than the window) takes one
-// explicit division.
+// half-gcd for small inputs (deg a < HGCD_BASE): rounds of half-gcds of
+// the top 128 coefficients.  A window's quotients are those of the full
+// pair while the window remainders keep at least half the window's degree
+// (the same rule as the recursion), so each round applies a correct prefix
+// of the remainder sequence, about 64 degrees long, whose matrix has
+// single-word entries.  A round that cannot step (a quotient longer than
+// the window) takes one explicit division.
 void hgcd_base(const Poly &a, const Poly &b, int64_t m, HMat &M, Poly &c, Poly &d) {
     M = HMat{Poly{1}, Poly(), Poly(), Poly{1}};
     c = a;
@@ -584,12 +594,12 @@ void hgcd_base(const Poly &a, const Poly &b, int64_t m, HMat &M, Poly &c, Poly &
     Poly t0, t1, t2, t3, q, r;
     while (deg(d) >= m) {
         const int64_t dc = deg(c);
-        const int64_t sh = std::max<int64_t>(0, dc - 63);
-        const u64 cw = window64(c, (u64)sh), dw = window64(d, (u64)sh);
+        const int64_t sh = std::max<int64_t>(0, dc - 127);
+        const u128 cw = window128(c, (u64)sh), dw = window128(d, (u64)sh);
         const int stop = sh == 0 ? (int)m
                                  : (int)std::max<int64_t>((dc - sh + 1) / 2, m - sh);
         int steps = 0;
-        W2 W = stop <= 63 ? hgcd_word(cw, dw, std::max(stop, 0), steps) : W2{1, 0, 0, 1};
+        W2 W = stop <= 127 ? hgcd_word(cw, dw, std::max(stop, 0), steps) : W2{1, 0, 0, 1};
         if (steps == 0) {
             divmod(c, d, q, r);
             Poly n10 = add(M.m00, mul_fast(q, M.m10));

@@ -53,10 +53,11 @@ def _bits_for_chunks(rng, c):
 
 # (2^mp, e, ca): products of N = ca + cb - 1 = 2^mp + e chunk coefficients,
 # which run on 2^mp-point transforms plus a product of the top e chunks
-# (wrapped products): e = 1, e = 2^mp / 8 (the largest wrapped), balanced
+# (wrapped products): e = 1, e = 2^mp / 4 (the largest wrapped), balanced
 # and unbalanced operands, and e one past the limit (an ordinary product)
-WRAPPED = [(11, 1, 1025), (12, 512, 2304), (13, 2, 8000), (13, 3, 6000), (16, 3000, 34268),
-           (17, 16384, 73728), (15, 4097, 18432), (14, 700, 16384)]
+WRAPPED = [(11, 1, 1025), (12, 512, 2304), (12, 1024, 2560), (13, 2, 8000), (13, 3, 6000),
+           (16, 3000, 34268), (17, 16384, 73728), (17, 32768, 81920), (15, 8193, 20480),
+           (14, 700, 16384)]
 
 
 @pytest.mark.parametrize("mp,e,ca", WRAPPED)
@@ -70,12 +71,13 @@ def test_wrapped_product_vs_oracle(eng, mp, e, ca):
     assert np.array_equal(eng.mul(b, bbits, a, abits), O.fft_mul(a, abits, b, bbits))
 
 
-@pytest.mark.parametrize("r", [530017, 589000, 600011])
+@pytest.mark.parametrize("r", [530017, 589000, 655360, 700001])
 def test_wrapped_slab_mulmod_vs_oracle(eng, r):
     """gf2t_mulmod_slab at r where the product's 2 ceil(r/32) - 1 chunk
-    coefficients lie just above 2^15 (530017: 359 over, 589000: 4045 over)
-    and, for comparison, past the wrapped range (600011), with F + x as the
-    second operand on every other call."""
+    coefficients lie just above 2^15 (530017: 359 over, 589000: 4045 over,
+    655360: 8191 over, the last wrapped) and, for comparison, past the
+    wrapped range (700001), with F + x as the second operand on every
+    other call."""
     import ctypes as C
     from mersenne_trinomial_amd._capi import check, lib
     from mersenne_trinomial_amd.device import DeviceBuffer

@@ -282,6 +282,19 @@ def test_field_multiplies(kl):
         assert kl.kl_gf64_mul(int(a), int(b)) == O.gf64_mul_bitserial(int(a), int(b))
         x, y = int(a) & 0xFFFFFFFF, int(b) & 0xFFFFFFFF
         assert kl.kl_clmul32(x, y) == O.lib().gf2o_clmul32(x, y)
+    # dense operands: the column counts that the paired multiply-adds of
+    # clmul32 rely on peak at all-ones (one dense operand and one sparse
+    # one, then both dense with a few bits cleared)
+    dense = [0xFFFFFFFF, 0xFFFFFFFE, 0x7FFFFFFF, 0xEFFFFFFF, 0xFFFF7FFF, 0x11111111,
+             0x88888888, 0xF0F0F0F0, 0x0F0F0F0F, 0xAAAAAAAA, 0x55555555, 1, 0x80000000]
+    for x in dense:
+        for y in dense:
+            assert kl.kl_clmul32(x, y) == O.lib().gf2o_clmul32(x, y), (hex(x), hex(y))
+            xx, yy = (x << 32) | x, (y << 32) | (y ^ 0xFFFFFFFF)
+            assert kl.kl_gf64_mul(xx, yy) == O.gf64_mul_bitserial(xx, yy), (hex(xx), hex(yy))
+    for a, b in rng.integers(0, 2**32, (2000, 2), dtype=np.uint64):
+        x, y = 0xFFFFFFFF ^ (1 << (int(a) & 31)), 0xFFFFFFFF & ~int(b) | 0xFFFF0FFF
+        assert kl.kl_clmul32(x, y) == O.lib().gf2o_clmul32(x, y), (hex(x), hex(y))
     for d in (2, 5, 16, 17, 31, 32):
         q = O.find_primitive_poly(d)
         for a, b in rng.integers(0, 1 << d, (300, 2), dtype=np.uint64):
