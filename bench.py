@@ -101,6 +101,8 @@ def parse():
     ap.add_argument("--scan-stagger", type=int, default=None,
                     help="parts a deep-scan cohort is staggered into (gf2t_tune scan_stagger: "
                          "0 = automatic, -1 = never)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="a gf2t_tune setting applied after the tuning file (repeatable)")
     ap.add_argument("--gcd-conc", type=int, default=8,
                     help="gcd mode: gcds run at once after the single one")
     ap.add_argument("--mode",
@@ -775,6 +777,9 @@ def apply_tuning(args):
         check(lib.gf2t_tune(b"sq_fuse", args.sq_fuse))
     if args.scan_stagger is not None:
         check(lib.gf2t_tune(b"scan_stagger", args.scan_stagger))
+    for kv in args.tune:
+        key, _, val = kv.partition("=")
+        check(lib.gf2t_tune(key.encode(), int(val)))
 
 
 def chain_launches(steps: int, fuse: bool):

@@ -81,17 +81,13 @@ struct PackArg {
     u32 lmask[PACK_LIST];
 };
 
-// W(t) for t < 2^(8 ntab): only the tables that exist are read
+// W(t) for t < 2^32: four independent table reads, no branches (the
+// tables past the ones a transform needs hold only their entry 0 = 0,
+// which is all that a t below 2^(8 ntab) reads there)
 GF2T_DEV u64 twiddle(const TwTables &tw, u64 t) {
-    u64 w = tw.t[0][t & 255];
-    if (t >> 8) {
-        w ^= tw.t[1][(t >> 8) & 255];
-        if (t >> 16) {
-            w ^= tw.t[2][(t >> 16) & 255];
-            if (t >> 24) w ^= tw.t[3][(t >> 24) & 255];
-        }
-    }
-    return w;
+    const u32 v = (u32)t;
+    return xor3(tw.t[0][v & 255], tw.t[1][(v >> 8) & 255], tw.t[2][(v >> 16) & 255]) ^
+           tw.t[3][v >> 24];
 }
 
 // the global index of tile element li (li < 2^(c+w)): its low c bits stay,
@@ -140,6 +136,7 @@ GF2T_DEV void fft_pass_tile(u64 *__restrict__ f, const PassDesc &pd, bool pack,
             }
         tw.t[k][v] = acc;
     }
+    if (pd.nbf && threadIdx.x < 4 - ntab) tw.t[3 - threadIdx.x][0] = 0;
     // load the tile (low c bits contiguous in memory)
     for (u32 li = threadIdx.x; li < tsize; li += nthr) {
         const u64 g = global_index(pd, li, gbase);
