@@ -299,6 +299,24 @@ int gf2t_square_fold_slab_g(const uint64_t *din, uint64_t *dout, size_t pitch,
 int gf2t_square2_fold_slab_g(const uint64_t *din, uint64_t *dout, size_t pitch,
                              uint64_t r, const uint64_t *ds, size_t n, int group,
                              void *stream);
+/* The same two squarings in work units (tiles one period apart that pass
+ * window copies on, as gf2t_square_fold_batch's two-squaring launches),
+ * with each group's tile layout computed on the host once: a plan holds
+ * the layouts of up to ngroups groups (s values hs[g], host memory) at one
+ * r; _set replaces them (at most the created count, e.g. after a cohort
+ * is compacted; it waits for the stream's earlier work); destroy it only
+ * when no launch that uses it is still queued.  Same result
+ * as gf2t_square2_fold_slab_g with ds = hs; n / group rounded up must
+ * not exceed the groups set.  No reference counterpart (see above). */
+typedef struct gf2t_sq2_slab_plan gf2t_sq2_slab_plan;
+int gf2t_sq2_slab_plan_create(uint64_t r, const uint64_t *hs, size_t ngroups,
+                              gf2t_sq2_slab_plan **plan);
+int gf2t_sq2_slab_plan_set(gf2t_sq2_slab_plan *plan, const uint64_t *hs, size_t ngroups,
+                           void *stream);
+int gf2t_sq2_slab_plan_destroy(gf2t_sq2_slab_plan *plan);
+int gf2t_square2_fold_slab_p(const gf2t_sq2_slab_plan *plan, const uint64_t *din,
+                             uint64_t *dout, size_t pitch, size_t n, int group,
+                             void *stream);
 /* dc_i = x^m + sum_j h_(i,j) x^(m-1-j) mod T_i = prod_t (x + x^(2^(k+1+t)))
  * (horner_C: k_horner_fused + k_horner_tail); 2m < r. */
 int gf2t_horner_slab(const uint64_t *dh, size_t slot_pitch, int m, uint64_t r,

@@ -122,6 +122,10 @@ _SIGS = {
     "gf2t_ddf_init_slab": (C.c_int, [_vp, _vp, _sz, C.c_int, _u64, _vp, _u64, C.c_int, _vp]),
     "gf2t_square_fold_slab_g": (C.c_int, [_vp, _vp, _sz, _u64, _vp, _sz, C.c_int, _vp]),
     "gf2t_square2_fold_slab_g": (C.c_int, [_vp, _vp, _sz, _u64, _vp, _sz, C.c_int, _vp]),
+    "gf2t_sq2_slab_plan_create": (C.c_int, [_u64, _p64, _sz, C.POINTER(_vp)]),
+    "gf2t_sq2_slab_plan_set": (C.c_int, [_vp, _p64, _sz, _vp]),
+    "gf2t_sq2_slab_plan_destroy": (C.c_int, [_vp]),
+    "gf2t_square2_fold_slab_p": (C.c_int, [_vp, _vp, _vp, _sz, _sz, C.c_int, _vp]),
     "gf2t_horner_slab": (C.c_int, [_vp, _sz, C.c_int, _u64, _vp, _vp, _sz, C.c_int, _vp]),
     "gf2t_mulx_xor_slab": (C.c_int, [_vp, _sz, _vp, _sz, _u64, _vp, C.c_int, _vp]),
     "gf2t_xor_word0_slab": (C.c_int, [_vp, _sz, _u64, C.c_int, _vp]),

@@ -152,13 +152,22 @@ def tune_block_m(r: int, reps: int = 3):
     for i in range(n):                          # valid residues (bits >= r clear)
         check(lib.gf2t_set_x(a.word_ptr(i * pitch), nw, None))
         check(lib.gf2t_set_x(b.word_ptr(i * pitch), nw, None))
-    s = DeviceBuffer.from_array(np.array([(r - 1) // 3 - i for i in range(n)], dtype=np.uint64))
+    hs = np.array([(r - 1) // 3 - i for i in range(n)], dtype=np.uint64)
+    s = DeviceBuffer.from_array(hs)
     work = DeviceBuffer(lib.gf2t_mulmod_slab_workspace_bytes(r, n))
     # per squaring, in the launches the deep scan uses for this r: two
-    # squarings per launch from 2^18 bits (when sq_fuse is on), else one
+    # squarings per launch in work units from 2^18 bits (when sq_fuse is
+    # on), else one
     if r >= (1 << 18) and tune_get("sq_fuse"):
-        t_sq = _time_ms(lambda: check(lib.gf2t_square2_fold_slab_g(a.ptr, o.ptr, pitch, r, s.ptr,
-                                                                    n, 1, None)), 8 * reps) / (2 * n)
+        plan = C.c_void_p()
+        check(lib.gf2t_sq2_slab_plan_create(r, hs.ctypes.data_as(P64), n, C.byref(plan)))
+        try:
+            t_sq = _time_ms(lambda: check(lib.gf2t_square2_fold_slab_p(plan, a.ptr, o.ptr, pitch,
+                                                                        n, 1, None)),
+                            8 * reps) / (2 * n)
+        finally:
+            synchronize()
+            check(lib.gf2t_sq2_slab_plan_destroy(plan))
     else:
         t_sq = _time_ms(lambda: check(lib.gf2t_square_fold_slab_g(a.ptr, o.ptr, pitch, r, s.ptr,
                                                                    n, 1, None)), 8 * reps) / n

@@ -153,6 +153,13 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     std::vector<u64> svals(B);
     for (int i = 0; i < B; i++) svals[i] = live[i].s;
     SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, st));
+    // the slots' tile layouts for the two-squaring work units (r >= 2^18)
+    gf2t_sq2_slab_plan *sq2 = nullptr;
+    if (r >= (1u << 18)) SCAN_CHECK(gf2t_sq2_slab_plan_create(r, svals.data(), (size_t)B, &sq2));
+    struct PlanGuard {   // destroyed after the stream is synchronized (declared before it)
+        gf2t_sq2_slab_plan *p;
+        ~PlanGuard() { gf2t_sq2_slab_plan_destroy(p); }
+    } plan_guard{sq2};
     uint64_t *cur = T0.w(), *oth = T1.w();
     if (sg) sg->wait(turn);   // staggered: the previous part has its first products
     double t_dev = 0, t_gcd = 0, t_res = 0, t0 = now_s();
@@ -175,7 +182,7 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
         std::vector<u64> host((size_t)B * pitch, 0);
         for (int i = 0; i < B; i++) host[(size_t)i * pitch] = 2;   // F = x
         SCAN_CHECK(gf2t_memcpy_h2d(cur, host.data(), host.size() * 8, st));
-        SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)B, 1, P.k0, st));
+        SCAN_CHECK(slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)B, 1, P.k0, st, sq2));
     }
     // One block of the interval on the first nslot slots: m degrees of the
     // blocked scheme (Horner product C, ACC *= C, m squarings of the
@@ -188,7 +195,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             SCAN_CHECK(gf2t_horner_slab(c, pitch, m, r, SV.w(), C.w(), pitch, nslot, st));
             SCAN_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(), pitch,
                                         nslot, WORK.p, st));
-            SCAN_CHECK(slab_square_steps(c, o, pitch, r, SV.w(), (size_t)nslot * m, m, (u64)m, st));
+            SCAN_CHECK(
+                slab_square_steps(c, o, pitch, r, SV.w(), (size_t)nslot * m, m, (u64)m, st, sq2));
         } else {
             SCAN_CHECK(gf2t_square_fold_slab(c, o, pitch, r, SV.w(), nslot, st));
             std::swap(c, o);
@@ -349,8 +357,10 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
             }
             nl++;
         }
-        if (nl != n && nl > 0)
+        if (nl != n && nl > 0) {
             SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)nl * 8, st));
+            if (sq2) SCAN_CHECK(gf2t_sq2_slab_plan_set(sq2, svals.data(), (size_t)nl, st));
+        }
         if (P.verbose)
             fprintf(stderr, "  interval (%llu,%llu]: %d live, %zu hits, %d continue"
                     " (m=%d; device wait %.3fs, gcd %.3fs, resolve %.3fs so far)\n",

@@ -115,14 +115,17 @@ inline bool resolve_hit(const Poly &g, u64 ka, u64 kb, u64 rhigh, gf2t_scan_resu
 // Advances a slab of n residues (residue y uses ds[y / group]) by `steps`
 // squarings, ping-ponging between cur and oth (cur ends holding the
 // result).  Two squarings per launch when the residues are large enough
-// for the two-squaring tiles to be full (r >= 2^18).
+// for the two-squaring tiles to be full (r >= 2^18): in work units when a
+// plan with the groups' layouts is given, else in plain tiles.
 inline int slab_square_steps(u64 *&cur, u64 *&oth, size_t pitch, u64 r, const u64 *ds, size_t n,
-                             int group, u64 steps, void *stream) {
+                             int group, u64 steps, void *stream,
+                             const gf2t_sq2_slab_plan *plan = nullptr) {
     const bool two = r >= (1u << 18) && gf2t_host::square_fuse_enabled();
     for (u64 t = 0; t < steps;) {
         int rc;
         if (two && steps - t >= 2) {
-            rc = gf2t_square2_fold_slab_g(cur, oth, pitch, r, ds, n, group, stream);
+            rc = plan ? gf2t_square2_fold_slab_p(plan, cur, oth, pitch, n, group, stream)
+                      : gf2t_square2_fold_slab_g(cur, oth, pitch, r, ds, n, group, stream);
             t += 2;
         } else {
             rc = gf2t_square_fold_slab_g(cur, oth, pitch, r, ds, n, group, stream);

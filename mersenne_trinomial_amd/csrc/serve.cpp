@@ -54,6 +54,10 @@ struct gf2t_server {
     uint64_t *cur = nullptr, *oth = nullptr;
     std::vector<Slot> live;   // slot i of the slabs
     std::vector<u64> svals;   // s of slot i (mirrors SV)
+    // the slots' two-squaring tile layouts (r >= 2^18), reset from svals
+    // before a round when they changed
+    gf2t_sq2_slab_plan *sq2 = nullptr;
+    bool sq2_stale = true;
     std::deque<Slot> queue;
     std::deque<Done> done;
     std::mutex mu;            // one caller at a time
@@ -85,7 +89,15 @@ struct gf2t_server {
         SV_CHECK(WORK.alloc(gf2t_mulmod_slab_workspace_bytes(r, (int)cap)));
         cur = T0.w();
         oth = T1.w();
+        if (r >= (1u << 18)) {
+            const std::vector<u64> one(cap, 1);
+            SV_CHECK(gf2t_sq2_slab_plan_create(r, one.data(), cap, &sq2));
+        }
         return GF2T_OK;
+    }
+    ~gf2t_server() {
+        (void)gf2t_stream_sync(nullptr);
+        gf2t_sq2_slab_plan_destroy(sq2);
     }
 
     void finish(const Slot &q, const gf2t_scan_result &R, const std::string &hex) {
@@ -148,6 +160,7 @@ struct gf2t_server {
             queue.pop_front();
         }
         SV_CHECK(gf2t_memcpy_h2d(SV.w() + n, svals.data() + n, add * 8, nullptr));
+        sq2_stale = true;
         size_t i = n;
         while (i < n + add) {
             size_t j = i;
@@ -213,6 +226,7 @@ struct gf2t_server {
             live.resize(nl);
             svals.resize(nl);
             if (nl) SV_CHECK(gf2t_memcpy_h2d(SV.w(), svals.data(), nl * 8, nullptr));
+            sq2_stale = true;
         }
         return GF2T_OK;
     }
@@ -224,8 +238,12 @@ struct gf2t_server {
         SV_CHECK(gf2t_horner_slab(cur, pitch, m, r, SV.w(), C.w(), pitch, n, nullptr));
         SV_CHECK(gf2t_mulmod_slab(ACC.w(), pitch, C.w(), pitch, 0, r, SV.w(), ACC.w(), pitch, n,
                                   WORK.p, nullptr));
+        if (sq2 && sq2_stale) {
+            SV_CHECK(gf2t_sq2_slab_plan_set(sq2, svals.data(), (size_t)n, nullptr));
+            sq2_stale = false;
+        }
         SV_CHECK(gf2t_scan::slab_square_steps(cur, oth, pitch, r, SV.w(), (size_t)n * m, m,
-                                              (u64)m, nullptr));
+                                              (u64)m, nullptr, sq2));
         for (auto &q : live) q.kdone += (u64)m;
         return GF2T_OK;
     }

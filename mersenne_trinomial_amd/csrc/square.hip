@@ -252,6 +252,24 @@ __global__ __launch_bounds__(SQ_THREADS) void k_square2_fold_slab_g(
     square2_tile<false, W12>(in + (u64)y * pitch, out + (u64)y * pitch, c, blockIdx.x, gw);
 }
 
+// The same two squarings of a slab in work units (as the batched kernel):
+// desc[g] holds group g's constants and tile layout, computed on the host
+// once per cohort (gf2t_sq2_slab_plan); blockIdx.x is the work unit.
+struct Sq2SlabDesc {
+    SqConst c;
+    Sq2Chain h;
+};
+template <bool NT, bool W12>
+__global__ __launch_bounds__(SQ_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void
+k_square2_fold_slab_u(const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch,
+                      const Sq2SlabDesc *__restrict__ desc, unsigned n, unsigned group) {
+    __shared__ __align__(16) u64 gw[SQ2_WINS * SQ2_LEN];
+    const unsigned y = blockIdx.z * gridDim.y + blockIdx.y;
+    if (y >= n) return;   // whole block
+    const Sq2SlabDesc d = desc[y / group];
+    square2_unit<NT, W12>(in + (u64)y * pitch, out + (u64)y * pitch, d.c, d.h, blockIdx.x, gw);
+}
+
 // slab: residue y at in + y*pitch, its s in svec[y]
 __global__ __launch_bounds__(SQ_THREADS) void k_square_fold_slab(
         const u64 *__restrict__ in, u64 *__restrict__ out, u64 pitch, u64 r,
@@ -782,5 +800,108 @@ extern "C" int gf2t_square2_fold_slab_g(const uint64_t *din, uint64_t *dout, siz
                            (hipStream_t)stream, din, dout, (u64)pitch, r, ds, (unsigned)n,
                            (unsigned)group);
     GF2T_CHECK_LAUNCH("k_square2_fold_slab_g");
+    return GF2T_OK;
+}
+
+// ---- two squarings of a slab in work units, with per-group layouts
+struct gf2t_sq2_slab_plan {
+    u64 r = 0;
+    size_t cap = 0, ngroups = 0;
+    unsigned gx = 0;   // work units of the group with the most
+    Sq2SlabDesc *d = nullptr;
+    std::vector<Sq2SlabDesc> h;   // host copy (source of the last upload)
+};
+
+extern "C" int gf2t_sq2_slab_plan_set(gf2t_sq2_slab_plan *p, const uint64_t *hs, size_t ngroups,
+                                      void *stream) {
+    if (!p || !hs || ngroups == 0 || ngroups > p->cap) {
+        gf2t::set_error("gf2t_sq2_slab_plan_set: bad arguments (at most %zu groups)",
+                        p ? p->cap : (size_t)0);
+        return GF2T_EINVAL;
+    }
+    for (size_t g = 0; g < ngroups; g++)
+        if (!valid_rs(p->r, hs[g])) {
+            gf2t::set_error("gf2t_sq2_slab_plan_set: group %zu has bad s", g);
+            return GF2T_EINVAL;
+        }
+    // the previous upload may still be in flight from h: let it finish
+    // before h changes
+    GF2T_HIP(hipStreamSynchronize((hipStream_t)stream));
+    unsigned gx = 0;
+    for (size_t g = 0; g < ngroups; g++) {
+        const SqConst c = sq_const(p->r, hs[g]);
+        p->h[g] = Sq2SlabDesc{c, sq2_chain(c)};
+        gx = std::max(gx, (unsigned)sq2_units(p->h[g].h));
+    }
+    GF2T_HIP(hipMemcpyAsync(p->d, p->h.data(), ngroups * sizeof(Sq2SlabDesc),
+                            hipMemcpyHostToDevice, (hipStream_t)stream));
+    p->ngroups = ngroups;
+    p->gx = gx;
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_sq2_slab_plan_create(uint64_t r, const uint64_t *hs, size_t ngroups,
+                                         gf2t_sq2_slab_plan **plan) {
+    if (!plan || !hs || ngroups == 0 || r < 5 || r >= (1ULL << 34)) {
+        gf2t::set_error("gf2t_sq2_slab_plan_create: bad arguments");
+        return GF2T_EINVAL;
+    }
+    *plan = nullptr;
+    gf2t_sq2_slab_plan *p = new gf2t_sq2_slab_plan;
+    p->r = r;
+    p->cap = ngroups;
+    p->h.resize(ngroups);
+    hipError_t e = hipMalloc((void **)&p->d, ngroups * sizeof(Sq2SlabDesc));
+    if (e != hipSuccess) {
+        delete p;
+        return gf2t::hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    int rc = gf2t_sq2_slab_plan_set(p, hs, ngroups, nullptr);
+    if (rc == GF2T_OK) rc = hipStreamSynchronize(nullptr) == hipSuccess ? GF2T_OK : GF2T_EHIP;
+    if (rc) {
+        (void)hipFree(p->d);
+        delete p;
+        return rc;
+    }
+    *plan = p;
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_sq2_slab_plan_destroy(gf2t_sq2_slab_plan *p) {
+    if (!p) return GF2T_OK;
+    if (p->d) (void)hipFree(p->d);
+    delete p;
+    return GF2T_OK;
+}
+
+extern "C" int gf2t_square2_fold_slab_p(const gf2t_sq2_slab_plan *p, const uint64_t *din,
+                                        uint64_t *dout, size_t pitch, size_t n, int group,
+                                        void *stream) {
+    if (!p || !din || !dout || din == dout || group < 1 || pitch < words_of(p->r) ||
+        (pitch & 1) || (((uintptr_t)din | (uintptr_t)dout) & 15) || n > 0xFFFFFFFFu ||
+        (n + (size_t)group - 1) / (size_t)group > p->ngroups) {
+        gf2t::set_error("gf2t_square2_fold_slab_p: bad arguments (even pitch, 16-byte aligned "
+                        "slabs, n / group within the plan's groups)");
+        return GF2T_EINVAL;
+    }
+    if (n == 0) return GF2T_OK;
+    const unsigned gy = n < 65535 ? (unsigned)n : 65535u;
+    const unsigned gz = (unsigned)((n + gy - 1) / gy);
+    const dim3 grid(p->gx, gy, gz);
+    const bool w12 = sq_w12_ok(p->r), nt = sq_nt();
+    hipStream_t st = (hipStream_t)stream;
+    if (nt && w12)
+        hipLaunchKernelGGL((k_square2_fold_slab_u<true, true>), grid, dim3(SQ_THREADS), 0, st, din,
+                           dout, (u64)pitch, p->d, (unsigned)n, (unsigned)group);
+    else if (nt)
+        hipLaunchKernelGGL((k_square2_fold_slab_u<true, false>), grid, dim3(SQ_THREADS), 0, st,
+                           din, dout, (u64)pitch, p->d, (unsigned)n, (unsigned)group);
+    else if (w12)
+        hipLaunchKernelGGL((k_square2_fold_slab_u<false, true>), grid, dim3(SQ_THREADS), 0, st,
+                           din, dout, (u64)pitch, p->d, (unsigned)n, (unsigned)group);
+    else
+        hipLaunchKernelGGL((k_square2_fold_slab_u<false, false>), grid, dim3(SQ_THREADS), 0, st,
+                           din, dout, (u64)pitch, p->d, (unsigned)n, (unsigned)group);
+    GF2T_CHECK_LAUNCH("k_square2_fold_slab_u");
     return GF2T_OK;
 }

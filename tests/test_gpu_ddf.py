@@ -102,6 +102,34 @@ def test_two_squarings_on_a_slab(dev, r, group):
         assert np.array_equal(got[y, :nw], O.square_fold_steps(a[y], r, s, 2)), (y, s)
     # odd pitch: rejected (the kernel stores 16-byte pairs)
     assert lib.gf2t_square2_fold_slab_g(A.ptr, B.ptr, pitch + 1, r, S.ptr, 1, 1, None) != 0
+    if r < 5:
+        return
+    # the work-unit form with the groups' layouts in a plan: the same
+    # result; then the plan reset to the groups in reverse order (as after
+    # a compaction) and a slab of fewer groups
+    import ctypes as C
+    from mersenne_trinomial_amd._capi import _p64
+    hs = np.array(svals, np.uint64)
+    plan = C.c_void_p()
+    assert lib.gf2t_sq2_slab_plan_create(r, hs.ctypes.data_as(_p64), len(svals),
+                                         C.byref(plan)) == 0
+    try:
+        Bp = device.DeviceBuffer.words(n * pitch)
+        assert lib.gf2t_square2_fold_slab_p(plan, A.ptr, Bp.ptr, pitch, n, group, None) == 0
+        assert np.array_equal(Bp.download().reshape(n, pitch)[:, :nw], got[:, :nw])
+        rev = hs[::-1].copy()
+        assert lib.gf2t_sq2_slab_plan_set(plan, rev.ctypes.data_as(_p64), len(rev), None) == 0
+        n2 = n - group
+        assert lib.gf2t_square2_fold_slab_p(plan, A.ptr, Bp.ptr, pitch, n2, group, None) == 0
+        got2 = Bp.download().reshape(n, pitch)
+        for y in range(n2):
+            s = int(rev[y // group])
+            assert np.array_equal(got2[y, :nw], O.square_fold_steps(a[y], r, s, 2)), (y, s)
+        # more residues than the plan's groups cover: rejected
+        assert lib.gf2t_square2_fold_slab_p(plan, A.ptr, Bp.ptr, pitch, n + 1, group, None) != 0
+        device.synchronize()
+    finally:
+        assert lib.gf2t_sq2_slab_plan_destroy(plan) == 0
 
 
 def test_slab_helpers(dev):
