@@ -766,7 +766,11 @@ def mode_serve(args):
 
 
 def apply_tuning(args):
+    """Command-line knobs, applied after gf2t_init has loaded the tuning
+    file (which would otherwise override the knobs it lists)."""
+    from mersenne_trinomial_amd import device
     from mersenne_trinomial_amd._capi import check, lib
+    device.init(0)
     if args.sq_ppt:
         check(lib.gf2t_tune(b"sq_ppt", args.sq_ppt))
     if args.sq_order >= 0:
