@@ -130,6 +130,10 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  *              least this many bits; 0 = host Karatsuba only
  *   "fft_ept"  elements of a transform tile per thread (2, 4, 8, 16;
  *              default 2): fewer threads per tile
+ *   "fft_low_bits" low index bits kept local in every transform pass,
+ *              i.e. contiguous elements per HBM run of a tile (4, 5 or 6;
+ *              default 4): longer runs, but fewer window bits per pass
+ *              and so more passes.  Plans made before keep theirs
  *   "hm_zc_words" products of the host-buffer service (gf2t_mat2n_host,
  *              gf2t_mat2_host) whose operands and results take at most
  *              this many words together are read and written by the GPU

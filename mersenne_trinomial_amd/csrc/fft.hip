@@ -38,7 +38,7 @@ constexpr int FFT_MAX_OPS = 64;
 constexpr int FFT_THREADS = 512;    // most threads per block (a tile of 2^T elements uses 2^(T-1))
 constexpr int FFT_TILE_BITS = 13;   // largest tile: 8192 elements = 64 KiB of LDS
 constexpr int FFT_MIN_TILE_BITS = 6;
-constexpr int FFT_LOW_BITS = 4;     // 16 contiguous elements = 128 bytes
+constexpr int FFT_LOW_BITS = 4;     // default: 16 contiguous elements = 128 bytes
 constexpr int PACK_LIST = 12;   // four matrix entries + four vector pairs
 
 enum : uint8_t { OP_TF = 0, OP_TI = 1, OP_BF = 2, OP_BI = 3 };
@@ -304,11 +304,12 @@ std::vector<Op> op_sequence(int m, bool forward) {
 
 // Pack the op sequence of a 2^m-point transform into passes over tiles of
 // 2^min(m, tile_bits) elements.
-std::vector<PassDesc> schedule(int m, bool forward, int tile_bits) {
+std::vector<PassDesc> schedule(int m, bool forward, int tile_bits, int low_bits) {
     std::vector<Op> rem = op_sequence(m, forward);
     std::vector<PassDesc> passes;
     int T = std::min(m, tile_bits);
-    int c = std::min(FFT_LOW_BITS, T);
+    // at least two window bits, so that every Taylor step fits some window
+    int c = std::min(low_bits, std::max(T - 2, 1));
     int w = T - c;
     while (!rem.empty()) {
         std::vector<Window> cands;
@@ -437,7 +438,7 @@ struct gf2t_fft_plan {
 namespace {
 constexpr int FFT_MAX_M = 27;
 std::mutex g_plan_mu;
-std::map<int, gf2t_fft_plan *> g_plans;   // key m * 32 + T
+std::map<int, gf2t_fft_plan *> g_plans;   // key (m * 32 + T) * 8 + low bits
 
 // Tile bits per transform size: whole 8192-element tiles from 2^19 points
 // up; below that smaller tiles, so that a transform still spreads over
@@ -462,6 +463,10 @@ int tile_bits_for(int m) {
 // FFT_THREADS.  Fewer threads per tile leave room for more blocks of
 // other launches on a CU (concurrent products of small transforms).
 std::atomic<int> g_fft_ept{2};
+
+// Low index bits of every pass's tile (gf2t_tune "fft_low_bits", 4..6):
+// runs of 2^c contiguous elements per HBM access of a tile
+std::atomic<int> g_fft_low{FFT_LOW_BITS};
 unsigned threads_for_tile(int T) {
     const u64 ept = (u64)g_fft_ept.load(std::memory_order_relaxed);
     return (unsigned)std::max<u64>(64, std::min<u64>(FFT_THREADS, (1ULL << (T > 0 ? T : 0)) / ept));
@@ -469,11 +474,13 @@ unsigned threads_for_tile(int T) {
 
 const gf2t_fft_plan *cached_plan_T(int m, int T) {
     T = std::max(FFT_MIN_TILE_BITS, std::min(T, FFT_TILE_BITS));
+    const int c = g_fft_low.load(std::memory_order_relaxed);
+    const int key = (m * 32 + T) * 8 + c;
     std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plans.find(m * 32 + T);
+    auto it = g_plans.find(key);
     if (it != g_plans.end()) return it->second;
-    gf2t_fft_plan *p = new gf2t_fft_plan{m, T, schedule(m, true, T), schedule(m, false, T)};
-    g_plans[m * 32 + T] = p;
+    gf2t_fft_plan *p = new gf2t_fft_plan{m, T, schedule(m, true, T, c), schedule(m, false, T, c)};
+    g_plans[key] = p;
     return p;
 }
 const gf2t_fft_plan *cached_plan(int m) { return cached_plan_T(m, tile_bits_for(m)); }
@@ -551,6 +558,15 @@ int gf2t_fft_set_ept(int64_t ept) {
     return GF2T_OK;
 }
 int gf2t_fft_ept() { return g_fft_ept.load(); }
+int gf2t_fft_set_low_bits(int64_t c) {
+    if (c < 4 || c > 6) {
+        gf2t::set_error("gf2t_tune: fft_low_bits needs 4, 5 or 6");
+        return GF2T_EINVAL;
+    }
+    g_fft_low.store((int)c);
+    return GF2T_OK;
+}
+int gf2t_fft_low_bits() { return g_fft_low.load(); }
 
 extern "C" int gf2t_cantor_basis(uint64_t *hbasis, int *len) {
     build_basis_host();
@@ -570,7 +586,8 @@ extern "C" int gf2t_fft_plan_create(int m, gf2t_fft_plan **plan) {
         return GF2T_EINTERNAL;
     }
     const int T = std::max(FFT_MIN_TILE_BITS, std::min(tile_bits_for(m), FFT_TILE_BITS));
-    *plan = new gf2t_fft_plan{m, T, schedule(m, true, T), schedule(m, false, T)};
+    const int c = g_fft_low.load(std::memory_order_relaxed);
+    *plan = new gf2t_fft_plan{m, T, schedule(m, true, T, c), schedule(m, false, T, c)};
     return GF2T_OK;
 }
 

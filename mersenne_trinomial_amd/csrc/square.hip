@@ -468,6 +468,8 @@ int gf2t_fft_set_tile_bits(int m, int64_t T);   // fft.hip
 int gf2t_fft_tile_bits(int m);
 int gf2t_fft_set_ept(int64_t ept);                // fft.hip
 int gf2t_fft_ept();
+int gf2t_fft_set_low_bits(int64_t c);             // fft.hip
+int gf2t_fft_low_bits();
 int gf2t_hm_set_zc_words(int64_t words);          // fft.hip
 int64_t gf2t_hm_zc_words();
 int gf2t_sieve_tune(const char *key, int64_t v);   // sieve.hip
@@ -498,6 +500,7 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
         return GF2T_OK;
     }
     if (!strcmp(key, "fft_ept")) return gf2t_fft_set_ept(value);
+    if (!strcmp(key, "fft_low_bits")) return gf2t_fft_set_low_bits(value);
     if (!strcmp(key, "hm_zc_words")) return gf2t_hm_set_zc_words(value);
     if (!strcmp(key, "scan_stagger")) {
         if (value < -1 || value > 64) {
@@ -525,6 +528,10 @@ extern "C" int gf2t_tune_get(const char *key, int64_t *value) {
     }
     if (!strcmp(key, "fft_ept")) {
         *value = gf2t_fft_ept();
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "fft_low_bits")) {
+        *value = gf2t_fft_low_bits();
         return GF2T_OK;
     }
     if (!strcmp(key, "hm_zc_words")) {

@@ -173,8 +173,9 @@ def test_trin_gpu_interval_product(eng):
 def test_every_tile_size_bit_identical(eng, m):
     """The transform is the same whatever tile the passes use: every tile
     size from 2^6 to 2^13 elements (gf2t_tune "fft_tile_<m>"), with 2, 8
-    and 16 elements per thread (gf2t_tune "fft_ept"), forward against the
-    oracle and back, and a product through the same plans."""
+    and 16 elements per thread (gf2t_tune "fft_ept") and 4, 5 and 6
+    contiguous low index bits per tile (gf2t_tune "fft_low_bits"), forward
+    against the oracle and back, and a product through the same plans."""
     from mersenne_trinomial_amd._capi import lib
     import ctypes as C
     rng = np.random.default_rng(100 + m)
@@ -183,22 +184,26 @@ def test_every_tile_size_bit_identical(eng, m):
     bits = (1 << (m + 4)) - 77         # a product that needs 2^m points
     a, b = rand_bits(rng, bits), rand_bits(rng, 29)
     pwant = O.fft_mul(a, bits, b, 29)
-    old, old_ept = C.c_int64(), C.c_int64()
+    old, old_ept, old_low = C.c_int64(), C.c_int64(), C.c_int64()
     assert lib.gf2t_tune_get(f"fft_tile_{m}".encode(), C.byref(old)) == 0
     assert lib.gf2t_tune_get(b"fft_ept", C.byref(old_ept)) == 0
+    assert lib.gf2t_tune_get(b"fft_low_bits", C.byref(old_low)) == 0
     try:
-        for ept in (2, 8, 16):
+        for ept, low in ((2, 4), (8, 5), (16, 6)):
             assert lib.gf2t_tune(b"fft_ept", ept) == 0
+            assert lib.gf2t_tune(b"fft_low_bits", low) == 0
             for T in range(6, 14):
-                print(f"m={m} tile 2^{T}, {ept} elements per thread", flush=True)
+                print(f"m={m} tile 2^{T}, {ept} elements per thread, {low} low bits", flush=True)
                 assert lib.gf2t_tune(f"fft_tile_{m}".encode(), T) == 0
                 F = eng.fft_fwd(f, m)
-                assert np.array_equal(F, want), (T, ept)
-                assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f), (T, ept)
-                assert np.array_equal(eng.mul(a, bits, b, 29), pwant), (T, ept)
+                assert np.array_equal(F, want), (T, ept, low)
+                assert np.array_equal(eng.fft_fwd(F, m, inverse=True), f), (T, ept, low)
+                assert np.array_equal(eng.mul(a, bits, b, 29), pwant), (T, ept, low)
     finally:
         assert lib.gf2t_tune(f"fft_tile_{m}".encode(), old.value) == 0
         assert lib.gf2t_tune(b"fft_ept", old_ept.value) == 0
+        assert lib.gf2t_tune(b"fft_low_bits", old_low.value) == 0
     assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 5) != 0
     assert lib.gf2t_tune(b"fft_ept", 3) != 0
+    assert lib.gf2t_tune(b"fft_low_bits", 3) != 0 and lib.gf2t_tune(b"fft_low_bits", 7) != 0
     assert lib.gf2t_tune(f"fft_tile_{m}".encode(), 14) != 0
