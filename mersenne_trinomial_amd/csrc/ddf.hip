@@ -16,6 +16,7 @@
 #include "gf2t_internal.hpp"
 #include "host_poly.hpp"
 #include "../../include/gf2t.h"
+#include <vector>
 
 namespace {
 
@@ -255,29 +256,36 @@ extern "C" int gf2t_ddf_init_slab(uint64_t *dh, uint64_t *dh_alt, size_t slot_pi
     const dim3 gp(blocks_for((nw + 1) / 2, DDF_THREADS), grid_y((unsigned)n));
     int rc = gf2t_memset(dh, 0, rp * (size_t)n * 8, stream);
     if (rc) return rc;
+    // slot j's current value lives in dh or in the same slot of dh_alt
+    // (alt[j]): a squaring writes the other table, so no copies back
+    std::vector<char> alt(m, 0);
+    auto slot = [&](int j) { return (alt[j] ? dh_alt : dh) + (size_t)j * slot_pitch; };
     for (int j = 1; j <= m; j++) {
         for (int i = j - 1; i >= 0; i--) {
             if (i > 0) {
-                // h[i-1] <- h[i-1]^2 (through the same slot of dh_alt), then
-                // h[i] ^= x h[i-1]
-                u64 *prev = dh + (size_t)(i - 1) * slot_pitch, *cur = dh + (size_t)i * slot_pitch;
-                u64 *tmp = dh_alt + (size_t)(i - 1) * slot_pitch;
+                // h[i-1] <- h[i-1]^2, then h[i] ^= x h[i-1]
+                u64 *prev = slot(i - 1);
+                alt[i - 1] ^= 1;
+                u64 *sq = slot(i - 1);
                 if (sq_w12_ok(r))
                     hipLaunchKernelGGL(k_square_fold_slab_g<true>, gp, dim3(DDF_THREADS), 0, st,
-                                       prev, tmp, (u64)rp, r, ds, (unsigned)n, 1u);
+                                       prev, sq, (u64)rp, r, ds, (unsigned)n, 1u);
                 else
                     hipLaunchKernelGGL(k_square_fold_slab_g<false>, gp, dim3(DDF_THREADS), 0, st,
-                                       prev, tmp, (u64)rp, r, ds, (unsigned)n, 1u);
-                hipLaunchKernelGGL(k_copy_slab, gw, dim3(DDF_THREADS), 0, st, prev, (u64)rp, tmp,
-                                   (u64)rp, nw, (unsigned)n);
-                hipLaunchKernelGGL(k_mulx_xor_slab, gw, dim3(DDF_THREADS), 0, st, cur, (u64)rp,
-                                   prev, (u64)rp, nw, r, ds, (unsigned)n);
+                                       prev, sq, (u64)rp, r, ds, (unsigned)n, 1u);
+                hipLaunchKernelGGL(k_mulx_xor_slab, gw, dim3(DDF_THREADS), 0, st, slot(i), (u64)rp,
+                                   sq, (u64)rp, nw, r, ds, (unsigned)n);
             } else {
                 hipLaunchKernelGGL(k_xor_word0_slab, dim3(blocks_for((size_t)n, 256)), dim3(256),
-                                   0, st, dh, (u64)rp, 2ULL, (unsigned)n);
+                                   0, st, slot(0), (u64)rp, 2ULL, (unsigned)n);
             }
         }
     }
+    for (int j = 0; j < m; j++)   // the slots whose value ended in dh_alt
+        if (alt[j])
+            hipLaunchKernelGGL(k_copy_slab, gw, dim3(DDF_THREADS), 0, st,
+                               dh + (size_t)j * slot_pitch, (u64)rp,
+                               dh_alt + (size_t)j * slot_pitch, (u64)rp, nw, (unsigned)n);
     GF2T_CHECK_LAUNCH("gf2t_ddf_init_slab");
     u64 *cur = dh, *oth = dh_alt;
     const bool two = r >= (1u << 18) && gf2t_host::square_fuse_enabled();
