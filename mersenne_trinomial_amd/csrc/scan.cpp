@@ -155,7 +155,8 @@ int run_cohort(u64 r, const gf2t_scan_params &P, u64 rhigh, bool limited,
     SCAN_CHECK(gf2t_memcpy_h2d(SV.p, svals.data(), (size_t)B * 8, st));
     // the slots' tile layouts for the two-squaring work units (r >= 2^18)
     gf2t_sq2_slab_plan *sq2 = nullptr;
-    if (r >= (1u << 18)) SCAN_CHECK(gf2t_sq2_slab_plan_create(r, svals.data(), (size_t)B, &sq2));
+    if (r >= (1u << 18) && B > 0)
+        SCAN_CHECK(gf2t_sq2_slab_plan_create(r, svals.data(), (size_t)B, &sq2));
     struct PlanGuard {   // destroyed after the stream is synchronized (declared before it)
         gf2t_sq2_slab_plan *p;
         ~PlanGuard() { gf2t_sq2_slab_plan_destroy(p); }
