@@ -530,26 +530,35 @@ struct W2 {
 // deg(c) - stop, so both fit in one word when stop >= 64 (or deg c0 <=
 // 63 + stop).
 W2 hgcd_word(u128 c, u128 d, int stop, int &steps) {
-    W2 W{1, 0, 0, 1};
+    // rows of the step matrix: (m00, m01) for c, (m10, m11) for d; the
+    // quotient is applied one set bit at a time (the new d row is the c
+    // row plus x^k times the d row for each bit k of q), so no carry-less
+    // multiply is needed, and the degrees are carried from step to step
+    u64 m00 = 1, m01 = 0, m10 = 0, m11 = 1;
     steps = 0;
-    while (d && deg128(d) >= stop) {
-        const int dd = deg128(d);
-        u64 q = 0;
+    int dc = deg128(c), dd = deg128(d);
+    while (dd >= stop && dd >= 0) {
         u128 r = c;
-        for (int dr = deg128(r); dr >= dd; dr = deg128(r)) {
-            q |= 1ULL << (dr - dd);
-            r ^= d << (dr - dd);
+        u64 n10 = m00, n11 = m01;
+        int dr = dc;
+        while (dr >= dd) {
+            const int k = dr - dd;
+            r ^= d << k;
+            n10 ^= m10 << k;
+            n11 ^= m11 << k;
+            dr = deg128(r);
         }
-        u64 n10 = W.m00 ^ clmul_lo(q, W.m10), n11 = W.m01 ^ clmul_lo(q, W.m11);
-        W.m00 = W.m10;
-        W.m01 = W.m11;
-        W.m10 = n10;
-        W.m11 = n11;
+        m00 = m10;
+        m01 = m11;
+        m10 = n10;
+        m11 = n11;
         c = d;
+        dc = dd;
         d = r;
+        dd = dr;
         steps++;
     }
-    return W;
+    return W2{m00, m01, m10, m11};
 }
 
 // out = w0 x + w1 y (word polynomials times multiword ones), one pass:
