@@ -529,14 +529,84 @@ struct W2 {
 // deg(c0) - deg(current c) <= 127 - stop, and each quotient at most
 // deg(c) - stop, so both fit in one word when stop >= 64 (or deg c0 <=
 // 63 + stop).
-W2 hgcd_word(u128 c, u128 d, int stop, int &steps) {
-    // rows of the step matrix: (m00, m01) for c, (m10, m11) for d; the
-    // quotient is applied one set bit at a time (the new d row is the c
-    // row plus x^k times the d row for each bit k of q), so no carry-less
-    // multiply is needed, and the degrees are carried from step to step
+// the steps of a pair of at most 64-bit words (deg c > deg d) while deg d
+// >= stop, applied to the step matrix rows as in hgcd_word below
+inline W2 hgcd_word64(u64 c, u64 d, int stop, int &steps) {
     u64 m00 = 1, m01 = 0, m10 = 0, m11 = 1;
     steps = 0;
+    int dc = deg64(c), dd = deg64(d);
+    while (dd >= stop && dd >= 0) {
+        u64 r = c, n10 = m00, n11 = m01;
+        int dr = dc;
+        while (dr >= dd) {
+            const int k = dr - dd;
+            r ^= d << k;
+            n10 ^= m10 << k;
+            n11 ^= m11 << k;
+            dr = deg64(r);
+        }
+        m00 = m10;
+        m01 = m11;
+        m10 = n10;
+        m11 = n11;
+        c = d;
+        dc = dd;
+        d = r;
+        dd = dr;
+        steps++;
+    }
+    return W2{m00, m01, m10, m11};
+}
+
+// low 128 bits of the carry-less product of a word and a 128-bit value
+inline u128 clmul_w128(u64 w, u128 v) {
+    u64 l0, h0, l1, h1;
+    clmul(w, (u64)v, l0, h0);
+    clmul(w, (u64)(v >> 64), l1, h1);
+    (void)h1;
+    return ((u128)(h0 ^ l1) << 64) | l0;
+}
+// the product of two step matrices A B (entries stay single words: the
+// product is itself a step matrix of at most 63 degrees)
+inline W2 w2_mul(const W2 &A, const W2 &B) {
+    return W2{clmul_lo(A.m00, B.m00) ^ clmul_lo(A.m01, B.m10),
+              clmul_lo(A.m00, B.m01) ^ clmul_lo(A.m01, B.m11),
+              clmul_lo(A.m10, B.m00) ^ clmul_lo(A.m11, B.m10),
+              clmul_lo(A.m10, B.m01) ^ clmul_lo(A.m11, B.m11)};
+}
+
+// Remainder steps of a pair of at most 128-bit polynomials (deg c > deg d)
+// while deg d >= stop; returns the step matrix.  Each entry has degree
+// deg(c0) - deg(current c) <= 127 - stop, and each quotient at most
+// deg(c) - stop, so both fit in one word when stop >= 64 (or deg c0 <=
+// 63 + stop).  The steps run on the top 64 bits of the pair where they
+// can (a sub-round takes the steps of those bits while their remainders
+// keep half the 64-bit window, the rule of the recursion), and the
+// sub-round's matrix is applied to the 128-bit pair by carry-less
+// multiplies; the steps left run on 128 bits, the quotient applied to the
+// matrix rows one set bit at a time.
+W2 hgcd_word(u128 c, u128 d, int stop, int &steps) {
+    W2 W{1, 0, 0, 1};
+    steps = 0;
     int dc = deg128(c), dd = deg128(d);
+    for (;;) {
+        if (dd < stop || dd < 0) return W;
+        if (dc < 64 + 32) break;   // too few steps left for a 64-bit sub-round
+        const int sh = dc - 63;
+        const int st64 = std::max(32, stop - sh);
+        int n = 0;
+        const W2 S = hgcd_word64((u64)(c >> sh), (u64)(d >> sh), st64, n);
+        if (n == 0) break;
+        const u128 nc = clmul_w128(S.m00, c) ^ clmul_w128(S.m01, d);
+        const u128 nd = clmul_w128(S.m10, c) ^ clmul_w128(S.m11, d);
+        c = nc;
+        d = nd;
+        W = w2_mul(S, W);
+        steps += n;
+        dc = deg128(c);
+        dd = deg128(d);
+    }
+    u64 m00 = W.m00, m01 = W.m01, m10 = W.m10, m11 = W.m11;
     while (dd >= stop && dd >= 0) {
         u128 r = c;
         u64 n10 = m00, n11 = m01;
