@@ -135,8 +135,10 @@ std::future<void> spawn(F &&f) {
 }
 constexpr size_t PAR_WORDS = 512;   // products with both operands this large may go parallel
 
-constexpr size_t KBASE = 48;           // words: schoolbook below this
-constexpr int64_t HGCD_BASE = 1536;    // bits: step-by-step half-gcd below this
+// words: Karatsuba's schoolbook base (gf2t_tune "gcd_school_words")
+std::atomic<size_t> g_kbase{48};
+// bits: the word-window half-gcd below this (gf2t_tune "gcd_base_bits")
+std::atomic<int64_t> g_hgcd_base{1536};
 constexpr int64_t GCD_BASE = 4096;     // bits: plain Euclid below this
 
 inline void clmul(u64 a, u64 b, u64 &lo, u64 &hi) {
@@ -292,7 +294,7 @@ void kara_mix(u64 *out, const u64 *z1, size_t h, size_t l) {
 
 // out[0, 2n) = a[0,n) * b[0,n); ws holds at least 4n words per level
 void kara(const u64 *a, const u64 *b, size_t n, u64 *out, u64 *ws) {
-    if (n <= KBASE) {
+    if (n <= g_kbase.load(std::memory_order_relaxed)) {
         std::memset(out, 0, 2 * n * 8);
         school(a, n, b, n, out);
         return;
@@ -340,7 +342,7 @@ Poly host_mul(const u64 *a, size_t na, const u64 *b, size_t nb) {
         std::swap(na, nb);
     }
     Poly out(na + nb + 1, 0);
-    if (nb <= KBASE) {
+    if (nb <= g_kbase.load(std::memory_order_relaxed)) {
         school(a, na, b, nb, out.data());
         trim(out);
         return out;
@@ -657,7 +659,7 @@ void lin2(u64 w0, const Poly &x, u64 w1, const Poly &y, Poly &out) {
     trim(out);
 }
 
-// half-gcd for small inputs (deg a < HGCD_BASE): rounds of half-gcds of
+// half-gcd for small inputs (deg a < the "gcd_base_bits" knob): rounds of half-gcds of
 // the top 128 coefficients.  A window's quotients are those of the full
 // pair while the window remainders keep at least half the window's degree
 // (the same rule as the recursion), so each round applies a correct prefix
@@ -755,6 +757,19 @@ bool set_tuned_block_m(int k, int m) {
     return true;
 }
 
+bool set_gcd_base_bits(int64_t bits) {
+    if (bits < 128 || bits > 65536) return false;
+    g_hgcd_base.store(bits);
+    return true;
+}
+int64_t gcd_base_bits() { return g_hgcd_base.load(); }
+bool set_school_words(int64_t words) {
+    if (words < 8 || words > 512) return false;
+    g_kbase.store((size_t)words);
+    return true;
+}
+int64_t school_words() { return (int64_t)g_kbase.load(); }
+
 void set_device_product_bits(u64 bits) { g_dev_bits.store(bits); }
 u64 device_product_bits() { return g_dev_bits.load(); }
 
@@ -785,7 +800,7 @@ void hgcd(const Poly &a, const Poly &b, HMat &M, Poly &c, Poly &d, bool want_M) 
         trim(d);
         return;
     }
-    if (n < HGCD_BASE) {
+    if (n < g_hgcd_base.load(std::memory_order_relaxed)) {
         PhaseTimer pt(t_stats.base_calls, t_stats.base_ns);
         hgcd_base(a, b, m, M, c, d);
         return;

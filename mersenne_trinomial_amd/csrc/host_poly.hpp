@@ -69,6 +69,14 @@ void xor_shift_words(u64 *a, const u64 *b, size_t bw, u64 sh);
 // device product service; 0 keeps every product on the host.
 void set_device_product_bits(u64 bits);
 u64 device_product_bits();
+// The half-gcd's base-case size in bits (128..65536; below it the
+// word-window rounds run) and Karatsuba's schoolbook base in words
+// (8..512): gf2t_tune "gcd_base_bits" and "gcd_school_words"; false for
+// a value out of range
+bool set_gcd_base_bits(int64_t bits);
+int64_t gcd_base_bits();
+bool set_school_words(int64_t words);
+int64_t school_words();
 // whether chains take two squarings per launch ("sq_fuse"; square.hip)
 bool square_fuse_enabled();
 Poly mul_fast(const Poly &a, const Poly &b);       // Karatsuba / device FFT

@@ -499,6 +499,16 @@ extern "C" int gf2t_tune(const char *key, int64_t value) {
         gf2t_host::set_device_product_bits(value > 0 ? (uint64_t)value : 0);
         return GF2T_OK;
     }
+    if (!strcmp(key, "gcd_base_bits")) {
+        if (gf2t_host::set_gcd_base_bits(value)) return GF2T_OK;
+        gf2t::set_error("gf2t_tune: gcd_base_bits needs 128 <= bits <= 65536");
+        return GF2T_EINVAL;
+    }
+    if (!strcmp(key, "gcd_school_words")) {
+        if (gf2t_host::set_school_words(value)) return GF2T_OK;
+        gf2t::set_error("gf2t_tune: gcd_school_words needs 8 <= words <= 512");
+        return GF2T_EINVAL;
+    }
     if (!strcmp(key, "fft_ept")) return gf2t_fft_set_ept(value);
     if (!strcmp(key, "fft_low_bits")) return gf2t_fft_set_low_bits(value);
     if (!strcmp(key, "hm_zc_words")) return gf2t_hm_set_zc_words(value);
@@ -524,6 +534,14 @@ extern "C" int gf2t_tune_get(const char *key, int64_t *value) {
     if (!strcmp(key, "sq_fuse")) { *value = sq_fuse() ? 1 : 0; return GF2T_OK; }
     if (!strcmp(key, "gcd_dev_bits")) {
         *value = (int64_t)gf2t_host::device_product_bits();
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "gcd_base_bits")) {
+        *value = gf2t_host::gcd_base_bits();
+        return GF2T_OK;
+    }
+    if (!strcmp(key, "gcd_school_words")) {
+        *value = gf2t_host::school_words();
         return GF2T_OK;
     }
     if (!strcmp(key, "fft_ept")) {

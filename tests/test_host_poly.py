@@ -72,6 +72,47 @@ def test_half_gcd_vs_euclid(lib, seed):
         assert np.array_equal(out[:n], g2[:n]) and not np.any(out[n:])
 
 
+@pytest.mark.parametrize("base,school", [(128, 8), (1024, 48), (4096, 96), (16384, 512)])
+def test_half_gcd_every_base_size(lib, base, school):
+    """The gcd is the same whatever base case and schoolbook sizes the
+    half-gcd uses (gf2t_tune "gcd_base_bits" from 128 to 16384 bits,
+    "gcd_school_words" from 8 to 512 words): planted common factors and
+    coprime pairs across the base case, and a trinomial gcd at r = 132049."""
+    old_b, old_s = C.c_int64(), C.c_int64()
+    assert lib.gf2t_tune_get(b"gcd_base_bits", C.byref(old_b)) == 0
+    assert lib.gf2t_tune_get(b"gcd_school_words", C.byref(old_s)) == 0
+    rng = np.random.default_rng(base + school)
+    try:
+        assert lib.gf2t_tune(b"gcd_base_bits", base) == 0
+        assert lib.gf2t_tune(b"gcd_school_words", school) == 0
+        for trial in range(12):
+            gb = int(rng.integers(1, 3000))
+            ab, bb = int(rng.integers(1, 40000)), int(rng.integers(1, 40000))
+            g = rand_poly(rng, gb)
+            a = O.mul_comb(g, gb, rand_poly(rng, ab), ab)
+            b = O.mul_comb(g, gb, rand_poly(rng, bb), bb) if trial % 3 else rand_poly(rng, bb)
+            cap = max(len(a), len(b)) + 1
+            out = np.zeros(cap, dtype=np.uint64)
+            d = lib.gf2t_gcd_host(ptr(a), len(a), ptr(b), len(b), ptr(out), cap)
+            d2, g2 = O.gcd(a, b)
+            n = O.words(d2 + 1)
+            assert d == d2, trial
+            assert np.array_equal(out[:n], g2[:n]) and not np.any(out[n:])
+        r, s = 132049, 1201
+        t = O.trinomial(r, s)
+        a = rand_poly(rng, r - 5)
+        out = np.zeros(len(t) + 1, dtype=np.uint64)
+        assert lib.gf2t_gcd_host(ptr(a), len(a), ptr(t), len(t), ptr(out), len(out)) == \
+            O.gcd(a, t)[0]
+    finally:
+        assert lib.gf2t_tune(b"gcd_base_bits", old_b.value) == 0
+        assert lib.gf2t_tune(b"gcd_school_words", old_s.value) == 0
+    for bad in (127, 65537):
+        assert lib.gf2t_tune(b"gcd_base_bits", bad) != 0
+    for bad in (7, 513):
+        assert lib.gf2t_tune(b"gcd_school_words", bad) != 0
+
+
 def test_half_gcd_trinomial_planted_factor(lib):
     # T = x^756839 + x^s + 1 against (residue with a planted small factor
     # of T): the gcd is that factor
