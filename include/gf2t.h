@@ -130,7 +130,7 @@ int gf2t_square_fold_batch(uint64_t *const *dres, uint64_t *const *dtmp,
  *              least this many bits; 0 = host Karatsuba only
  *   "gcd_base_bits" the host half-gcd runs its base case (rounds on the
  *              top 128 coefficients) below this size in bits (128..65536;
- *              default 1536)
+ *              default 3072)
  *   "gcd_school_words" the host Karatsuba's schoolbook base in words
  *              (8..512; default 48)
  *   "fft_ept"  elements of a transform tile per thread (2, 4, 8, 16;

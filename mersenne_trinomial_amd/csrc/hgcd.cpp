@@ -138,7 +138,7 @@ constexpr size_t PAR_WORDS = 512;   // products with both operands this large ma
 // words: Karatsuba's schoolbook base (gf2t_tune "gcd_school_words")
 std::atomic<size_t> g_kbase{48};
 // bits: the word-window half-gcd below this (gf2t_tune "gcd_base_bits")
-std::atomic<int64_t> g_hgcd_base{1536};
+std::atomic<int64_t> g_hgcd_base{3072};
 constexpr int64_t GCD_BASE = 4096;     // bits: plain Euclid below this
 
 inline void clmul(u64 a, u64 b, u64 &lo, u64 &hi) {
