@@ -633,10 +633,43 @@ W2 hgcd_word(u128 c, u128 d, int stop, int &steps) {
     return W2{m00, m01, m10, m11};
 }
 
+// lin2 with 512-bit VPCLMULQDQ, eight words per step: the even words'
+// 128-bit products land on output words (i+2l, i+2l+1) and the odd
+// words' one word higher, the odd accumulator shifted up by a word with
+// its top word carried into the next step (as in rows512).  Words past
+// x's or y's end load as zero; out gets ceil(n/8)*8 + 1 words.
+__attribute__((target("avx512f,vpclmulqdq")))
+void lin2_512(u64 w0, const u64 *x, size_t nx, u64 w1, const u64 *y, size_t ny, u64 *out,
+              size_t n) {
+    const __m512i W0 = _mm512_set1_epi64((long long)w0), W1 = _mm512_set1_epi64((long long)w1);
+    auto mask = [](size_t len, size_t i) -> __mmask8 {
+        return i >= len ? 0 : len - i >= 8 ? (__mmask8)0xFF : (__mmask8)((1u << (len - i)) - 1);
+    };
+    __m512i prevO = _mm512_setzero_si512();
+    size_t i = 0;
+    for (; i < n; i += 8) {
+        const __m512i X = _mm512_maskz_loadu_epi64(mask(nx, i), x + i);
+        const __m512i Y = _mm512_maskz_loadu_epi64(mask(ny, i), y + i);
+        const __m512i E = _mm512_xor_si512(_mm512_clmulepi64_epi128(X, W0, 0x00),
+                                           _mm512_clmulepi64_epi128(Y, W1, 0x00));
+        const __m512i O = _mm512_xor_si512(_mm512_clmulepi64_epi128(X, W0, 0x01),
+                                           _mm512_clmulepi64_epi128(Y, W1, 0x01));
+        _mm512_storeu_si512(out + i, _mm512_xor_si512(E, _mm512_alignr_epi64(O, prevO, 7)));
+        prevO = O;
+    }
+    out[i] = (u64)_mm256_extract_epi64(_mm512_extracti64x4_epi64(prevO, 1), 3);
+}
+
 // out = w0 x + w1 y (word polynomials times multiword ones), one pass:
 // word i is lo(w0 x[i] + w1 y[i]) plus the previous word's high half
 void lin2(u64 w0, const Poly &x, u64 w1, const Poly &y, Poly &out) {
     const size_t nx = x.size(), ny = y.size(), n = std::max(nx, ny);
+    if (n >= 8 && use512()) {
+        out.resize(((n + 7) & ~(size_t)7) + 1);
+        lin2_512(w0, x.data(), nx, w1, y.data(), ny, out.data(), n);
+        trim(out);
+        return;
+    }
     out.resize(n + 1);
     const __m128i W = _mm_set_epi64x((long long)w1, (long long)w0);
     u64 carry = 0;
