@@ -17,6 +17,7 @@ its block size the same way, src/tsfactor.cu:1986-1989 and auto_m
 - gcd_dev_bits: the product size from which the half-gcd uses the device,
   timed on concurrent full-size gcds (the deep scan's interval gcds at
   r = 136279841).
+- gcd_base_bits: the half-gcd's base-case size, on the same concurrent gcds.
 - block_m_<k>: the deep scan's degrees per interval product for
   2^k <= r < 2^(k+1), from the measured per-degree cost
   m * t_sq + t_mul / m of the blocked scheme (t_sq and t_mul: a
@@ -186,11 +187,13 @@ def tune_block_m(r: int, reps: int = 3):
             "chosen": best, "ms_per_degree": {m: round(v, 4) for m, v in cost.items()}}
 
 
-def tune_gcd(r: int, conc: int, thresholds):
-    """gcd_dev_bits: the size from which the half-gcd's products go to the
-    device, timed on `conc` concurrent gcds of random r-bit residues
-    against x^r + x^3 + 1 (the deep scan runs its interval gcds
-    concurrently)."""
+def tune_gcd(r: int, conc: int, thresholds, key: str = "gcd_dev_bits"):
+    """gcd_dev_bits (or another gcd knob named by `key`, such as
+    gcd_base_bits): the value among `thresholds` that runs `conc`
+    concurrent gcds of random r-bit residues against x^r + x^3 + 1
+    fastest (the deep scan runs its interval gcds concurrently).
+    gcd_dev_bits is the size from which the half-gcd's products go to the
+    device; gcd_base_bits the size below which its base case runs."""
     import threading
     nw = (r + 63) // 64
     t = np.zeros((r + 64) // 64, dtype=np.uint64)
@@ -217,21 +220,22 @@ def tune_gcd(r: int, conc: int, thresholds):
             th.join()
         return time.perf_counter() - t0
 
-    default = tune_get("gcd_dev_bits")
+    default = tune_get(key)
     times = {}
     for b in thresholds:
-        tune_set("gcd_dev_bits", b)
+        tune_set(key, b)
         run_all()                                   # plans and buffers for this size mix
         times[b] = run_all()
     best = min(times, key=times.get)
-    tune_set("gcd_dev_bits", best)
-    print(f"gcd_dev_bits: {best} ({times[best]:.3f} s for {conc} gcds at r={r}; default "
+    tune_set(key, best)
+    print(f"{key}: {best} ({times[best]:.3f} s for {conc} gcds at r={r}; default "
           f"{default} {times.get(default, float('nan')):.3f} s)", file=sys.stderr, flush=True)
-    return {"r": r, "concurrent": conc, "default": default, "chosen": best,
+    return {"key": key, "r": r, "concurrent": conc, "default": default, "chosen": best,
             "s": {b: round(v, 3) for b, v in times.items()}}
 
 
-def write(path: str, fft: dict, sq: dict, blocks: list, device: str, gcd: dict | None) -> None:
+def write(path: str, fft: dict, sq: dict, blocks: list, device: str, gcd: dict | None,
+          gcd_base: dict | None = None) -> None:
     lines = [f"# gf2t autotuner ({device.strip()}), {time.strftime('%Y-%m-%d %H:%M:%S')}",
              "# key value -- applied by gf2t_init; see mersenne_trinomial_amd/autotune.py"]
     for m, v in sorted(fft.items()):
@@ -242,6 +246,8 @@ def write(path: str, fft: dict, sq: dict, blocks: list, device: str, gcd: dict |
         lines.append(f"block_m_{b['k']} {b['chosen']}")
     if gcd:
         lines.append(f"gcd_dev_bits {gcd['chosen']}")
+    if gcd_base:
+        lines.append(f"gcd_base_bits {gcd_base['chosen']}")
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -271,10 +277,14 @@ def main(argv=None):
     blocks = [tune_block_m(int(r)) for r in args.block_r.split(",") if r]
     gcd = tune_gcd(args.gcd_r, args.gcd_conc,
                    [1 << k for k in range(15, 19)]) if args.gcd_conc > 0 else None
-    write(args.out, fft, sq, blocks, dev, gcd)
+    # the base-case size, timed with the chosen device threshold
+    gcd_base = tune_gcd(args.gcd_r, args.gcd_conc, [1536, 2048, 3072, 4096],
+                        key="gcd_base_bits") if args.gcd_conc > 0 else None
+    write(args.out, fft, sq, blocks, dev, gcd, gcd_base)
     print(json.dumps({"mode": "autotune", "device": dev.strip(), "file": args.out,
                       "seconds": round(time.perf_counter() - t0, 1), "fft_tile": fft,
-                      "squaring": sq, "block_m": blocks, "gcd": gcd}), flush=True)
+                      "squaring": sq, "block_m": blocks, "gcd": gcd, "gcd_base": gcd_base}),
+          flush=True)
 
 
 if __name__ == "__main__":
